@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the force path on one MI355X.
+
+Workload (the GPU configuration): N = 128^3 = 2,097,152 particles on a
+lattice displaced by Gaussian noise of 0.2 cell (seed 1), Plummer softening
+eps = 1/(50*128), GM = 1/N.  One step is one all-pairs FP32 compensated kick
+(N^2 = 4.398e12 interactions, with the half-kick/rung epilogue) over inputs
+already resident in HBM.  The same run times the 3-D complex FP32 FFT
+(256^3 round trip of that configuration and a 512^3 forward transform) and
+the drift.
+
+Prints one JSON line.  value = interactions/s of the kick.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "kick interactions/s + 3D FFT GB/s; step time PM+MLP vs direct at ≤1e-3 force err"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_INTERACTION = 20
+
+
+class HipEvents:
+    """hipEvent timing on the library's stream (where every kernel runs)."""
+
+    def __init__(self, stream: int):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.stream = C.c_void_p(stream)
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e):
+        assert self.hip.hipEventRecord(e, self.stream) == 0
+
+    def elapsed_ms(self, a, b) -> float:
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+    def device_sync(self):
+        assert self.hip.hipDeviceSynchronize() == 0
+
+
+def time_calls(ev: HipEvents, fn, reps: int) -> list[float]:
+    out = []
+    for _ in range(reps):
+        a, b = ev.event(), ev.event()
+        ev.record(a)
+        fn()
+        ev.record(b)
+        out.append(ev.elapsed_ms(a, b))
+    return out
+
+
+def cpu_baseline(x, y, z, soft, gm, target_s: float):
+    """FP64 oracle kick timed on the host cores on a bounded sample of sinks
+    (every source), scaled to interactions/s."""
+    from oracle import oracle as O
+    n = x.size
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(7)
+    m = 32
+    O.kick(x, y, z, gm, soft, O.KERNEL_PLUMMER, sinks=rng.choice(n, m, replace=False))  # warm-up
+    t0 = time.perf_counter()
+    O.kick(x, y, z, gm, soft, O.KERNEL_PLUMMER, sinks=rng.choice(n, m, replace=False))
+    dt = time.perf_counter() - t0
+    m = int(min(n, max(m, m * target_s / max(dt, 1e-6))))
+    sinks = rng.choice(n, m, replace=False)
+    t0 = time.perf_counter()
+    O.kick(x, y, z, gm, soft, O.KERNEL_PLUMMER, sinks=sinks)
+    dt = time.perf_counter() - t0
+    return {"value": m * n / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 all-pairs kick of {m} random sinks against all {n} sources "
+                      f"({m * n:.3e} interactions, {dt:.1f} s, OpenMP threads={threads})"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=128, help="lattice side (N = side^3)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--fft-reps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for how "
+              "the kick and the FFT would shard)", file=sys.stderr)
+        return 2
+
+    from ct2_amd import api
+    api.init(0)
+    ev = HipEvents(api.stream())
+
+    side = args.side
+    n = side ** 3
+    soft, gm = 1.0 / (50 * side), 1.0 / n
+    x, y, z = api.ic_lattice(side, 0.2, 1)
+    api.particle_allocate(n)
+    api.particles_upload(x, y, z, np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32),
+                         np.zeros(n, np.int8))
+    api.set_force_options(soft, gm, api.KERNEL_PLUMMER)
+    # top-level kick of kick_step (main.cpp:50-68): ascending, descending, top
+    params = api.kick_params(ascending=True, descending=True, top=True, first=False, rung=0, t0=1e-3,
+                             eta=0.15, max_dt=1e-3, scale=1.0)
+    print(f"[bench] kick N={n} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        api.kick(params)
+    api.synchronize()
+    ev.device_sync()
+    t0 = time.perf_counter()
+    kick_ms = time_calls(ev, lambda: api.kick(params), args.steps)
+    api.synchronize()
+    ev.device_sync()
+    wall = time.perf_counter() - t0
+    inter = float(n) * n
+    kick_avg = sum(kick_ms) / len(kick_ms)
+    value = inter * args.steps / wall
+    achieved_tflops = FLOP_PER_INTERACTION * inter / (kick_avg * 1e-3) / 1e12
+    print(f"[bench] kick {kick_avg:.1f} ms/step, {inter / kick_avg * 1e3:.3e} int/s", file=sys.stderr, flush=True)
+
+    # drift: 36 B/particle
+    drift_ms = time_calls(ev, lambda: api.drift(1e-9, 1.0), 20)
+    drift_avg = float(np.median(drift_ms))
+
+    # FFT: 256^3 round trip (96 n^3 B) and 512^3 forward (48 n^3 B), FP32
+    fft = {}
+    for nf, mode in ((256, "round_trip"), (512, "forward")):
+        buf = api.DeviceBuffer(nf ** 3 * 8)
+        rng = np.random.default_rng(nf)
+        buf.upload((rng.standard_normal(2 * nf ** 3, dtype=np.float32)).view(np.complex64))
+        if mode == "round_trip":
+            def op():
+                api.fft3d_device(buf, nf, 1)
+                api.fft3d_device(buf, nf, -1)
+            alg = 96 * nf ** 3
+        else:
+            def op():
+                api.fft3d_device(buf, nf, 1)
+            alg = 48 * nf ** 3
+        op()
+        ms = float(np.median(time_calls(ev, op, args.fft_reps)))
+        fft[f"{nf}^3_{mode}"] = {"ms": round(ms, 4), "GB/s": round(alg / ms / 1e6, 1),
+                                 "frac_hbm": round(alg / ms / 1e6 / HBM_PEAK_GBS, 4),
+                                 "compulsory_GB/s": round(16 * nf ** 3 * (2 if mode == "round_trip" else 1) / ms / 1e6, 1)}
+        buf.close()
+        print(f"[bench] fft {nf}^3 {mode}: {ms:.3f} ms", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"all-pairs direct-sum kick, N={side}^3 lattice+Gaussian(0.2 cell), "
+                               f"Plummer eps=1/(50*{side}), FP32 compensated, with half-kick/rung epilogue",
+                   "n_particles": n, "interactions_per_step": inter},
+        "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "kick_direct (+kick_finalize)", "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "avg_launch_ms": round(kick_avg, 3)},
+        "fft_fp32": fft,
+        "drift": {"ms": round(drift_avg, 4), "GB/s": round(36 * n / drift_avg / 1e6, 1),
+                  "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(x, y, z, soft, gm, args.cpu_seconds)
+    api.finalize()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

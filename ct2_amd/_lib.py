@@ -1,0 +1,108 @@
+"""ctypes declarations of libct2hip.so (the C ABI in include/ct2hip.h).
+
+The library is built in-tree by ``ct2_amd/build.py``.  Loading fails loudly
+when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libct2hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "ct2hip.h")
+
+OK = 0
+ERR_INVALID = 1
+ERR_STATE = 2
+ERR_NOMEM = 3
+ERR_HIP = 4
+ERR_NODEVICE = 5
+
+KERNEL_PLUMMER = 0
+KERNEL_CT2 = 1
+KICK_NAIVE = 0x1
+KICK_STORE = 0x2
+
+
+class Ct2Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ct2 error {code}: {msg}")
+        self.code = code
+
+
+class ForceOptions(C.Structure):
+    _fields_ = [("hsoft", C.c_float), ("GM", C.c_float), ("kernel", C.c_int)]
+
+
+class KickParams(C.Structure):
+    """kick_params, kick.hpp:10-20."""
+
+    _fields_ = [("ascending", C.c_int), ("descending", C.c_int), ("top", C.c_int),
+                ("first", C.c_int), ("rung", C.c_int), ("t0", C.c_float), ("eta", C.c_float),
+                ("max_dt", C.c_float), ("scale", C.c_float)]
+
+
+class ParticlePtrs(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("vx", C.c_void_p),
+                ("vy", C.c_void_p), ("vz", C.c_void_p), ("rung", C.c_void_p), ("n", C.c_size_t)]
+
+
+_P = C.c_void_p
+_SIGS = {
+    "ct2_last_error": ([], C.c_char_p),
+    "ct2_init": ([C.c_int], C.c_int),
+    "ct2_finalize": ([], C.c_int),
+    "ct2_get_stream": ([C.POINTER(C.c_void_p)], C.c_int),
+    "ct2_synchronize": ([], C.c_int),
+    "ct2_set_force_options": ([C.POINTER(ForceOptions)], C.c_int),
+    "ct2_particles_alloc": ([C.c_size_t], C.c_int),
+    "ct2_particle_count": ([], C.c_size_t),
+    "ct2_particles_upload": ([_P] * 7, C.c_int),
+    "ct2_particles_download": ([_P] * 7, C.c_int),
+    "ct2_particles_device_ptrs": ([C.POINTER(ParticlePtrs)], C.c_int),
+    "ct2_kick": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
+    "ct2_forces_download": ([_P] * 4, C.c_int),
+    "ct2_kick_sample_fp64": ([_P, C.c_size_t, C.c_int, _P, _P, _P, _P], C.c_int),
+    "ct2_drift": ([C.c_float, C.c_float], C.c_int),
+    "ct2_fft3d_c2c": ([_P, C.c_int, C.c_int], C.c_int),
+    "ct2_fft3d_c2c_f64": ([_P, C.c_int, C.c_int], C.c_int),
+    "ct2_dev_alloc": ([C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
+    "ct2_dev_free": ([_P], C.c_int),
+    "ct2_memcpy_h2d": ([_P, _P, C.c_size_t], C.c_int),
+    "ct2_memcpy_d2h": ([_P, _P, C.c_size_t], C.c_int),
+    "ct2_ic_lattice": ([C.c_int, C.c_double, C.c_uint64, _P, _P, _P], C.c_int),
+    "ct2_ic_plummer_mix": ([C.c_size_t, C.c_size_t, C.c_int, C.c_double, C.c_double, C.c_uint64,
+                            _P, _P, _P], C.c_int),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libct2hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`"
+                               " (there is no CPU fallback)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise Ct2Error(rc, load().ct2_last_error().decode(errors="replace"))
+
+
+def declared_symbols() -> list[str]:
+    """Function names declared in include/ct2hip.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ct2_[a-z0-9_]+)\s*\(", text)))

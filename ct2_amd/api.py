@@ -1,0 +1,197 @@
+"""Python face of libct2hip.so, named after the reference's free functions.
+
+The reference keeps its state in process globals (particle.hpp:22-28,
+options.cpp:116) and exposes free functions; this module does the same over
+the C ABI: ``particle_allocate``, ``kick``, ``drift``, ``fft3d`` ...  Arrays
+cross as numpy; every call checks the C status and raises ``Ct2Error``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import Ct2Error, ForceOptions, KickParams  # noqa: F401  (re-exported)
+
+KERNEL_PLUMMER = L.KERNEL_PLUMMER
+KERNEL_CT2 = L.KERNEL_CT2
+KICK_NAIVE = L.KICK_NAIVE
+KICK_STORE = L.KICK_STORE
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous, "arrays must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def init(device: int = 0) -> None:
+    L.check(L.load().ct2_init(device))
+
+
+def finalize() -> None:
+    L.check(L.load().ct2_finalize())
+
+
+def synchronize() -> None:
+    L.check(L.load().ct2_synchronize())
+
+
+def stream() -> int:
+    s = C.c_void_p()
+    L.check(L.load().ct2_get_stream(C.byref(s)))
+    return s.value or 0
+
+
+def set_force_options(hsoft: float, GM: float, kernel: int = KERNEL_PLUMMER) -> None:
+    """options.hsoft / options.GM (options.cpp:155,190) and the pair kernel."""
+    o = ForceOptions(hsoft, GM, kernel)
+    L.check(L.load().ct2_set_force_options(C.byref(o)))
+
+
+# ---------------------------------------------------------------- particles
+def particle_allocate(count: int) -> None:
+    """particle_allocate (particle.cpp:273-288) on the device."""
+    L.check(L.load().ct2_particles_alloc(count))
+
+
+def particle_count() -> int:
+    return int(L.load().ct2_particle_count())
+
+
+def particles_upload(x=None, y=None, z=None, vx=None, vy=None, vz=None, rung=None) -> None:
+    n = particle_count()
+    arrs = []
+    for a, dt in ((x, np.uint32), (y, np.uint32), (z, np.uint32), (vx, np.float32),
+                  (vy, np.float32), (vz, np.float32), (rung, np.int8)):
+        if a is None:
+            arrs.append(None)
+            continue
+        a = np.ascontiguousarray(a, dtype=dt)
+        if a.size != n:
+            raise ValueError(f"array of {a.size} elements for {n} particles")
+        arrs.append(a)
+    L.check(L.load().ct2_particles_upload(*[_p(a) for a in arrs]))
+
+
+@dataclass
+class Particles:
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    rung: np.ndarray
+
+
+def particles_download() -> Particles:
+    n = particle_count()
+    out = [np.empty(n, np.uint32) for _ in range(3)] + [np.empty(n, np.float32) for _ in range(3)]
+    out.append(np.empty(n, np.int8))
+    L.check(L.load().ct2_particles_download(*[_p(a) for a in out]))
+    return Particles(*out)
+
+
+# ---------------------------------------------------------------- kick / drift
+def kick_params(ascending=False, descending=False, top=False, first=False, rung=0, t0=1.0, eta=0.15,
+                max_dt=1.0, scale=1.0) -> KickParams:
+    return KickParams(int(ascending), int(descending), int(top), int(first), rung, t0, eta, max_dt, scale)
+
+
+def kick(params: KickParams | None = None, flags: int = 0) -> int:
+    """int kick(const kick_params&) (kick.cpp:105-125); returns the max rung.
+    With ``params`` None the kick only evaluates forces."""
+    if params is None:
+        params = kick_params()
+    mr = C.c_int(0)
+    L.check(L.load().ct2_kick(C.byref(params), flags, C.byref(mr)))
+    return mr.value
+
+
+def forces_download():
+    n = particle_count()
+    out = [np.empty(n, np.float32) for _ in range(4)]
+    L.check(L.load().ct2_forces_download(*[_p(a) for a in out]))
+    return tuple(out)
+
+
+def kick_sample_fp64(idx, periodic: bool = False):
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = [np.empty(idx.size, np.float64) for _ in range(4)]
+    L.check(L.load().ct2_kick_sample_fp64(_p(idx), idx.size, int(periodic), *[_p(a) for a in out]))
+    return tuple(out)
+
+
+def drift(dt: float, scale: float) -> None:
+    """void drift(float dt, float scale) (drift.cpp:4-18)."""
+    L.check(L.load().ct2_drift(dt, scale))
+
+
+# ---------------------------------------------------------------- device buffers / FFT
+class DeviceBuffer:
+    """A device allocation made through the library (freed on close/del)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        L.check(L.load().ct2_dev_alloc(nbytes, C.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        L.check(L.load().ct2_memcpy_h2d(self.ptr, _p(a), a.nbytes))
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        assert a.flags.c_contiguous and a.nbytes <= self.nbytes
+        L.check(L.load().ct2_memcpy_d2h(_p(a), self.ptr, a.nbytes))
+        return a
+
+    def close(self) -> None:
+        if self.ptr:
+            L.check(L.load().ct2_dev_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                L.load().ct2_dev_free(self.ptr)
+        except Exception:
+            pass
+
+
+def fft3d_device(buf: DeviceBuffer, n: int, dir: int = 1, f64: bool = False) -> None:
+    fn = L.load().ct2_fft3d_c2c_f64 if f64 else L.load().ct2_fft3d_c2c
+    L.check(fn(buf.ptr, n, dir))
+
+
+def fft3d(a: np.ndarray, dir: int = 1) -> np.ndarray:
+    """3-D complex FFT of a host cube through the device (complex64 -> FP32
+    kernels, complex128 -> FP64 kernels); unnormalised, FFTW sign."""
+    n = a.shape[0]
+    assert a.shape == (n, n, n) and a.dtype in (np.complex64, np.complex128)
+    buf = DeviceBuffer(a.nbytes)
+    try:
+        buf.upload(a)
+        fft3d_device(buf, n, dir, a.dtype == np.complex128)
+        return buf.download(np.empty_like(a))
+    finally:
+        buf.close()
+
+
+# ---------------------------------------------------------------- initial conditions
+def ic_lattice(n: int, sigma_cells: float, seed: int):
+    N = n ** 3
+    out = [np.empty(N, np.uint32) for _ in range(3)]
+    L.check(L.load().ct2_ic_lattice(n, sigma_cells, seed, *[_p(a) for a in out]))
+    return tuple(out)
+
+
+def ic_plummer_mix(n: int, n_sph: int, nspheres: int, a: float, rmax: float, seed: int):
+    out = [np.empty(n, np.uint32) for _ in range(3)]
+    L.check(L.load().ct2_ic_plummer_mix(n, n_sph, nspheres, a, rmax, seed, *[_p(x) for x in out]))
+    return tuple(out)

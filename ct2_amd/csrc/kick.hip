@@ -1,0 +1,407 @@
+// kick.hip -- the direct-sum kick of kick.cpp:3-125 on gfx950.
+//
+// kick_direct: one sink per lane; sources stream through LDS in tiles of
+// KICK_TILE (coalesced SoA loads, double-buffered, one barrier per tile) and
+// are read back as 16-byte LDS broadcasts.  Separations are formed in the
+// fixed32 integer domain (exact minimum image) and converted to FP32 in
+// units of 2^-32, so no per-pair scale multiply is needed.  Each tile is
+// summed in plain FP32 and folded into the running total with TwoSum
+// (compensated accumulation); CT2_KICK_NAIVE sums straight into the total.
+// Partial sums of a source chunk go to HBM as FP64 so that the chunked
+// (split) launch used for small N stays deterministic.
+//
+// kick_finalize: adds the chunks in order, scales by GM, stores the force
+// and runs the half-kick/rung epilogue of kick.cpp:75-99.
+//
+// kick_sample_fp64: the FP64 judge; one workgroup per sampled sink, sources
+// strided over the lanes, reduced with wavefront shuffles then LDS.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace ct2 {
+namespace {
+
+constexpr int KICK_BLOCK = 256;  // sinks per workgroup
+constexpr int KICK_TILE = 256;   // sources per LDS tile (== KICK_BLOCK: one load per lane)
+
+struct KickConst {
+    float eps2u;   // Plummer: (h 2^32)^2
+    float h2u;     // ct2 kernel: (h 2^32)^2
+    float hinvu;   //             1/(h 2^32)
+    float hinv3u;  //             1/(h 2^32)^3
+};
+
+// Epilogue constants, kick.cpp:75-99, precomputed on the host in FP32.
+struct EpiConst {
+    int kick_ascending;  // ascending && !first
+    int descending;
+    float sgn;     // top ? +1 : -1
+    float dtk;     // 0.5 t0 / 2^rung
+    float eta;
+    float scale_h;  // scale * hsoft
+    float t0;
+    float max_dt;
+    float two_r;  // 2^rung
+    int prung;
+};
+
+__device__ __forceinline__ void two_sum(float& s, float& c, float v) {
+#pragma clang fp contract(off)
+    const float t = s + v;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (v - bp);
+    s = t;
+    c += e;
+}
+
+template <int KERNEL>
+__device__ __forceinline__ void interact(const uint4 src, uint32_t xi, uint32_t yi, uint32_t zi,
+                                         const KickConst& kc, float& tx, float& ty, float& tz,
+                                         float& tp) {
+    const float dx = (float)(int32_t)(src.x - xi);
+    const float dy = (float)(int32_t)(src.y - yi);
+    const float dz = (float)(int32_t)(src.z - zi);
+    if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, kc.eps2u)));
+        const float w = __builtin_amdgcn_rsqf(r2);
+        const float w3 = (w * w) * w;
+        tx = fmaf(dx, w3, tx);
+        ty = fmaf(dy, w3, ty);
+        tz = fmaf(dz, w3, tz);
+        tp += w;
+    } else {
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const bool near = r2 < kc.h2u;
+        const float w = __builtin_amdgcn_rsqf(near ? kc.h2u : r2);
+        const float k = near ? kc.hinv3u : (w * w) * w;
+        const float p = near ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : w;
+        tx = fmaf(dx, k, tx);
+        ty = fmaf(dy, k, ty);
+        tz = fmaf(dz, k, tz);
+        tp += p;
+    }
+}
+
+// grid: (ceil(nsink/KICK_BLOCK), nsplit); block y sums sources
+// [y*jchunk, min(n, (y+1)*jchunk)).  Output: partial[(a*nsplit + y)*nsink + i]
+// for a in {x,y,z,pot}, in units of 2^-64 (force) and 2^-32... see scaling
+// in kick_finalize.
+template <int KERNEL, bool NAIVE>
+__global__ __launch_bounds__(KICK_BLOCK) void kick_direct(const uint32_t* __restrict__ X,
+                                                          const uint32_t* __restrict__ Y,
+                                                          const uint32_t* __restrict__ Z, int n,
+                                                          int jchunk, KickConst kc,
+                                                          double* __restrict__ partial,
+                                                          int* __restrict__ maxrung) {
+    __shared__ uint4 tile[2][KICK_TILE];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * KICK_BLOCK + tid;
+    const int ic = i < n ? i : n - 1;
+    const uint32_t xi = X[ic], yi = Y[ic], zi = Z[ic];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) *maxrung = 0;
+
+    const int j0 = blockIdx.y * jchunk;
+    const int j1 = min(n, j0 + jchunk);
+    float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
+    float cx = 0.f, cy = 0.f, cz = 0.f, cp = 0.f;
+
+    uint4 pre = make_uint4(0u, 0u, 0u, 0u);
+    if (j0 + tid < j1) pre = make_uint4(X[j0 + tid], Y[j0 + tid], Z[j0 + tid], 0u);
+    int buf = 0;
+    for (int jt = j0; jt < j1; jt += KICK_TILE) {
+        tile[buf][tid] = pre;
+        __syncthreads();
+        const int jn = jt + KICK_TILE + tid;
+        if (jn < j1) pre = make_uint4(X[jn], Y[jn], Z[jn], 0u);
+        const int cnt = min(KICK_TILE, j1 - jt);
+        float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
+        if (NAIVE) {
+            tx = sx;
+            ty = sy;
+            tz = sz;
+            tp = sp;
+        }
+        const uint4* T = tile[buf];
+        if (cnt == KICK_TILE) {
+#pragma unroll 16
+            for (int j = 0; j < KICK_TILE; ++j) interact<KERNEL>(T[j], xi, yi, zi, kc, tx, ty, tz, tp);
+        } else {
+            for (int j = 0; j < cnt; ++j) interact<KERNEL>(T[j], xi, yi, zi, kc, tx, ty, tz, tp);
+        }
+        if (NAIVE) {
+            sx = tx;
+            sy = ty;
+            sz = tz;
+            sp = tp;
+        } else {
+            two_sum(sx, cx, tx);
+            two_sum(sy, cy, ty);
+            two_sum(sz, cz, tz);
+            two_sum(sp, cp, tp);
+        }
+        buf ^= 1;
+    }
+    if (i < n) {
+        const long ns = gridDim.y, y = blockIdx.y;
+        partial[(0 * ns + y) * n + i] = (double)sx + (double)cx;
+        partial[(1 * ns + y) * n + i] = (double)sy + (double)cy;
+        partial[(2 * ns + y) * n + i] = (double)sz + (double)cz;
+        partial[(3 * ns + y) * n + i] = (double)sp + (double)cp;
+    }
+}
+
+// kick.cpp:75-99 for one sink, in the FP32 operation order of the
+// reference (no contraction), so that it matches the CPU restatement bit
+// for bit given the same force.
+__device__ __forceinline__ void epilogue(float f0, float f1, float f2, float& vx, float& vy,
+                                         float& vz, int8_t& rung, const EpiConst& e, int& rmax) {
+#pragma clang fp contract(off)
+    if (e.kick_ascending) {
+        vx = vx + e.sgn * f0 * e.dtk;
+        vy = vy + e.sgn * f1 * e.dtk;
+        vz = vz + e.sgn * f2 * e.dtk;
+    }
+    if (e.descending) {
+        const float g2 = f0 * f0 + f1 * f1 + f2 * f2;
+        float dt = fminf(e.eta * sqrtf(e.scale_h / sqrtf(g2)), e.t0);
+        dt = fminf(e.max_dt, dt);
+        const int up = (e.t0 / dt) > e.two_r;
+        int r = rung;
+        r = max(e.prung + up, r - 1);
+        rung = (int8_t)r;
+        rmax = max(rmax, r);
+        vx = vx + e.sgn * f0 * e.dtk;
+        vy = vy + e.sgn * f1 * e.dtk;
+        vz = vz + e.sgn * f2 * e.dtk;
+    }
+}
+
+__global__ __launch_bounds__(256) void kick_finalize(const double* __restrict__ partial, int nsplit,
+                                                     int n, double fscale, double pscale, EpiConst e,
+                                                     float* __restrict__ vx, float* __restrict__ vy,
+                                                     float* __restrict__ vz, int8_t* __restrict__ rung,
+                                                     float* __restrict__ fx, float* __restrict__ fy,
+                                                     float* __restrict__ fz, float* __restrict__ pot,
+                                                     int* __restrict__ maxrung) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int rmax = 0;
+    if (i < n) {
+        double a[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double s = 0.0;
+            for (int y = 0; y < nsplit; ++y) s += partial[((long)c * nsplit + y) * n + i];
+            a[c] = s;
+        }
+        const float f0 = (float)(a[0] * fscale), f1 = (float)(a[1] * fscale), f2 = (float)(a[2] * fscale);
+        fx[i] = f0;
+        fy[i] = f1;
+        fz[i] = f2;
+        pot[i] = (float)(a[3] * pscale);
+        if (e.kick_ascending || e.descending) {
+            float u = vx[i], v = vy[i], w = vz[i];
+            int8_t r = rung[i];
+            epilogue(f0, f1, f2, u, v, w, r, e, rmax);
+            vx[i] = u;
+            vy[i] = v;
+            vz[i] = w;
+            rung[i] = r;
+        }
+    }
+    // wave-level max, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) rmax = max(rmax, __shfl_xor(rmax, off));
+    if ((threadIdx.x & 63) == 0 && rmax > 0) atomicMax(maxrung, rmax);
+}
+
+// FP64 judge: block s handles sink idx[s]; 256 lanes stride the sources.
+template <int KERNEL>
+__global__ __launch_bounds__(256) void kick_sample_fp64(const uint32_t* __restrict__ X,
+                                                        const uint32_t* __restrict__ Y,
+                                                        const uint32_t* __restrict__ Z, int n,
+                                                        const long* __restrict__ idx, int m, double soft,
+                                                        double gm, double* __restrict__ out) {
+    __shared__ double red[4][4];
+    const int s = blockIdx.x;
+    const long i = idx[s];
+    const uint32_t xi = X[i], yi = Y[i], zi = Z[i];
+    const double e2 = soft * soft, hinv = 1.0 / soft, hinv3 = hinv * hinv * hinv;
+    double ax = 0.0, ay = 0.0, az = 0.0, ap = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        const double dx = (double)(int32_t)(X[j] - xi) * 0x1p-32;
+        const double dy = (double)(int32_t)(Y[j] - yi) * 0x1p-32;
+        const double dz = (double)(int32_t)(Z[j] - zi) * 0x1p-32;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double k, p;
+        if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+            const double w = 1.0 / sqrt(r2 + e2);
+            k = w * w * w;
+            p = w;
+        } else {
+            if (r2 < e2) {
+                k = hinv3;
+                p = 1.5 * hinv - 0.5 * r2 * hinv3;
+            } else {
+                const double w = 1.0 / sqrt(r2);
+                k = w * w * w;
+                p = w;
+            }
+        }
+        ax += dx * k;
+        ay += dy * k;
+        az += dz * k;
+        ap += p;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_xor(ax, off);
+        ay += __shfl_xor(ay, off);
+        az += __shfl_xor(az, off);
+        ap += __shfl_xor(ap, off);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = ax;
+        red[wave][1] = ay;
+        red[wave][2] = az;
+        red[wave][3] = ap;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int c = threadIdx.x;
+        const double t = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+        out[(long)c * m + s] = gm * t;
+    }
+}
+
+double* g_sample_out = nullptr;
+size_t g_sample_bytes = 0;
+long* g_sample_idx = nullptr;
+size_t g_sample_idx_bytes = 0;
+
+}  // namespace
+
+void release_kick_buffers() {
+    Context& c = ctx();
+    dev_free(c.partial);
+    c.partial = nullptr;
+    c.partial_bytes = 0;
+    dev_free(c.maxrung);
+    c.maxrung = nullptr;
+    dev_free(g_sample_out);
+    g_sample_out = nullptr;
+    g_sample_bytes = 0;
+    dev_free(g_sample_idx);
+    g_sample_idx = nullptr;
+    g_sample_idx_bytes = 0;
+}
+
+}  // namespace ct2
+
+using namespace ct2;
+
+extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung) {
+    CT2_TRY(require_particles());
+    if (!p) return set_error(CT2_ERR_INVALID, "null kick_params");
+    if (flags & ~(unsigned)(CT2_KICK_NAIVE | CT2_KICK_STORE))
+        return set_error(CT2_ERR_INVALID, "unknown kick flags 0x%x", flags);
+    const bool epi = (p->ascending && !p->first) || p->descending;
+    if (epi) {
+        if (p->rung < 0 || p->rung > 30) return set_error(CT2_ERR_INVALID, "rung %d out of [0,30]", p->rung);
+        if (!(p->t0 > 0.0f)) return set_error(CT2_ERR_INVALID, "t0 must be positive");
+    }
+    Context& c = ctx();
+    const int n = (int)c.n;
+    const double two32 = 4294967296.0;
+    const double h = c.opts.hsoft;
+    KickConst kc;
+    kc.eps2u = (float)((h * two32) * (h * two32));
+    kc.h2u = kc.eps2u;
+    kc.hinvu = (float)(1.0 / (h * two32));
+    kc.hinv3u = (float)(1.0 / ((h * two32) * (h * two32) * (h * two32)));
+
+    // sink blocks and source chunks: enough workgroups to fill 256 CUs
+    const int nb = (n + KICK_BLOCK - 1) / KICK_BLOCK;
+    const int max_split = std::max(1, n / (4 * KICK_TILE));
+    const int nsplit = std::min(max_split, std::max(1, (4096 + nb - 1) / nb));
+    int jchunk = (n + nsplit - 1) / nsplit;
+    jchunk = (jchunk + KICK_TILE - 1) / KICK_TILE * KICK_TILE;
+    const int nsplit_eff = (n + jchunk - 1) / jchunk;
+
+    CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, sizeof(double) * 4 * (size_t)nsplit_eff * n,
+                           "kick partial sums"));
+    if (!c.maxrung) CT2_TRY(dev_alloc((void**)&c.maxrung, sizeof(int), "max rung"));
+
+    const dim3 grid(nb, nsplit_eff);
+    const bool naive = flags & CT2_KICK_NAIVE;
+    if (c.opts.kernel == CT2_KERNEL_PLUMMER) {
+        if (naive)
+            kick_direct<CT2_KERNEL_PLUMMER, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+        else
+            kick_direct<CT2_KERNEL_PLUMMER, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+    } else {
+        if (naive)
+            kick_direct<CT2_KERNEL_CT2, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+        else
+            kick_direct<CT2_KERNEL_CT2, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+    }
+    CT2_LAUNCH_CHECK();
+
+    EpiConst e;
+    {
+#pragma clang fp contract(off)
+        e.kick_ascending = p->ascending && !p->first;
+        e.descending = p->descending;
+        e.sgn = p->top ? 1.0f : -1.0f;
+        e.dtk = epi ? 0.5f * p->t0 / (float)(1 << p->rung) : 0.0f;
+        e.eta = p->eta;
+        e.scale_h = p->scale * c.opts.hsoft;
+        e.t0 = p->t0;
+        e.max_dt = p->max_dt;
+        e.two_r = epi ? std::ldexp(1.0f, p->rung) : 1.0f;
+        e.prung = p->rung;
+    }
+    // force in box units: sum(dxu * w3u) * 2^64 * GM; potential: sum(wu) * 2^32 * GM
+    const double fscale = (double)c.opts.GM * two32 * two32;
+    const double pscale = (double)c.opts.GM * two32;
+    kick_finalize<<<(n + 255) / 256, 256, 0, c.stream>>>(c.partial, nsplit_eff, n, fscale, pscale, e, c.vx, c.vy,
+                                                          c.vz, c.rung, c.fx, c.fy, c.fz, c.pot, c.maxrung);
+    CT2_LAUNCH_CHECK();
+    c.forces_valid = true;
+    if (max_rung) {
+        CT2_HIP(hipMemcpyAsync(max_rung, c.maxrung, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        CT2_HIP(hipStreamSynchronize(c.stream));
+    }
+    (void)(flags & CT2_KICK_STORE);  // forces are always kept; the flag documents intent
+    return CT2_OK;
+}
+
+extern "C" int ct2_kick_sample_fp64(const int64_t* idx, size_t m, int periodic, double* ax, double* ay,
+                                    double* az, double* pot) {
+    CT2_TRY(require_particles());
+    Context& c = ctx();
+    if (!idx || m == 0) return set_error(CT2_ERR_INVALID, "empty sample");
+    if (periodic) return set_error(CT2_ERR_INVALID, "periodic judge not available in this build");
+    for (size_t s = 0; s < m; ++s)
+        if (idx[s] < 0 || (size_t)idx[s] >= c.n)
+            return set_error(CT2_ERR_INVALID, "sample index %lld out of range", (long long)idx[s]);
+    if (m > (size_t)INT32_MAX) return set_error(CT2_ERR_INVALID, "sample too large");
+    CT2_TRY(ensure_scratch((void**)&g_sample_out, &g_sample_bytes, sizeof(double) * 4 * m, "judge output"));
+    CT2_TRY(ensure_scratch((void**)&g_sample_idx, &g_sample_idx_bytes, sizeof(long) * m, "judge indices"));
+    CT2_HIP(hipMemcpyAsync(g_sample_idx, idx, sizeof(long) * m, hipMemcpyHostToDevice, c.stream));
+    const double soft = c.opts.hsoft, gm = c.opts.GM;
+    if (c.opts.kernel == CT2_KERNEL_PLUMMER)
+        kick_sample_fp64<CT2_KERNEL_PLUMMER><<<(unsigned)m, 256, 0, c.stream>>>(c.x, c.y, c.z, (int)c.n, g_sample_idx,
+                                                                               (int)m, soft, gm, g_sample_out);
+    else
+        kick_sample_fp64<CT2_KERNEL_CT2><<<(unsigned)m, 256, 0, c.stream>>>(c.x, c.y, c.z, (int)c.n, g_sample_idx,
+                                                                           (int)m, soft, gm, g_sample_out);
+    CT2_LAUNCH_CHECK();
+    double* dst[4] = {ax, ay, az, pot};
+    for (int a = 0; a < 4; ++a)
+        if (dst[a])
+            CT2_HIP(hipMemcpyAsync(dst[a], g_sample_out + (size_t)a * m, sizeof(double) * m, hipMemcpyDeviceToHost,
+                                   c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    return CT2_OK;
+}

@@ -1,0 +1,158 @@
+/*
+ * ct2hip.h -- C ABI of libct2hip.so, the MI355X (gfx950) force path of
+ * CosmicTiger 2: direct-sum kick, drift, 3-D FFT and the particle-mesh
+ * force, on ct2's structure-of-arrays particle container.
+ *
+ * Conventions
+ *   - Every call returns an int status (CT2_OK == 0); ct2_last_error()
+ *     describes the last failure.  No C++ exception crosses this boundary.
+ *   - The library owns the device buffers it allocates (particles, forces,
+ *     grids); the caller owns every host pointer it passes.
+ *   - One device and one HIP stream per process (ct2_init).  Calls are
+ *     thread-compatible, not re-entrant: serialise them.
+ *   - Positions are ct2 fixed32 (sfmm::fixed32): [0,1) <-> [0,2^32), periodic
+ *     by integer wrap; distances are the wrapped difference read as int32.
+ *
+ * Each declaration names the reference interface it replaces (paths are
+ * relative to the ct2 source tree).
+ */
+#ifndef CT2HIP_H
+#define CT2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+#define CT2_OK 0
+#define CT2_ERR_INVALID 1  /* bad argument: size, null pointer, not a power of two */
+#define CT2_ERR_STATE 2    /* wrong call order: not initialised, no particles */
+#define CT2_ERR_NOMEM 3    /* device allocation failed */
+#define CT2_ERR_HIP 4      /* HIP runtime error */
+#define CT2_ERR_NODEVICE 5 /* no usable gfx950 device */
+
+/* Text of the last error (static storage, overwritten by the next error). */
+const char* ct2_last_error(void);
+
+/* ------------------------------------------------------------ context */
+/* Replaces the process set-up of src/main.cpp:94-102 (hpx_main): binds the
+ * device, creates the library stream.  Idempotent for the same device. */
+int ct2_init(int device);
+/* Frees every device buffer and the stream. */
+int ct2_finalize(void);
+/* The library's HIP stream (hipStream_t) for callers that time or order work. */
+int ct2_get_stream(void** stream);
+int ct2_synchronize(void);
+
+/* ------------------------------------------------------------ options */
+#define CT2_KERNEL_PLUMMER 0 /* F = GM dx (r^2+h^2)^-3/2 (default) */
+#define CT2_KERNEL_CT2 1     /* uniform-density sphere of radius h, kick.cpp:52-60 */
+
+/* The options the force path reads from get_options() (options.hpp:14-35):
+ * hsoft (options.cpp:155), GM = 1/Ngrid^3 (options.cpp:190). */
+typedef struct {
+    float hsoft;
+    float GM;
+    int kernel;
+} ct2_force_options;
+int ct2_set_force_options(const ct2_force_options* opts);
+
+/* ------------------------------------------------------------ particles */
+/* Replaces particle_allocate(size_t) (particle.cpp:273-288): device SoA
+ * arrays x,y,z (uint32 fixed32), vx,vy,vz (float), rung (int8).  Contents
+ * are zeroed.  Re-allocating discards the previous set. */
+int ct2_particles_alloc(size_t n);
+/* Replaces particle_count() (particle.hpp:60, particle.cpp:27-29). */
+size_t ct2_particle_count(void);
+/* Host <-> device copies of the container (the accessors of
+ * particle.hpp:30-56 operate on host copies).  A NULL pointer skips that
+ * array. */
+int ct2_particles_upload(const uint32_t* x, const uint32_t* y, const uint32_t* z, const float* vx,
+                         const float* vy, const float* vz, const int8_t* rung);
+int ct2_particles_download(uint32_t* x, uint32_t* y, uint32_t* z, float* vx, float* vy,
+                           float* vz, int8_t* rung);
+
+/* Device pointers of the container (particle_x_ptr ... particle_rung_ptr,
+ * particle.hpp:22-28) for callers with their own HIP code. */
+typedef struct {
+    uint32_t *x, *y, *z;
+    float *vx, *vy, *vz;
+    int8_t* rung;
+    size_t n;
+} ct2_particle_ptrs;
+int ct2_particles_device_ptrs(ct2_particle_ptrs* out);
+
+/* ------------------------------------------------------------ kick */
+/* kick_params, kick.hpp:10-20 (bools as int). */
+typedef struct {
+    int ascending;
+    int descending;
+    int top;
+    int first;
+    int rung;
+    float t0;
+    float eta;
+    float max_dt;
+    float scale;
+} ct2_kick_params;
+
+#define CT2_KICK_NAIVE 0x1 /* plain FP32 accumulation instead of compensated */
+#define CT2_KICK_STORE 0x2 /* keep per-particle force and potential for ct2_forces_download */
+
+/* Replaces int kick(const kick_params&) (kick.hpp:24, kick.cpp:105-125):
+ * all-pairs FP32 direct sum over the whole container with compensated
+ * accumulation, then the half-kick and rung epilogue of kick.cpp:75-99.
+ * *max_rung receives the return value of kick(). */
+int ct2_kick(const ct2_kick_params* params, unsigned flags, int* max_rung);
+/* The force (GM-scaled, toward the sources) and positive potential of the
+ * last ct2_kick made with CT2_KICK_STORE.  NULL skips an array. */
+int ct2_forces_download(float* fx, float* fy, float* fz, float* pot);
+
+/* FP64 direct sum for m sampled sinks (indices into the container), the
+ * judge of every FP32 force: same kernel and options as ct2_kick.
+ * periodic != 0 adds the Ewald images of every source (with a uniform
+ * neutralising background) through a tabulated FP64 correction. */
+int ct2_kick_sample_fp64(const int64_t* idx, size_t m, int periodic, double* ax, double* ay,
+                         double* az, double* pot);
+
+/* ------------------------------------------------------------ drift */
+/* Replaces void drift(float dt, float scale) (drift.hpp:12, drift.cpp:4-18):
+ * x += fixed32(v * (dt/scale)) per axis, with fixed32(s) defined as
+ * RNE(s * 2^32) mod 2^32 (0 for non-finite s).  Bit-exact. */
+int ct2_drift(float dt, float scale);
+
+/* ------------------------------------------------------------ FFT */
+/* In-place 3-D complex transform of an n^3 cube in device memory, natural
+ * order, index (i*n + j)*n + k, interleaved (re, im).  dir = +1 is
+ * e^{-2 pi i k.x/n} (FFTW_FORWARD), dir = -1 its conjugate; unnormalised.
+ * n is a power of two in [8, 1024] (FP32) or [8, 512] (FP64).  Replaces the
+ * 3-D transforms the PM path calls (fft.hpp:28-29, particle.cpp:230,261). */
+int ct2_fft3d_c2c(void* data, int n, int dir);
+int ct2_fft3d_c2c_f64(void* data, int n, int dir);
+
+/* Device memory for callers without HIP code of their own. */
+int ct2_dev_alloc(size_t bytes, void** ptr);
+int ct2_dev_free(void* ptr);
+int ct2_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int ct2_memcpy_d2h(void* dst, const void* src, size_t bytes);
+
+/* ------------------------------------------------------------ initial conditions */
+/* Seeded synthetic particle sets (host side; replace particles_init,
+ * particle.cpp:364-374, whose libc rand() is not reproducible).
+ * Lattice (i+1/2)/n per axis plus a Gaussian displacement of sigma_cells
+ * lattice cells per axis, wrapped, rounded RNE to fixed32.  n^3 particles,
+ * ordered (i*n + j)*n + k. */
+int ct2_ic_lattice(int n, double sigma_cells, uint64_t seed, uint32_t* x, uint32_t* y,
+                   uint32_t* z);
+/* n particles: the first n_sph drawn from `nspheres` Plummer spheres of
+ * scale radius `a` truncated at rmax (centres uniform), the rest uniform. */
+int ct2_ic_plummer_mix(size_t n, size_t n_sph, int nspheres, double a, double rmax,
+                       uint64_t seed, uint32_t* x, uint32_t* y, uint32_t* z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,223 @@
+"""ctypes front end of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path in ``ct2_amd``.
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg import this module; the product never does.
+
+Each wrapper names the reference file:line its C function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+KERNEL_PLUMMER = 0
+KERNEL_CT2 = 1
+
+
+def build() -> None:
+    """Compile liboracle.so from the C sources in this folder."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+class KickParams(C.Structure):
+    """kick_params, kick.hpp:10-20."""
+
+    _fields_ = [("ascending", C.c_int), ("descending", C.c_int), ("top", C.c_int),
+                ("first", C.c_int), ("rung", C.c_int), ("t0", C.c_float), ("eta", C.c_float),
+                ("max_dt", C.c_float), ("scale", C.c_float)]
+
+
+class Cosmo(C.Structure):
+    _fields_ = [("H", C.c_double), ("omega_m", C.c_double), ("omega_r", C.c_double),
+                ("omega_lam", C.c_double)]
+
+
+_P = C.c_void_p
+
+
+def _declare(L):
+    sz, i, d, f = C.c_size_t, C.c_int, C.c_double, C.c_float
+    L.orc_kick.argtypes = [sz, _P, _P, _P, sz, _P, d, d, i, _P, _P, _P, _P]
+    L.orc_kick_ld.argtypes = L.orc_kick.argtypes
+    L.orc_kick_epilogue.argtypes = [sz, _P, _P, _P, _P, _P, _P, _P, C.POINTER(KickParams), f]
+    L.orc_kick_epilogue.restype = i
+    L.orc_fixed32_from_float.argtypes = [f]
+    L.orc_fixed32_from_float.restype = C.c_uint32
+    L.orc_drift.argtypes = [sz, _P, _P, _P, _P, _P, _P, f, f]
+    L.orc_scramble.argtypes = [_P, i, i, i]
+    L.orc_twiddles.argtypes = [i, _P, _P, _P]
+    L.orc_transpose_yz.argtypes = [_P, i]
+    L.orc_transpose_r2c.argtypes = [_P, _P, i]
+    L.orc_transpose_c2r.argtypes = [_P, _P, i]
+    L.orc_fft_complex.argtypes = [_P, i, i, i]
+    L.orc_fft_real.argtypes = [_P, i, i, i]
+    L.orc_fft3d.argtypes = [_P, i, i]
+    L.orc_fft_r2c_3d.argtypes = [_P, _P, i]
+    L.orc_fft_c2r_3d.argtypes = [_P, _P, i]
+    L.orc_cosmos_update.argtypes = [C.POINTER(Cosmo), _P, _P, _P, d]
+    L.orc_cosmos_dadt.argtypes = [C.POINTER(Cosmo), d]
+    L.orc_cosmos_dadt.restype = d
+    L.orc_cosmos_conformal_time.argtypes = [C.POINTER(Cosmo), d, d]
+    L.orc_cosmos_conformal_time.restype = d
+    L.orc_cosmos_time.argtypes = [C.POINTER(Cosmo), d, d]
+    L.orc_cosmos_time.restype = d
+
+
+def _p(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ---------------------------------------------------------------- kick
+def kick(x, y, z, gm, soft, kernel=KERNEL_PLUMMER, sinks=None, long_double=False):
+    """FP64 all-pairs kick (kick.cpp:36-74).  Returns (ax, ay, az, pot) for the
+    sinks (all particles when ``sinks`` is None)."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    n = x.size
+    if sinks is None:
+        m, sp = n, None
+    else:
+        sinks = np.ascontiguousarray(sinks, dtype=np.int64)
+        m, sp = sinks.size, _p(sinks)
+    out = [np.empty(m, np.float64) for _ in range(4)]
+    fn = lib().orc_kick_ld if long_double else lib().orc_kick
+    fn(n, _p(x), _p(y), _p(z), m, sp, gm, soft, kernel, *[_p(o) for o in out])
+    return tuple(out)
+
+
+def kick_epilogue(fx, fy, fz, vx, vy, vz, rung, params: KickParams, hsoft):
+    """Half-kick and rung update (kick.cpp:75-99) in FP32; updates vx, vy, vz
+    and rung in place and returns the maximum rung."""
+    fx, fy, fz = (np.ascontiguousarray(a, np.float32) for a in (fx, fy, fz))
+    for a in (vx, vy, vz):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    assert rung.dtype == np.int8
+    return lib().orc_kick_epilogue(fx.size, _p(fx), _p(fy), _p(fz), _p(vx), _p(vy), _p(vz),
+                                   _p(rung), C.byref(params), hsoft)
+
+
+def fixed32_from_float(s: float) -> int:
+    return int(lib().orc_fixed32_from_float(s))
+
+
+def drift(x, y, z, vx, vy, vz, dt, scale):
+    """drift.cpp:4-17 on copies; returns the new (x, y, z)."""
+    x, y, z = (np.array(a, dtype=np.uint32, copy=True) for a in (x, y, z))
+    vx, vy, vz = (np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz))
+    lib().orc_drift(x.size, _p(x), _p(y), _p(z), _p(vx), _p(vy), _p(vz), dt, scale)
+    return x, y, z
+
+
+# ---------------------------------------------------------------- FFT
+def scramble(X, N1, N2, N3):
+    X = np.array(X, dtype=np.float64, copy=True)
+    lib().orc_scramble(_p(X), N1, N2, N3)
+    return X
+
+
+def twiddles(N):
+    c, s, i = (np.empty(N) for _ in range(3))
+    lib().orc_twiddles(N, _p(c), _p(s), _p(i))
+    return c, s, i
+
+
+def fft_complex(X, N, dir=1, lanes=1):
+    """fft_complex_vector.cpp:9-65 on a copy of the interleaved lane layout."""
+    X = np.array(X, dtype=np.float64, copy=True)
+    lib().orc_fft_complex(_p(X), N, dir, lanes)
+    return X
+
+
+def fft_real(X, N, dir=1, lanes=1):
+    """fft_real_scalar.cpp / fft_real_vector.cpp on a copy."""
+    X = np.array(X, dtype=np.float64, copy=True)
+    lib().orc_fft_real(_p(X), N, dir, lanes)
+    return X
+
+
+def fft3d(Z: np.ndarray, dir=1) -> np.ndarray:
+    """3-D complex transform of a complex128 n^3 cube, natural order."""
+    Z = np.array(Z, dtype=np.complex128, copy=True)
+    n = Z.shape[0]
+    lib().orc_fft3d(_p(Z), n, dir)
+    return Z
+
+
+def fft_r2c_3d(X: np.ndarray) -> np.ndarray:
+    N = X.shape[0]
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    out = np.empty(2 * N * N * (N // 2 + 1))
+    lib().orc_fft_r2c_3d(_p(X), _p(out), N)
+    return out
+
+
+def fft_c2r_3d(Y: np.ndarray, N: int) -> np.ndarray:
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    out = np.empty(N * N * N)
+    lib().orc_fft_c2r_3d(_p(Y), _p(out), N)
+    return out.reshape(N, N, N)
+
+
+def transpose_yz(X, N):
+    X = np.array(X, dtype=np.float64, copy=True)
+    lib().orc_transpose_yz(_p(X), N)
+    return X
+
+
+def transpose_r2c(X, N):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    out = np.empty(2 * N * N * (N // 2 + 1))
+    lib().orc_transpose_r2c(_p(X), _p(out), N)
+    return out
+
+
+def transpose_c2r(Y, N):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    out = np.zeros(N * N * N)
+    lib().orc_transpose_c2r(_p(Y), _p(out), N)
+    return out
+
+
+# ---------------------------------------------------------------- cosmology
+def cosmo(H, omega_m=1.0, omega_r=0.0, omega_lam=0.0) -> Cosmo:
+    return Cosmo(H, omega_m, omega_r, omega_lam)
+
+
+def cosmos_update(c: Cosmo, adotdot, adot, a, dt0):
+    v = [C.c_double(adotdot), C.c_double(adot), C.c_double(a)]
+    lib().orc_cosmos_update(C.byref(c), C.byref(v[0]), C.byref(v[1]), C.byref(v[2]), dt0)
+    return v[0].value, v[1].value, v[2].value
+
+
+def cosmos_dadt(c: Cosmo, a):
+    return lib().orc_cosmos_dadt(C.byref(c), a)
+
+
+def cosmos_conformal_time(c: Cosmo, a0, a1):
+    return lib().orc_cosmos_conformal_time(C.byref(c), a0, a1)
+
+
+def cosmos_time(c: Cosmo, a0, a1):
+    return lib().orc_cosmos_time(C.byref(c), a0, a1)
