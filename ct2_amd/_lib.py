@@ -66,6 +66,11 @@ _SIGS = {
     "ct2_forces_download": ([_P] * 4, C.c_int),
     "ct2_kick_sample_fp64": ([_P, C.c_size_t, C.c_int, _P, _P, _P, _P], C.c_int),
     "ct2_drift": ([C.c_float, C.c_float], C.c_int),
+    "ct2_pm_init": ([C.c_int, C.c_double], C.c_int),
+    "ct2_pm_forces": ([], C.c_int),
+    "ct2_kick_pm": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
+    "ct2_pm_density_download": ([_P], C.c_int),
+    "ct2_pm_grid_download": ([C.c_int, _P], C.c_int),
     "ct2_fft3d_c2c": ([_P, C.c_int, C.c_int], C.c_int),
     "ct2_fft3d_c2c_f64": ([_P, C.c_int, C.c_int], C.c_int),
     "ct2_dev_alloc": ([C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),

@@ -131,6 +131,43 @@ def drift(dt: float, scale: float) -> None:
     L.check(L.load().ct2_drift(dt, scale))
 
 
+# ---------------------------------------------------------------- particle mesh
+_pm_ng = 0
+
+
+def pm_init(ng: int, rs_cells: float = 1.25) -> None:
+    """Allocates the PM grids and the FP64 Green's table (ct2_pm_init)."""
+    global _pm_ng
+    L.check(L.load().ct2_pm_init(ng, rs_cells))
+    _pm_ng = ng
+
+
+def pm_forces():
+    """PM force and potential of every particle: (fx, fy, fz, pot)."""
+    L.check(L.load().ct2_pm_forces())
+    return forces_download()
+
+
+def kick_pm(params: KickParams | None = None, flags: int = 0) -> int:
+    if params is None:
+        params = kick_params()
+    mr = C.c_int(0)
+    L.check(L.load().ct2_kick_pm(C.byref(params), flags, C.byref(mr)))
+    return mr.value
+
+
+def pm_density_download() -> np.ndarray:
+    g = np.empty(_pm_ng ** 3, np.uint64)
+    L.check(L.load().ct2_pm_density_download(_p(g)))
+    return g.reshape((_pm_ng,) * 3)
+
+
+def pm_grid_download(which: int) -> np.ndarray:
+    g = np.empty(_pm_ng ** 3, np.complex64)
+    L.check(L.load().ct2_pm_grid_download(which, _p(g)))
+    return g.reshape((_pm_ng,) * 3)
+
+
 # ---------------------------------------------------------------- device buffers / FFT
 class DeviceBuffer:
     """A device allocation made through the library (freed on close/del)."""

@@ -113,6 +113,8 @@ int ct2_finalize(void) {
     free_particles();
     release_kick_buffers();
     release_fft_plans();
+    release_pm();
+    release_ewald();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device = -1;

@@ -53,9 +53,19 @@ int ensure_scratch(void** p, size_t* have, size_t bytes, const char* what);
 // In-place 3-D complex FFT on the library stream (fft.hip).
 int fft3d(void* data, int n, int dir, bool f64);
 
+// PM force of the container into the context force arrays (pm.hip).
+int pm_forces();
+int pm_require();
+void pm_grids(int* g, const unsigned long long** W, const float2** A, const float2** B);
+
+// Periodic FP64 judge of m sinks (device index list) into d_out[4][m] (ewald.hip).
+int ewald_judge(const long* d_idx, int m, double* d_out);
+
 // Called by ct2_finalize to release module-owned buffers.
 void release_kick_buffers();
 void release_fft_plans();
+void release_pm();
+void release_ewald();
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(long v) {

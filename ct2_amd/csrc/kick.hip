@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "epilogue.h"
 #include "internal.h"
 
 namespace ct2 {
@@ -31,20 +32,6 @@ struct KickConst {
     float h2u;     // ct2 kernel: (h 2^32)^2
     float hinvu;   //             1/(h 2^32)
     float hinv3u;  //             1/(h 2^32)^3
-};
-
-// Epilogue constants, kick.cpp:75-99, precomputed on the host in FP32.
-struct EpiConst {
-    int kick_ascending;  // ascending && !first
-    int descending;
-    float sgn;     // top ? +1 : -1
-    float dtk;     // 0.5 t0 / 2^rung
-    float eta;
-    float scale_h;  // scale * hsoft
-    float t0;
-    float max_dt;
-    float two_r;  // 2^rung
-    int prung;
 };
 
 __device__ __forceinline__ void two_sum(float& s, float& c, float v) {
@@ -152,32 +139,6 @@ __global__ __launch_bounds__(KICK_BLOCK) void kick_direct(const uint32_t* __rest
     }
 }
 
-// kick.cpp:75-99 for one sink, in the FP32 operation order of the
-// reference (no contraction), so that it matches the CPU restatement bit
-// for bit given the same force.
-__device__ __forceinline__ void epilogue(float f0, float f1, float f2, float& vx, float& vy,
-                                         float& vz, int8_t& rung, const EpiConst& e, int& rmax) {
-#pragma clang fp contract(off)
-    if (e.kick_ascending) {
-        vx = vx + e.sgn * f0 * e.dtk;
-        vy = vy + e.sgn * f1 * e.dtk;
-        vz = vz + e.sgn * f2 * e.dtk;
-    }
-    if (e.descending) {
-        const float g2 = f0 * f0 + f1 * f1 + f2 * f2;
-        float dt = fminf(e.eta * sqrtf(e.scale_h / sqrtf(g2)), e.t0);
-        dt = fminf(e.max_dt, dt);
-        const int up = (e.t0 / dt) > e.two_r;
-        int r = rung;
-        r = max(e.prung + up, r - 1);
-        rung = (int8_t)r;
-        rmax = max(rmax, r);
-        vx = vx + e.sgn * f0 * e.dtk;
-        vy = vy + e.sgn * f1 * e.dtk;
-        vz = vz + e.sgn * f2 * e.dtk;
-    }
-}
-
 __global__ __launch_bounds__(256) void kick_finalize(const double* __restrict__ partial, int nsplit,
                                                      int n, double fscale, double pscale, EpiConst e,
                                                      float* __restrict__ vx, float* __restrict__ vy,
@@ -214,6 +175,29 @@ __global__ __launch_bounds__(256) void kick_finalize(const double* __restrict__ 
     for (int off = 32; off > 0; off >>= 1) rmax = max(rmax, __shfl_xor(rmax, off));
     if ((threadIdx.x & 63) == 0 && rmax > 0) atomicMax(maxrung, rmax);
 }
+
+// Epilogue over stored forces (mesh-based kicks).
+__global__ __launch_bounds__(256) void epilogue_kernel(const float* __restrict__ fx, const float* __restrict__ fy,
+                                                       const float* __restrict__ fz, int n, EpiConst e,
+                                                       float* __restrict__ vx, float* __restrict__ vy,
+                                                       float* __restrict__ vz, int8_t* __restrict__ rung,
+                                                       int* __restrict__ maxrung) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int rmax = 0;
+    if (i < n) {
+        float u = vx[i], v = vy[i], w = vz[i];
+        int8_t r = rung[i];
+        epilogue(fx[i], fy[i], fz[i], u, v, w, r, e, rmax);
+        vx[i] = u;
+        vy[i] = v;
+        vz[i] = w;
+        rung[i] = r;
+    }
+    for (int off = 32; off > 0; off >>= 1) rmax = max(rmax, __shfl_xor(rmax, off));
+    if ((threadIdx.x & 63) == 0 && rmax > 0) atomicMax(maxrung, rmax);
+}
+
+__global__ void zero_int(int* p) { *p = 0; }
 
 // FP64 judge: block s handles sink idx[s]; 256 lanes stride the sources.
 template <int KERNEL>
@@ -296,20 +280,43 @@ void release_kick_buffers() {
     g_sample_idx_bytes = 0;
 }
 
+int check_kick_params(const ct2_kick_params* p) {
+    if (!p) return set_error(CT2_ERR_INVALID, "null kick_params");
+    if (epilogue_active(p)) {
+        if (p->rung < 0 || p->rung > 30) return set_error(CT2_ERR_INVALID, "rung %d out of [0,30]", p->rung);
+        if (!(p->t0 > 0.0f)) return set_error(CT2_ERR_INVALID, "t0 must be positive");
+    }
+    return CT2_OK;
+}
+
+int run_epilogue(const ct2_kick_params* p, int* max_rung) {
+    Context& c = ctx();
+    if (!c.maxrung) CT2_TRY(dev_alloc((void**)&c.maxrung, sizeof(int), "max rung"));
+    zero_int<<<1, 1, 0, c.stream>>>(c.maxrung);
+    CT2_LAUNCH_CHECK();
+    if (epilogue_active(p)) {
+        const EpiConst e = make_epilogue(p, c.opts.hsoft);
+        const int n = (int)c.n;
+        epilogue_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(c.fx, c.fy, c.fz, n, e, c.vx, c.vy, c.vz, c.rung,
+                                                               c.maxrung);
+        CT2_LAUNCH_CHECK();
+    }
+    if (max_rung) {
+        CT2_HIP(hipMemcpyAsync(max_rung, c.maxrung, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        CT2_HIP(hipStreamSynchronize(c.stream));
+    }
+    return CT2_OK;
+}
+
 }  // namespace ct2
 
 using namespace ct2;
 
 extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung) {
     CT2_TRY(require_particles());
-    if (!p) return set_error(CT2_ERR_INVALID, "null kick_params");
+    CT2_TRY(check_kick_params(p));
     if (flags & ~(unsigned)(CT2_KICK_NAIVE | CT2_KICK_STORE))
         return set_error(CT2_ERR_INVALID, "unknown kick flags 0x%x", flags);
-    const bool epi = (p->ascending && !p->first) || p->descending;
-    if (epi) {
-        if (p->rung < 0 || p->rung > 30) return set_error(CT2_ERR_INVALID, "rung %d out of [0,30]", p->rung);
-        if (!(p->t0 > 0.0f)) return set_error(CT2_ERR_INVALID, "t0 must be positive");
-    }
     Context& c = ctx();
     const int n = (int)c.n;
     const double two32 = 4294967296.0;
@@ -347,20 +354,7 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
     }
     CT2_LAUNCH_CHECK();
 
-    EpiConst e;
-    {
-#pragma clang fp contract(off)
-        e.kick_ascending = p->ascending && !p->first;
-        e.descending = p->descending;
-        e.sgn = p->top ? 1.0f : -1.0f;
-        e.dtk = epi ? 0.5f * p->t0 / (float)(1 << p->rung) : 0.0f;
-        e.eta = p->eta;
-        e.scale_h = p->scale * c.opts.hsoft;
-        e.t0 = p->t0;
-        e.max_dt = p->max_dt;
-        e.two_r = epi ? std::ldexp(1.0f, p->rung) : 1.0f;
-        e.prung = p->rung;
-    }
+    const EpiConst e = make_epilogue(p, c.opts.hsoft);
     // force in box units: sum(dxu * w3u) * 2^64 * GM; potential: sum(wu) * 2^32 * GM
     const double fscale = (double)c.opts.GM * two32 * two32;
     const double pscale = (double)c.opts.GM * two32;
@@ -381,7 +375,6 @@ extern "C" int ct2_kick_sample_fp64(const int64_t* idx, size_t m, int periodic, 
     CT2_TRY(require_particles());
     Context& c = ctx();
     if (!idx || m == 0) return set_error(CT2_ERR_INVALID, "empty sample");
-    if (periodic) return set_error(CT2_ERR_INVALID, "periodic judge not available in this build");
     for (size_t s = 0; s < m; ++s)
         if (idx[s] < 0 || (size_t)idx[s] >= c.n)
             return set_error(CT2_ERR_INVALID, "sample index %lld out of range", (long long)idx[s]);
@@ -390,7 +383,11 @@ extern "C" int ct2_kick_sample_fp64(const int64_t* idx, size_t m, int periodic, 
     CT2_TRY(ensure_scratch((void**)&g_sample_idx, &g_sample_idx_bytes, sizeof(long) * m, "judge indices"));
     CT2_HIP(hipMemcpyAsync(g_sample_idx, idx, sizeof(long) * m, hipMemcpyHostToDevice, c.stream));
     const double soft = c.opts.hsoft, gm = c.opts.GM;
-    if (c.opts.kernel == CT2_KERNEL_PLUMMER)
+    if (periodic) {
+        if (c.opts.kernel != CT2_KERNEL_PLUMMER)
+            return set_error(CT2_ERR_INVALID, "the periodic judge uses Plummer softening only");
+        CT2_TRY(ewald_judge(g_sample_idx, (int)m, g_sample_out));
+    } else if (c.opts.kernel == CT2_KERNEL_PLUMMER)
         kick_sample_fp64<CT2_KERNEL_PLUMMER><<<(unsigned)m, 256, 0, c.stream>>>(c.x, c.y, c.z, (int)c.n, g_sample_idx,
                                                                                (int)m, soft, gm, g_sample_out);
     else

@@ -1,0 +1,285 @@
+// pm.hip -- the particle-mesh force on gfx950.
+//
+// Replaces the reference's multipole-mesh far field (particle_compute_multipoles
+// and particle_compute_expansions, particle.cpp:180-271, evaluated by L2P at
+// kick.cpp:29) with cloud-in-cell mass assignment, keeping its pipeline:
+//   cic_deposit   fixed-point CIC weights, uint64 atomics: deterministic,
+//                 bit-identical to the CPU oracle (12 B/particle read)
+//   to_complex    density grid -> FP32 complex grid
+//   fft3d         forward FP32 transform (fft.hip)
+//   green_mul     rho_hat * G (FP64 Green's table, octant-symmetric), then
+//                 the spectral gradient; two real fields share one complex
+//                 grid: A = Fx + i Fy, B = Fz + i pot
+//   fft3d x2      inverse transforms (two instead of four)
+//   cic_interp    force and potential at the particles (same CIC weights)
+// Grid layout: (i*ng + j)*ng + k with i along x; node c at position c/ng.
+#include <cmath>
+#include <vector>
+
+#include "epilogue.h"
+#include "internal.h"
+
+namespace ct2 {
+namespace {
+
+constexpr int CIC_BITS = 21;  // weight resolution per axis
+constexpr int PM_BLOCK = 256;
+
+struct PmState {
+    int ng = 0, g = 0;
+    double rs_cells = 0.0;
+    float gm_table = 0.0f;  // GM the table was built for
+    unsigned long long* W = nullptr;  // density, 2^-32 particle units
+    float2* A = nullptr;              // rho_hat -> Fx + i Fy
+    float2* B = nullptr;              // Fz + i pot
+    double* G = nullptr;              // Green's table on the octant, (ng/2+1)^3
+} g_pm;
+
+__device__ __forceinline__ void cic_axis(uint32_t x, int s, uint32_t& i0, uint64_t& a0, uint64_t& a1) {
+    i0 = x >> s;
+    const uint32_t t = x & (uint32_t)((1ull << s) - 1ull);
+    a1 = (uint64_t)(t >> (s - CIC_BITS));
+    a0 = (1ull << CIC_BITS) - a1;
+}
+
+__global__ __launch_bounds__(PM_BLOCK) void fill_u64(unsigned long long* __restrict__ p, long n) {
+    const long i = (long)blockIdx.x * PM_BLOCK + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ __launch_bounds__(PM_BLOCK) void cic_deposit(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                       const uint32_t* __restrict__ Z, int n, int g,
+                                                       unsigned long long* __restrict__ W) {
+    const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+    if (p >= n) return;
+    const int s = 32 - g;
+    const uint32_t ng = 1u << g, mask = ng - 1;
+    uint32_t i0, j0, k0;
+    uint64_t ax[2], ay[2], az[2];
+    cic_axis(X[p], s, i0, ax[0], ax[1]);
+    cic_axis(Y[p], s, j0, ay[0], ay[1]);
+    cic_axis(Z[p], s, k0, az[0], az[1]);
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const uint64_t axy = ax[dx] * ay[dy];
+            const size_t row = ((size_t)((i0 + dx) & mask) * ng + ((j0 + dy) & mask)) * ng;
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz)
+                atomicAdd(W + row + ((k0 + dz) & mask), (unsigned long long)((axy * az[dz]) >> 31));
+        }
+}
+
+__global__ __launch_bounds__(PM_BLOCK) void to_complex(const unsigned long long* __restrict__ W,
+                                                       float2* __restrict__ A, long n3) {
+    const long c = (long)blockIdx.x * PM_BLOCK + threadIdx.x;
+    if (c < n3) A[c] = make_float2((float)((double)W[c] * 0x1p-32), 0.0f);
+}
+
+// One thread per k-cell.  phi_hat = G rho_hat (FP64); F_hat = -i k phi_hat
+// with the Nyquist component of k zeroed; pot_hat = -phi_hat.
+__global__ __launch_bounds__(PM_BLOCK) void green_mul(float2* __restrict__ A, float2* __restrict__ B,
+                                                      const double* __restrict__ G, int g) {
+    const long c = (long)blockIdx.x * PM_BLOCK + threadIdx.x;
+    const int ng = 1 << g, h = ng / 2 + 1;
+    if (c >= (long)ng * ng * ng) return;
+    const int l = (int)(c & (ng - 1)), j = (int)((c >> g) & (ng - 1)), i = (int)(c >> (2 * g));
+    const int mi = i < ng / 2 ? i : i - ng, mj = j < ng / 2 ? j : j - ng, ml = l < ng / 2 ? l : l - ng;
+    const double gv = G[((size_t)abs(mi) * h + abs(mj)) * h + abs(ml)];
+    const float2 r = A[c];
+    const double pr = gv * (double)r.x, pi = gv * (double)r.y;
+    const double twopi = 6.283185307179586476925;
+    const double kx = i == ng / 2 ? 0.0 : twopi * mi;
+    const double ky = j == ng / 2 ? 0.0 : twopi * mj;
+    const double kz = l == ng / 2 ? 0.0 : twopi * ml;
+    // A = F_hat_x + i F_hat_y ; B = F_hat_z + i pot_hat
+    A[c] = make_float2((float)(kx * pi + ky * pr), (float)(-kx * pr + ky * pi));
+    B[c] = make_float2((float)((kz + 1.0) * pi), (float)(-(kz + 1.0) * pr));
+}
+
+__global__ __launch_bounds__(PM_BLOCK) void cic_interp(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                      const uint32_t* __restrict__ Z, int n, int g,
+                                                      const float2* __restrict__ A, const float2* __restrict__ B,
+                                                      float* __restrict__ fx, float* __restrict__ fy,
+                                                      float* __restrict__ fz, float* __restrict__ pot) {
+    const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+    if (p >= n) return;
+    const int s = 32 - g;
+    const uint32_t ng = 1u << g, mask = ng - 1;
+    uint32_t i0, j0, k0;
+    uint64_t ax[2], ay[2], az[2];
+    cic_axis(X[p], s, i0, ax[0], ax[1]);
+    cic_axis(Y[p], s, j0, ay[0], ay[1]);
+    cic_axis(Z[p], s, k0, az[0], az[1]);
+    const float q = 0x1p-21f;
+    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const float wxy = ((float)ax[dx] * q) * ((float)ay[dy] * q);
+            const size_t row = ((size_t)((i0 + dx) & mask) * ng + ((j0 + dy) & mask)) * ng;
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz) {
+                const size_t cc = row + ((k0 + dz) & mask);
+                const float w = wxy * ((float)az[dz] * q);
+                const float2 a = A[cc], b = B[cc];
+                acc0 = fmaf(w, a.x, acc0);
+                acc1 = fmaf(w, a.y, acc1);
+                acc2 = fmaf(w, b.x, acc2);
+                acc3 = fmaf(w, b.y, acc3);
+            }
+        }
+    fx[p] = acc0;
+    fy[p] = acc1;
+    fz[p] = acc2;
+    pot[p] = acc3;
+}
+
+void free_pm() {
+    dev_free(g_pm.W);
+    dev_free(g_pm.A);
+    dev_free(g_pm.B);
+    dev_free(g_pm.G);
+    g_pm = PmState{};
+}
+
+// FP64 Green's table on the octant |m_d| <= ng/2 (host, libm):
+//   G(m) = -4 pi GM exp(-k^2 rs^2) / (k^2 W(m)^2), W = prod_d sinc^2(pi m_d/ng), G(0) = 0.
+std::vector<double> green_table(int ng, double gm, double rs_cells) {
+    const int h = ng / 2 + 1;
+    const double rs = rs_cells / ng;
+    std::vector<double> t((size_t)h * h * h);
+    std::vector<double> sinc2(h), k2(h);
+    for (int m = 0; m < h; ++m) {
+        const double u = M_PI * m / ng;
+        const double sc = m ? std::sin(u) / u : 1.0;
+        sinc2[m] = sc * sc;
+        k2[m] = (2.0 * M_PI * m) * (2.0 * M_PI * m);
+    }
+    for (int a = 0; a < h; ++a)
+        for (int b = 0; b < h; ++b)
+            for (int c = 0; c < h; ++c) {
+                double v = 0.0;
+                if (a || b || c) {
+                    const double w = sinc2[a] * sinc2[b] * sinc2[c];
+                    const double kk = k2[a] + k2[b] + k2[c];
+                    v = -4.0 * M_PI * gm * std::exp(-kk * rs * rs) / (kk * w * w);
+                }
+                t[((size_t)a * h + b) * h + c] = v;
+            }
+    return t;
+}
+
+int ensure_table() {
+    Context& c = ctx();
+    if (g_pm.gm_table == c.opts.GM) return CT2_OK;
+    const std::vector<double> t = green_table(g_pm.ng, (double)c.opts.GM, g_pm.rs_cells);
+    CT2_HIP(hipMemcpyAsync(g_pm.G, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    g_pm.gm_table = c.opts.GM;
+    return CT2_OK;
+}
+
+}  // namespace
+
+int pm_require() {
+    CT2_TRY(require_particles());
+    if (!g_pm.ng) return set_error(CT2_ERR_STATE, "ct2_pm_init has not been called");
+    return CT2_OK;
+}
+
+// The PM force and potential of the container into the context force arrays.
+int pm_forces() {
+    CT2_TRY(pm_require());
+    CT2_TRY(ensure_table());
+    Context& c = ctx();
+    const int n = (int)c.n, g = g_pm.g;
+    const long n3 = 1L << (3 * g);
+    const unsigned gb = (unsigned)((n3 + PM_BLOCK - 1) / PM_BLOCK), pb = (unsigned)((n + PM_BLOCK - 1) / PM_BLOCK);
+    fill_u64<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, n3);
+    CT2_LAUNCH_CHECK();
+    cic_deposit<<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.W);
+    CT2_LAUNCH_CHECK();
+    to_complex<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, g_pm.A, n3);
+    CT2_LAUNCH_CHECK();
+    CT2_TRY(fft3d(g_pm.A, g_pm.ng, 1, false));
+    green_mul<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.A, g_pm.B, g_pm.G, g);
+    CT2_LAUNCH_CHECK();
+    CT2_TRY(fft3d(g_pm.A, g_pm.ng, -1, false));
+    CT2_TRY(fft3d(g_pm.B, g_pm.ng, -1, false));
+    cic_interp<<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.A, g_pm.B, c.fx, c.fy, c.fz, c.pot);
+    CT2_LAUNCH_CHECK();
+    c.forces_valid = true;
+    return CT2_OK;
+}
+
+// Device grids for the learned correction's feature gather.
+void pm_grids(int* g, const unsigned long long** W, const float2** A, const float2** B) {
+    *g = g_pm.g;
+    *W = g_pm.W;
+    *A = g_pm.A;
+    *B = g_pm.B;
+}
+
+void release_pm() { free_pm(); }
+
+}  // namespace ct2
+
+using namespace ct2;
+
+extern "C" int ct2_pm_init(int ng, double rs_cells) {
+    CT2_TRY(require_init());
+    if (!is_pow2(ng) || ng < 8 || ng > 1024)
+        return set_error(CT2_ERR_INVALID, "PM grid %d is not a power of two in [8, 1024]", ng);
+    if (!(rs_cells >= 0.0) || rs_cells > 64.0) return set_error(CT2_ERR_INVALID, "bad split radius %g", rs_cells);
+    if (g_pm.ng == ng && g_pm.rs_cells == rs_cells) return CT2_OK;
+    free_pm();
+    const size_t n3 = (size_t)ng * ng * ng, h = ng / 2 + 1;
+    int rc;
+    if ((rc = dev_alloc((void**)&g_pm.W, n3 * sizeof(unsigned long long), "PM density")) ||
+        (rc = dev_alloc((void**)&g_pm.A, n3 * sizeof(float2), "PM grid A")) ||
+        (rc = dev_alloc((void**)&g_pm.B, n3 * sizeof(float2), "PM grid B")) ||
+        (rc = dev_alloc((void**)&g_pm.G, h * h * h * sizeof(double), "PM Green's table"))) {
+        free_pm();
+        return rc;
+    }
+    g_pm.ng = ng;
+    g_pm.g = ilog2(ng);
+    g_pm.rs_cells = rs_cells;
+    g_pm.gm_table = 0.0f;
+    return CT2_OK;
+}
+
+extern "C" int ct2_pm_forces(void) { return pm_forces(); }
+
+extern "C" int ct2_kick_pm(const ct2_kick_params* p, unsigned flags, int* max_rung) {
+    CT2_TRY(pm_require());
+    CT2_TRY(check_kick_params(p));
+    if (flags & ~(unsigned)CT2_KICK_STORE) return set_error(CT2_ERR_INVALID, "unknown flags 0x%x", flags);
+    CT2_TRY(pm_forces());
+    return run_epilogue(p, max_rung);
+}
+
+extern "C" int ct2_pm_density_download(uint64_t* host) {
+    CT2_TRY(require_init());
+    if (!g_pm.ng) return set_error(CT2_ERR_STATE, "ct2_pm_init has not been called");
+    if (!host) return set_error(CT2_ERR_INVALID, "null output");
+    Context& c = ctx();
+    const size_t n3 = (size_t)g_pm.ng * g_pm.ng * g_pm.ng;
+    CT2_HIP(hipMemcpyAsync(host, g_pm.W, n3 * 8, hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    return CT2_OK;
+}
+
+extern "C" int ct2_pm_grid_download(int which, float* host) {
+    CT2_TRY(require_init());
+    if (!g_pm.ng) return set_error(CT2_ERR_STATE, "ct2_pm_init has not been called");
+    if (!host || (which != 0 && which != 1)) return set_error(CT2_ERR_INVALID, "bad grid request");
+    Context& c = ctx();
+    const size_t n3 = (size_t)g_pm.ng * g_pm.ng * g_pm.ng;
+    CT2_HIP(hipMemcpyAsync(host, which ? g_pm.B : g_pm.A, n3 * 8, hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    return CT2_OK;
+}

@@ -113,10 +113,32 @@ int ct2_forces_download(float* fx, float* fy, float* fz, float* pot);
 
 /* FP64 direct sum for m sampled sinks (indices into the container), the
  * judge of every FP32 force: same kernel and options as ct2_kick.
- * periodic != 0 adds the Ewald images of every source (with a uniform
- * neutralising background) through a tabulated FP64 correction. */
+ * periodic != 0 sums every periodic image of every source with a uniform
+ * neutralising background (FP64 Ewald summation, Plummer softening of the
+ * nearest image), the force of particle.cpp:141-144 (greens + greens_ewald). */
 int ct2_kick_sample_fp64(const int64_t* idx, size_t m, int periodic, double* ax, double* ay,
                          double* az, double* pot);
+
+/* ------------------------------------------------------------ particle mesh */
+/* Particle-mesh far field, replacing the multipole-mesh path of
+ * particle_compute_multipoles / particle_compute_expansions / particle_greens
+ * (particle.cpp:116-271) and its evaluation by L2P (kick.cpp:29).
+ * ng^3 grid (power of two in [8, 1024]); cloud-in-cell assignment with
+ * fixed-point weights (deterministic); FP32 FFTs; FP64 Green's table
+ *   G(k) = -4 pi GM exp(-k^2 rs^2) / (k^2 W_cic(k)^2),  rs = rs_cells/ng,
+ * i.e. the long-range part of the periodic force for rs_cells > 0. */
+int ct2_pm_init(int ng, double rs_cells);
+/* PM force and potential of every particle into the stored-force arrays
+ * (ct2_forces_download). */
+int ct2_pm_forces(void);
+/* kick() with the PM force: forces as ct2_pm_forces, then the epilogue of
+ * kick.cpp:75-99. */
+int ct2_kick_pm(const ct2_kick_params* params, unsigned flags, int* max_rung);
+/* Grids of the last PM evaluation: the CIC density (uint64, 2^-32 particle
+ * units, ng^3) and the real-space fields (which = 0: Fx + i Fy, 1: Fz + i pot;
+ * interleaved float pairs, ng^3). */
+int ct2_pm_density_download(uint64_t* host);
+int ct2_pm_grid_download(int which, float* host);
 
 /* ------------------------------------------------------------ drift */
 /* Replaces void drift(float dt, float scale) (drift.hpp:12, drift.cpp:4-18):

@@ -72,6 +72,17 @@ void orc_fft3d(double* X, int n, int dir); /* interleaved complex n^3, natural o
 void orc_fft_r2c_3d(const double* in, double* out, int N); /* fft_r2c_3d.cpp intent  */
 void orc_fft_c2r_3d(const double* in, double* out, int N); /* fft_c2r_3d.cpp intent  */
 
+/* ---- particle mesh (particle.cpp:116-271 shape, CIC assignment) ---- */
+void orc_cic_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
+                     uint64_t* grid);
+void orc_pm_green_table(int ng, double gm, double rs_cells, double* table);
+void orc_pm_forces(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
+                   double gm, double rs_cells, double* fx, double* fy, double* fz, double* pot);
+/* periodic (Ewald) direct sum with Plummer softening of the nearest image */
+void orc_ewald_kick(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, size_t m,
+                    const int64_t* sinks, double gm, double soft, double alpha, int nimg, int kmax,
+                    double* ax, double* ay, double* az, double* pot);
+
 /* ---- cosmology (cosmology.cpp:11-102), explicit parameters ---- */
 typedef struct {
     double H;         /* Hubble constant in code time units */

@@ -72,6 +72,10 @@ def _declare(L):
     L.orc_fft3d.argtypes = [_P, i, i]
     L.orc_fft_r2c_3d.argtypes = [_P, _P, i]
     L.orc_fft_c2r_3d.argtypes = [_P, _P, i]
+    L.orc_cic_deposit.argtypes = [sz, _P, _P, _P, i, _P]
+    L.orc_pm_green_table.argtypes = [i, d, d, _P]
+    L.orc_pm_forces.argtypes = [sz, _P, _P, _P, i, d, d, _P, _P, _P, _P]
+    L.orc_ewald_kick.argtypes = [sz, _P, _P, _P, sz, _P, d, d, d, i, i, _P, _P, _P, _P]
     L.orc_cosmos_update.argtypes = [C.POINTER(Cosmo), _P, _P, _P, d]
     L.orc_cosmos_dadt.argtypes = [C.POINTER(Cosmo), d]
     L.orc_cosmos_dadt.restype = d
@@ -198,6 +202,44 @@ def transpose_c2r(Y, N):
     out = np.zeros(N * N * N)
     lib().orc_transpose_c2r(_p(Y), _p(out), N)
     return out
+
+
+# ---------------------------------------------------------------- particle mesh
+def cic_deposit(x, y, z, g):
+    """CIC mass assignment in 2^-32 particle units (uint64 grid of (2^g)^3)."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    ng = 1 << g
+    grid = np.empty(ng ** 3, np.uint64)
+    lib().orc_cic_deposit(x.size, _p(x), _p(y), _p(z), g, _p(grid))
+    return grid.reshape(ng, ng, ng)
+
+
+def pm_green_table(ng, gm, rs_cells=1.25):
+    h = ng // 2 + 1
+    t = np.empty(h ** 3)
+    lib().orc_pm_green_table(ng, gm, rs_cells, _p(t))
+    return t.reshape(h, h, h)
+
+
+def pm_forces(x, y, z, g, gm, rs_cells=1.25):
+    """FP64 PM force and potential at the particles: (fx, fy, fz, pot)."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    out = [np.empty(x.size) for _ in range(4)]
+    lib().orc_pm_forces(x.size, _p(x), _p(y), _p(z), g, gm, rs_cells, *[_p(o) for o in out])
+    return tuple(out)
+
+
+def ewald_kick(x, y, z, gm, soft, sinks=None, alpha=2.5, nimg=2, kmax=5):
+    """FP64 periodic direct sum (Ewald, explicit images): (ax, ay, az, pot)."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    if sinks is None:
+        m, sp = x.size, None
+    else:
+        sinks = np.ascontiguousarray(sinks, dtype=np.int64)
+        m, sp = sinks.size, _p(sinks)
+    out = [np.empty(m) for _ in range(4)]
+    lib().orc_ewald_kick(x.size, _p(x), _p(y), _p(z), m, sp, gm, soft, alpha, nimg, kmax, *[_p(o) for o in out])
+    return tuple(out)
 
 
 # ---------------------------------------------------------------- cosmology
