@@ -67,6 +67,48 @@ def time_calls(ev: HipEvents, fn, reps: int) -> list[float]:
     return out
 
 
+def rel_err(F, J):
+    return np.linalg.norm(F - J, axis=0) / np.linalg.norm(J, axis=0)
+
+
+def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, ng: int, reps: int) -> dict:
+    """Times the PM force and the PM + learned-correction force of the current
+    particles and measures both against the periodic FP64 judge."""
+    from ct2_amd import _lib
+    from ct2_amd.train_mlp import DEFAULT_OUT, load_weights
+
+    lib = _lib.load()
+    api.pm_init(ng, 1.25)
+    out = {"pm_grid": ng, "rs_cells": 1.25, "direct_kick_ms": round(kick_ms, 3)}
+    _lib.check(lib.ct2_pm_forces())
+    pm_ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), reps)))
+    fx, fy, fz, _ = api.forces_download()
+    jx, jy, jz, _ = api.kick_sample_fp64(idx, periodic=True)
+    J = np.stack([jx, jy, jz])
+    e_pm = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J)
+    out.update({"pm_ms": round(pm_ms, 3), "pm_median_rel_err": float(np.median(e_pm)),
+                "pm_p90_rel_err": float(np.percentile(e_pm, 90))})
+    trained = load_weights(DEFAULT_OUT)
+    if trained is None:
+        out["pm_mlp"] = "no trained weights (run ct2_amd/train_mlp.py)"
+        return out
+    w, meta = trained
+    api.mlp_init(1)
+    api.mlp_set_weights(w)
+    _lib.check(lib.ct2_pm_mlp_forces())
+    ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_mlp_forces()), reps)))
+    fx, fy, fz, _ = api.forces_download()
+    e_ml = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J)
+    out.update({"pm_mlp_ms": round(ms, 3), "pm_mlp_median_rel_err": float(np.median(e_ml)),
+                "pm_mlp_p90_rel_err": float(np.percentile(e_ml, 90)),
+                "speedup_pm_mlp_vs_direct": round(kick_ms / ms, 1),
+                "mlp_flops_per_particle": 189440, "mlp_tflops": round(189440 * n / (ms - pm_ms) / 1e9, 2)
+                if ms > pm_ms else None,
+                "target_median_1e-3_met": bool(np.median(e_ml) <= 1e-3),
+                "model": {k: meta[k] for k in ("ng", "rs_cells", "kinds", "seeds", "samples_per_set", "steps")}})
+    return out
+
+
 def cpu_baseline(x, y, z, soft, gm, target_s: float):
     """FP64 oracle kick timed on the host cores on a bounded sample of sinks
     (every source), scaled to interactions/s."""
@@ -98,6 +140,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--fft-reps", type=int, default=10)
+    ap.add_argument("--reps", type=int, default=5, help="repetitions of the PM timings")
+    ap.add_argument("--pm-ng", type=int, default=256, help="PM grid of the PM + MLP leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for how "
@@ -162,6 +206,21 @@ def main() -> int:
         buf.close()
         print(f"[bench] fft {nf}^3 {mode}: {ms:.3f} ms", file=sys.stderr, flush=True)
 
+    # force accuracy of the direct kick against the FP64 direct sum (same
+    # minimum-image force) on sampled sinks
+    api.kick(None, api.KICK_STORE)
+    fx, fy, fz, _ = api.forces_download()
+    idx = np.random.default_rng(42).choice(n, 4096, replace=False)
+    jx, jy, jz, _ = api.kick_sample_fp64(idx)
+    e_direct = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), np.stack([jx, jy, jz]))
+
+    # PM and PM + learned correction on the same particles, against the
+    # periodic FP64 direct sum (Ewald) on the same sinks
+    pmres = pm_leg(api, ev, n, side, kick_avg, idx, args.pm_ng, args.reps)
+    pmres["direct_fp32_vs_fp64_min_image"] = {"median_rel_err": float(np.median(e_direct)),
+                                              "p90": float(np.percentile(e_direct, 90)),
+                                              "p99": float(np.percentile(e_direct, 99))}
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -185,6 +244,7 @@ def main() -> int:
         "fft_fp32": fft,
         "drift": {"ms": round(drift_avg, 4), "GB/s": round(36 * n / drift_avg / 1e6, 1),
                   "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},
+        "pm_mlp_vs_direct": pmres,
     }
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

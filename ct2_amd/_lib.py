@@ -71,6 +71,19 @@ _SIGS = {
     "ct2_kick_pm": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_pm_density_download": ([_P], C.c_int),
     "ct2_pm_grid_download": ([C.c_int, _P], C.c_int),
+    "ct2_mlp_num_params": ([], C.c_size_t),
+    "ct2_mlp_init": ([C.c_uint64], C.c_int),
+    "ct2_mlp_get_weights": ([_P, C.c_size_t], C.c_int),
+    "ct2_mlp_set_weights": ([_P, C.c_size_t], C.c_int),
+    "ct2_mlp_forward": ([_P, C.c_size_t, _P], C.c_int),
+    "ct2_mlp_grad": ([_P, _P, C.c_size_t, _P, _P], C.c_int),
+    "ct2_mlp_features": ([_P, C.c_size_t, _P], C.c_int),
+    "ct2_mlp_data_reset": ([C.c_size_t], C.c_int),
+    "ct2_mlp_data_size": ([], C.c_size_t),
+    "ct2_mlp_data_append": ([_P, C.c_size_t], C.c_int),
+    "ct2_mlp_train": ([C.c_int, C.c_int, C.c_float, C.c_uint64, _P], C.c_int),
+    "ct2_pm_mlp_forces": ([], C.c_int),
+    "ct2_kick_pm_mlp": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_fft3d_c2c": ([_P, C.c_int, C.c_int], C.c_int),
     "ct2_fft3d_c2c_f64": ([_P, C.c_int, C.c_int], C.c_int),
     "ct2_dev_alloc": ([C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),

@@ -168,6 +168,87 @@ def pm_grid_download(which: int) -> np.ndarray:
     return g.reshape((_pm_ng,) * 3)
 
 
+# ---------------------------------------------------------------- learned correction
+MLP_KIN = 128
+
+
+def mlp_num_params() -> int:
+    return int(L.load().ct2_mlp_num_params())
+
+
+def mlp_init(seed: int = 1) -> None:
+    L.check(L.load().ct2_mlp_init(seed))
+
+
+def mlp_get_weights() -> np.ndarray:
+    w = np.empty(mlp_num_params(), np.float32)
+    L.check(L.load().ct2_mlp_get_weights(_p(w), w.size))
+    return w
+
+
+def mlp_set_weights(w) -> None:
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    L.check(L.load().ct2_mlp_set_weights(_p(w), w.size))
+
+
+def mlp_forward(X) -> np.ndarray:
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.empty((X.shape[0], 3), np.float32)
+    L.check(L.load().ct2_mlp_forward(_p(X), X.shape[0], _p(Y)))
+    return Y
+
+
+def mlp_grad(X, T):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.empty(mlp_num_params(), np.float32)
+    loss = C.c_float()
+    L.check(L.load().ct2_mlp_grad(_p(X), _p(T), X.shape[0], _p(g), C.byref(loss)))
+    return loss.value, g
+
+
+def mlp_features(idx) -> np.ndarray:
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    X = np.empty((idx.size, MLP_KIN), np.float32)
+    L.check(L.load().ct2_mlp_features(_p(idx), idx.size, _p(X)))
+    return X
+
+
+def mlp_data_reset(capacity: int) -> None:
+    L.check(L.load().ct2_mlp_data_reset(capacity))
+
+
+def mlp_data_size() -> int:
+    return int(L.load().ct2_mlp_data_size())
+
+
+def mlp_data_append(idx) -> None:
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    for a in range(0, idx.size, 1 << 20):
+        part = np.ascontiguousarray(idx[a:a + (1 << 20)])
+        L.check(L.load().ct2_mlp_data_append(_p(part), part.size))
+
+
+def mlp_train(steps: int, batch: int = 1024, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+    losses = np.empty(steps, np.float32)
+    L.check(L.load().ct2_mlp_train(steps, batch, lr, seed, _p(losses)))
+    return losses
+
+
+def pm_mlp_forces():
+    """PM force plus the learned correction: (fx, fy, fz, pot)."""
+    L.check(L.load().ct2_pm_mlp_forces())
+    return forces_download()
+
+
+def kick_pm_mlp(params: KickParams | None = None, flags: int = 0) -> int:
+    if params is None:
+        params = kick_params()
+    mr = C.c_int(0)
+    L.check(L.load().ct2_kick_pm_mlp(C.byref(params), flags, C.byref(mr)))
+    return mr.value
+
+
 # ---------------------------------------------------------------- device buffers / FFT
 class DeviceBuffer:
     """A device allocation made through the library (freed on close/del)."""

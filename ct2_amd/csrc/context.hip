@@ -115,6 +115,7 @@ int ct2_finalize(void) {
     release_fft_plans();
     release_pm();
     release_ewald();
+    release_mlp();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device = -1;

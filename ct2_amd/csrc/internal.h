@@ -66,6 +66,7 @@ void release_kick_buffers();
 void release_fft_plans();
 void release_pm();
 void release_ewald();
+void release_mlp();
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(long v) {

@@ -1,0 +1,734 @@
+// mlp.hip -- the learned short-range force correction, BF16 MFMA on gfx950.
+//
+// No ct2 counterpart: the reference evaluates its far field with L2P
+// (kick.cpp:29) and has no learned part.  Here a small network maps local
+// PM-grid features to the force the PM misses (the periodic FP64 direct sum
+// minus the PM force):
+//   input   108 = 27 nodes (3x3x3 around the particle's nearest node) x
+//           (density / mean, PM Fx, Fy, Fz / F_unit) + 3 offsets from that
+//           node in cell units; padded to KIN = 128 with zeros
+//   hidden  256 -> GELU -> 256 -> GELU
+//   output  3 force components in units of F_unit = GM ng^2 (padded to 32)
+// Weights are FP32 masters (Adam) with BF16 copies; every product runs on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Flat weight layout:
+//   W1[128][256] b1[256] W2[256][256] b2[256] W3[256][32] b3[32]
+// (row = input, column = output).  GELU is the tanh form.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "epilogue.h"
+#include "internal.h"
+
+namespace ct2 {
+namespace {
+
+constexpr int KIN = 128, HID = 256, KOUT = 32, NOUT = 3, NFEAT = 111;
+constexpr size_t OFF_W1 = 0, OFF_B1 = OFF_W1 + KIN * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
+                 OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * KOUT, NPARAM = OFF_B3 + KOUT;
+constexpr float GELU_C = 0.7978845608028654f;
+
+typedef unsigned short bf16_t;  // storage
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline bf16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)(u >> 16);  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ __forceinline__ float gelu(float z) {
+    const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
+    return 0.5f * z * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_d(float z) {
+    const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * GELU_C * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+// ------------------------------------------------------------------ GEMM
+// C[M][N] = A[M][K] * B[K][N]; A row-major (K contiguous, lda), B given as
+// Bt[N][K] (K contiguous, ldb).  One wave per 32x32 output tile (2x2 MFMA
+// 16x16x32 tiles); 4 waves per workgroup along M.  M % 128 == 0,
+// N % 32 == 0, K % 32 == 0.
+enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_DGELU = 3 };
+
+struct EpiArgs {
+    float* f32;      // fp32 output [M][ldc] (STORE, BIAS: result; BIAS_GELU: z; DGELU: dz)
+    int ldc;
+    bf16_t* h;       // bf16 output [M][ldc] (BIAS_GELU: gelu(z); DGELU: dz)
+    bf16_t* ht;      // bf16 transposed output [N][ldt] (optional)
+    int ldt;
+    const float* bias;  // [N]
+    const float* zin;   // DGELU: saved pre-activation [M][ldc]
+};
+
+__device__ __forceinline__ bf16x8 load8(const bf16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt,
+                                                 int ldb, int M, int N, int K, EpiArgs ep) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tm = (blockIdx.x * 4 + wave) * 32, tn = blockIdx.y * 32;
+    if (tm >= M) return;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    floatx4 acc[2][2] = {};
+    const bf16_t* a0 = A + (size_t)(tm + r16) * lda + kq;
+    const bf16_t* a1 = a0 + (size_t)16 * lda;
+    const bf16_t* b0 = Bt + (size_t)(tn + r16) * ldb + kq;
+    const bf16_t* b1 = b0 + (size_t)16 * ldb;
+    for (int k = 0; k < K; k += 32) {
+        const bf16x8 fa0 = load8(a0 + k), fa1 = load8(a1 + k), fb0 = load8(b0 + k), fb1 = load8(b1 + k);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
+    }
+    // lane holds C[tm + 16 i + 4 (lane/16) + j][tn + 16 jj + lane%16]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            const int col = tn + 16 * jj + r16;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = tm + 16 * i + 4 * (lane >> 4) + j;
+                const size_t o = (size_t)row * ep.ldc + col;
+                float v = acc[i][jj][j];
+                if constexpr (EPI == EPI_STORE) {
+                    ep.f32[o] = v;
+                } else if constexpr (EPI == EPI_BIAS) {
+                    ep.f32[o] = v + ep.bias[col];
+                } else if constexpr (EPI == EPI_BIAS_GELU) {
+                    const float z = v + ep.bias[col];
+                    if (ep.f32) ep.f32[o] = z;
+                    const bf16_t hv = f2bf(gelu(z));
+                    ep.h[o] = hv;
+                    if (ep.ht) ep.ht[(size_t)col * ep.ldt + row] = hv;
+                } else {  // EPI_DGELU
+                    const float d = v * gelu_d(ep.zin[o]);
+                    ep.f32[o] = d;
+                    const bf16_t hv = f2bf(d);
+                    if (ep.h) ep.h[o] = hv;
+                    if (ep.ht) ep.ht[(size_t)col * ep.ldt + row] = hv;
+                }
+            }
+        }
+}
+
+template <int EPI>
+int gemm(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K, const EpiArgs& ep) {
+    if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
+    gemm_bf16<EPI><<<dim3(M / 128, N / 32), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, ep);
+    CT2_LAUNCH_CHECK();
+    return CT2_OK;
+}
+
+// ------------------------------------------------------------------ weights
+// bf16 copies in the layouts the GEMMs read: transposed [out][in] for the
+// forward products, natural [in][out] for the input-gradient products.
+__global__ void pack_weights(const float* __restrict__ w, bf16_t* __restrict__ W1t, bf16_t* __restrict__ W2t,
+                             bf16_t* __restrict__ W3t, bf16_t* __restrict__ W2n, bf16_t* __restrict__ W3n) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < KIN * HID) {  // W1[k][n] -> W1t[n][k]
+        const int k = t / HID, n = t % HID;
+        W1t[n * KIN + k] = f2bf(w[OFF_W1 + t]);
+    }
+    if (t < HID * HID) {
+        const int k = t / HID, n = t % HID;
+        const bf16_t v = f2bf(w[OFF_W2 + t]);
+        W2t[n * HID + k] = v;
+        W2n[t] = v;
+    }
+    if (t < HID * KOUT) {
+        const int k = t / KOUT, n = t % KOUT;
+        const bf16_t v = f2bf(w[OFF_W3 + t]);
+        W3t[n * HID + k] = v;
+        W3n[t] = v;
+    }
+}
+
+__global__ void adam_step(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                          const float* __restrict__ g, int n, float lr, float b1, float b2, float c1, float c2, float eps) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ------------------------------------------------------------------ features
+struct FeatGrid {
+    int g;
+    const unsigned long long* W;
+    const float2* A;  // Fx + i Fy
+    const float2* B;  // Fz + i pot
+    float dscale;     // ng^3 / N: density in units of the mean
+    float fscale;     // 1 / F_unit
+};
+
+__device__ __forceinline__ void gather_features(uint32_t x, uint32_t y, uint32_t z, const FeatGrid& fg, float* f) {
+    const int s = 32 - fg.g;
+    const uint32_t ng = 1u << fg.g, mask = ng - 1, half = 1u << (s - 1);
+    const uint32_t ic = ((x + half) >> s) & mask, jc = ((y + half) >> s) & mask, kc = ((z + half) >> s) & mask;
+    const float inv = 1.0f / (float)(1u << s);
+    f[108] = (float)(int32_t)(x - (ic << s)) * inv;
+    f[109] = (float)(int32_t)(y - (jc << s)) * inv;
+    f[110] = (float)(int32_t)(z - (kc << s)) * inv;
+    int q = 0;
+    for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b) {
+            const size_t row = ((size_t)((ic + a) & mask) * ng + ((jc + b) & mask)) * ng;
+            for (int c = -1; c <= 1; ++c, ++q) {
+                const size_t cc = row + ((kc + c) & mask);
+                const float2 av = fg.A[cc], bv = fg.B[cc];
+                f[4 * q + 0] = (float)((double)fg.W[cc] * 0x1p-32) * fg.dscale;
+                f[4 * q + 1] = av.x * fg.fscale;
+                f[4 * q + 2] = av.y * fg.fscale;
+                f[4 * q + 3] = bv.x * fg.fscale;
+            }
+        }
+}
+
+// X[r][128] (bf16, optional XT[128][ldt] and fp32 copy) for particles idx[r] (or r0 + r)
+__global__ __launch_bounds__(256) void features_kernel(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                       const uint32_t* __restrict__ Z, const long* __restrict__ idx,
+                                                       long r0, int m, FeatGrid fg, bf16_t* __restrict__ out,
+                                                       float* __restrict__ outf) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    const long p = idx ? idx[r] : r0 + r;
+    float f[KIN];
+    gather_features(X[p], Y[p], Z[p], fg, f);
+#pragma unroll
+    for (int k = NFEAT; k < KIN; ++k) f[k] = 0.0f;
+    uint4* o = (uint4*)(out + (size_t)r * KIN);
+#pragma unroll
+    for (int v = 0; v < KIN / 8; ++v) {
+        uint4 w;
+        w.x = (uint32_t)f2bf(f[8 * v + 0]) | ((uint32_t)f2bf(f[8 * v + 1]) << 16);
+        w.y = (uint32_t)f2bf(f[8 * v + 2]) | ((uint32_t)f2bf(f[8 * v + 3]) << 16);
+        w.z = (uint32_t)f2bf(f[8 * v + 4]) | ((uint32_t)f2bf(f[8 * v + 5]) << 16);
+        w.w = (uint32_t)f2bf(f[8 * v + 6]) | ((uint32_t)f2bf(f[8 * v + 7]) << 16);
+        o[v] = w;
+    }
+    if (outf)
+        for (int k = 0; k < KIN; ++k) outf[(size_t)r * KIN + k] = f[k];
+}
+
+// ------------------------------------------------------------------ training data
+// target = (F_judge - F_pm) / F_unit; column 3 = the sample's loss weight
+// 1 / (|F_judge| / F_unit)^2, so that the loss measures relative force error
+__global__ void targets_kernel(const double* __restrict__ judge, const long* __restrict__ idx, int m,
+                               const float* __restrict__ fx, const float* __restrict__ fy, const float* __restrict__ fz,
+                               double inv_unit, float* __restrict__ T) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    const long p = idx[r];
+    const double jx = judge[r], jy = judge[(long)m + r], jz = judge[2L * m + r];
+    T[4 * (size_t)r + 0] = (float)((jx - fx[p]) * inv_unit);
+    T[4 * (size_t)r + 1] = (float)((jy - fy[p]) * inv_unit);
+    T[4 * (size_t)r + 2] = (float)((jz - fz[p]) * inv_unit);
+    const double f2 = (jx * jx + jy * jy + jz * jz) * inv_unit * inv_unit;
+    T[4 * (size_t)r + 3] = (float)(1.0 / (f2 + 1e-12));
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Batch of M rows drawn uniformly (with replacement) from the data set:
+// X[M][128], XT[128][M], T[M][4].
+__global__ void batch_kernel(const bf16_t* __restrict__ DX, const float* __restrict__ DT, long size, uint64_t key, int M,
+                             bf16_t* __restrict__ X, bf16_t* __restrict__ XT, float* __restrict__ T) {
+    const int r = blockIdx.x;
+    const long p = (long)(mix64(key + (uint64_t)r) % (uint64_t)size);
+    const int k = threadIdx.x;  // 128 lanes
+    const bf16_t v = DX[(size_t)p * KIN + k];
+    X[(size_t)r * KIN + k] = v;
+    XT[(size_t)k * M + r] = v;
+    if (k < 4) T[4 * (size_t)r + k] = DT[4 * (size_t)p + k];
+}
+
+// dY = 2 w (Y - T) / M on the first 3 outputs (w = T[r][3]), 0 elsewhere;
+// per-block partials of sum w |Y - T|^2
+__global__ void loss_kernel(const float* __restrict__ Yo, const float* __restrict__ T, int M, float* __restrict__ dYf,
+                            bf16_t* __restrict__ dY, bf16_t* __restrict__ dYT, float* __restrict__ partial) {
+    __shared__ float red[4];
+    const int r = blockIdx.x * 8 + (threadIdx.x >> 5), o = threadIdx.x & 31;
+    float e2 = 0.0f;
+    if (r < M) {
+        float d = 0.0f;
+        if (o < NOUT) {
+            const float wr = T[4 * (size_t)r + 3];
+            const float e = Yo[(size_t)r * KOUT + o] - T[4 * (size_t)r + o];
+            e2 = wr * e * e;
+            d = 2.0f * wr * e / (float)M;
+        }
+        dYf[(size_t)r * KOUT + o] = d;
+        const bf16_t h = f2bf(d);
+        dY[(size_t)r * KOUT + o] = h;
+        dYT[(size_t)o * M + r] = h;
+    }
+    for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e2;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// db[n] = sum_r D[r][n] (fixed order)
+__global__ void colsum_kernel(const float* __restrict__ D, int M, int N, float* __restrict__ db) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int r = 0; r < M; ++r) s += D[(size_t)r * N + n];
+    db[n] = s;
+}
+
+// F[p] += F_unit * Y[r][0..2] for p = r0 + r
+__global__ void apply_kernel(const float* __restrict__ Yo, long r0, int m, float unit, float* __restrict__ fx,
+                             float* __restrict__ fy, float* __restrict__ fz) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    const long p = r0 + r;
+    fx[p] += unit * Yo[(size_t)r * KOUT + 0];
+    fy[p] += unit * Yo[(size_t)r * KOUT + 1];
+    fz[p] += unit * Yo[(size_t)r * KOUT + 2];
+}
+
+// ------------------------------------------------------------------ state
+struct Mlp {
+    bool ready = false;
+    float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    bf16_t *W1t = nullptr, *W2t = nullptr, *W3t = nullptr, *W2n = nullptr, *W3n = nullptr;
+    long adam_t = 0;
+    // work buffers for up to cap rows
+    int cap = 0;
+    bf16_t *X = nullptr, *XT = nullptr, *H1 = nullptr, *H1T = nullptr, *H2 = nullptr, *H2T = nullptr;
+    bf16_t *dY = nullptr, *dYT = nullptr, *dZ2 = nullptr, *dZ2T = nullptr, *dZ1T = nullptr;
+    float *Z1 = nullptr, *Z2 = nullptr, *Yo = nullptr, *T = nullptr, *dYf = nullptr, *dZ2f = nullptr, *dZ1f = nullptr;
+    float* lpart = nullptr;
+    float* Xf = nullptr;  // fp32 features for host download
+    long* idx = nullptr;
+    double* judge = nullptr;
+    // training set
+    long dcap = 0, dsize = 0;
+    bf16_t* DX = nullptr;
+    float* DT = nullptr;
+} g_mlp;
+
+void free_work() {
+    for (void* p : {(void*)g_mlp.X, (void*)g_mlp.XT, (void*)g_mlp.H1, (void*)g_mlp.H1T, (void*)g_mlp.H2, (void*)g_mlp.H2T,
+                    (void*)g_mlp.dY, (void*)g_mlp.dYT, (void*)g_mlp.dZ2, (void*)g_mlp.dZ2T, (void*)g_mlp.dZ1T,
+                    (void*)g_mlp.Z1, (void*)g_mlp.Z2, (void*)g_mlp.Yo, (void*)g_mlp.T, (void*)g_mlp.dYf,
+                    (void*)g_mlp.dZ2f, (void*)g_mlp.dZ1f, (void*)g_mlp.lpart, (void*)g_mlp.Xf, (void*)g_mlp.idx,
+                    (void*)g_mlp.judge})
+        dev_free(p);
+    g_mlp.X = g_mlp.XT = g_mlp.H1 = g_mlp.H1T = g_mlp.H2 = g_mlp.H2T = nullptr;
+    g_mlp.dY = g_mlp.dYT = g_mlp.dZ2 = g_mlp.dZ2T = g_mlp.dZ1T = nullptr;
+    g_mlp.Z1 = g_mlp.Z2 = g_mlp.Yo = g_mlp.T = g_mlp.dYf = g_mlp.dZ2f = g_mlp.dZ1f = g_mlp.lpart = g_mlp.Xf = nullptr;
+    g_mlp.idx = nullptr;
+    g_mlp.judge = nullptr;
+    g_mlp.cap = 0;
+}
+
+// Work buffers for `rows` rows (rounded up to 128); training buffers only when asked.
+int ensure_work(int rows, bool training) {
+    rows = (rows + 127) / 128 * 128;
+    const bool have_train = g_mlp.Z1 != nullptr;
+    if (g_mlp.cap >= rows && (!training || have_train)) return CT2_OK;
+    const int cap = std::max(rows, g_mlp.cap);
+    free_work();
+    const size_t R = (size_t)cap;
+    int rc;
+#define ALLOC(p, bytes, what) \
+    if ((rc = dev_alloc((void**)&(p), (bytes), what))) { free_work(); return rc; }
+    ALLOC(g_mlp.X, R * KIN * 2, "mlp X");
+    ALLOC(g_mlp.H1, R * HID * 2, "mlp H1");
+    ALLOC(g_mlp.H2, R * HID * 2, "mlp H2");
+    ALLOC(g_mlp.Yo, R * KOUT * 4, "mlp Y");
+    ALLOC(g_mlp.idx, R * sizeof(long), "mlp index");
+    if (training) {
+        ALLOC(g_mlp.XT, R * KIN * 2, "mlp XT");
+        ALLOC(g_mlp.H1T, R * HID * 2, "mlp H1T");
+        ALLOC(g_mlp.H2T, R * HID * 2, "mlp H2T");
+        ALLOC(g_mlp.dY, R * KOUT * 2, "mlp dY");
+        ALLOC(g_mlp.dYT, R * KOUT * 2, "mlp dYT");
+        ALLOC(g_mlp.dZ2, R * HID * 2, "mlp dZ2");
+        ALLOC(g_mlp.dZ2T, R * HID * 2, "mlp dZ2T");
+        ALLOC(g_mlp.dZ1T, R * HID * 2, "mlp dZ1T");
+        ALLOC(g_mlp.Z1, R * HID * 4, "mlp Z1");
+        ALLOC(g_mlp.Z2, R * HID * 4, "mlp Z2");
+        ALLOC(g_mlp.T, R * 4 * 4, "mlp T");
+        ALLOC(g_mlp.dYf, R * KOUT * 4, "mlp dYf");
+        ALLOC(g_mlp.dZ2f, R * HID * 4, "mlp dZ2f");
+        ALLOC(g_mlp.dZ1f, R * HID * 4, "mlp dZ1f");
+        ALLOC(g_mlp.lpart, (R / 8 + 1) * 4, "mlp loss");
+        ALLOC(g_mlp.Xf, R * KIN * 4, "mlp X fp32");
+        ALLOC(g_mlp.judge, R * 4 * 8, "mlp judge");
+    }
+#undef ALLOC
+    g_mlp.cap = cap;
+    return CT2_OK;
+}
+
+int require_mlp() {
+    CT2_TRY(require_init());
+    if (!g_mlp.ready) return set_error(CT2_ERR_STATE, "ct2_mlp_init has not been called");
+    return CT2_OK;
+}
+
+int repack(hipStream_t st) {
+    pack_weights<<<(HID * HID + 255) / 256, 256, 0, st>>>(g_mlp.w, g_mlp.W1t, g_mlp.W2t, g_mlp.W3t, g_mlp.W2n, g_mlp.W3n);
+    CT2_LAUNCH_CHECK();
+    return CT2_OK;
+}
+
+// Forward of M rows already in g_mlp.X; keeps Z/H for training when train.
+int forward(hipStream_t st, int M, bool train) {
+    EpiArgs e1{train ? g_mlp.Z1 : nullptr, HID, g_mlp.H1, train ? g_mlp.H1T : nullptr, M, g_mlp.w + OFF_B1, nullptr};
+    CT2_TRY(gemm<EPI_BIAS_GELU>(st, g_mlp.X, KIN, g_mlp.W1t, KIN, M, HID, KIN, e1));
+    EpiArgs e2{train ? g_mlp.Z2 : nullptr, HID, g_mlp.H2, train ? g_mlp.H2T : nullptr, M, g_mlp.w + OFF_B2, nullptr};
+    CT2_TRY(gemm<EPI_BIAS_GELU>(st, g_mlp.H1, HID, g_mlp.W2t, HID, M, HID, HID, e2));
+    EpiArgs e3{g_mlp.Yo, KOUT, nullptr, nullptr, 0, g_mlp.w + OFF_B3, nullptr};
+    CT2_TRY(gemm<EPI_BIAS>(st, g_mlp.H2, HID, g_mlp.W3t, HID, M, KOUT, HID, e3));
+    return CT2_OK;
+}
+
+// Backward of the MSE loss for M rows with targets in g_mlp.T; gradient
+// into g_mlp.grad, per-block loss partials into g_mlp.lpart.
+int backward(hipStream_t st, int M) {
+    float* g = g_mlp.grad;
+    loss_kernel<<<(M + 7) / 8, 256, 0, st>>>(g_mlp.Yo, g_mlp.T, M, g_mlp.dYf, g_mlp.dY, g_mlp.dYT, g_mlp.lpart);
+    CT2_LAUNCH_CHECK();
+    // dW3 = H2^T dY, db3
+    EpiArgs s3{g + OFF_W3, KOUT, nullptr, nullptr, 0, nullptr, nullptr};
+    CT2_TRY(gemm<EPI_STORE>(st, g_mlp.H2T, M, g_mlp.dYT, M, HID, KOUT, M, s3));
+    colsum_kernel<<<1, 256, 0, st>>>(g_mlp.dYf, M, KOUT, g + OFF_B3);
+    CT2_LAUNCH_CHECK();
+    // dZ2 = (dY W3^T) * gelu'(Z2)
+    EpiArgs d2{g_mlp.dZ2f, HID, g_mlp.dZ2, g_mlp.dZ2T, M, nullptr, g_mlp.Z2};
+    CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dY, KOUT, g_mlp.W3n, KOUT, M, HID, KOUT, d2));
+    // dW2 = H1^T dZ2, db2
+    EpiArgs s2{g + OFF_W2, HID, nullptr, nullptr, 0, nullptr, nullptr};
+    CT2_TRY(gemm<EPI_STORE>(st, g_mlp.H1T, M, g_mlp.dZ2T, M, HID, HID, M, s2));
+    colsum_kernel<<<1, 256, 0, st>>>(g_mlp.dZ2f, M, HID, g + OFF_B2);
+    CT2_LAUNCH_CHECK();
+    // dZ1 = (dZ2 W2^T) * gelu'(Z1)
+    EpiArgs d1{g_mlp.dZ1f, HID, nullptr, g_mlp.dZ1T, M, nullptr, g_mlp.Z1};
+    CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dZ2, HID, g_mlp.W2n, HID, M, HID, HID, d1));
+    // dW1 = X^T dZ1, db1
+    EpiArgs s1{g + OFF_W1, HID, nullptr, nullptr, 0, nullptr, nullptr};
+    CT2_TRY(gemm<EPI_STORE>(st, g_mlp.XT, M, g_mlp.dZ1T, M, KIN, HID, M, s1));
+    colsum_kernel<<<1, 256, 0, st>>>(g_mlp.dZ1f, M, HID, g + OFF_B1);
+    CT2_LAUNCH_CHECK();
+    return CT2_OK;
+}
+
+FeatGrid feat_grid() {
+    Context& c = ctx();
+    FeatGrid fg;
+    pm_grids(&fg.g, &fg.W, &fg.A, &fg.B);
+    const double ng = (double)(1L << fg.g);
+    fg.dscale = (float)(ng * ng * ng / (double)c.n);
+    fg.fscale = (float)(1.0 / ((double)c.opts.GM * ng * ng));
+    return fg;
+}
+
+double force_unit() {
+    int g;
+    const unsigned long long* W;
+    const float2 *A, *B;
+    pm_grids(&g, &W, &A, &B);
+    const double ng = (double)(1L << g);
+    return (double)ctx().opts.GM * ng * ng;
+}
+
+}  // namespace
+
+// PM force plus the learned correction for every particle, in chunks.
+int mlp_correct_forces() {
+    CT2_TRY(require_mlp());
+    Context& c = ctx();
+    hipStream_t st = c.stream;
+    const long n = (long)c.n;
+    const int chunk = (int)std::min<long>(n, 1L << 22);
+    CT2_TRY(ensure_work(chunk, false));
+    const FeatGrid fg = feat_grid();
+    const float unit = (float)force_unit();
+    for (long r0 = 0; r0 < n; r0 += chunk) {
+        const int m = (int)std::min<long>(chunk, n - r0);
+        const int M = (m + 127) / 128 * 128;
+        features_kernel<<<(m + 255) / 256, 256, 0, st>>>(c.x, c.y, c.z, nullptr, r0, m, fg, g_mlp.X, nullptr);
+        CT2_LAUNCH_CHECK();
+        if (M > m) CT2_HIP(hipMemsetAsync(g_mlp.X + (size_t)m * KIN, 0, (size_t)(M - m) * KIN * 2, st));
+        CT2_TRY(forward(st, M, false));
+        apply_kernel<<<(m + 255) / 256, 256, 0, st>>>(g_mlp.Yo, r0, m, unit, c.fx, c.fy, c.fz);
+        CT2_LAUNCH_CHECK();
+    }
+    return CT2_OK;
+}
+
+void release_mlp() {
+    free_work();
+    for (void* p : {(void*)g_mlp.w, (void*)g_mlp.m, (void*)g_mlp.v, (void*)g_mlp.grad, (void*)g_mlp.W1t,
+                    (void*)g_mlp.W2t, (void*)g_mlp.W3t, (void*)g_mlp.W2n, (void*)g_mlp.W3n, (void*)g_mlp.DX,
+                    (void*)g_mlp.DT})
+        dev_free(p);
+    g_mlp = Mlp{};
+}
+
+}  // namespace ct2
+
+using namespace ct2;
+
+extern "C" size_t ct2_mlp_num_params(void) { return NPARAM; }
+
+extern "C" int ct2_mlp_init(uint64_t seed) {
+    CT2_TRY(require_init());
+    release_mlp();
+    int rc;
+    const size_t P = NPARAM;
+    if ((rc = dev_alloc((void**)&g_mlp.w, P * 4, "mlp weights")) || (rc = dev_alloc((void**)&g_mlp.m, P * 4, "adam m")) ||
+        (rc = dev_alloc((void**)&g_mlp.v, P * 4, "adam v")) || (rc = dev_alloc((void**)&g_mlp.grad, P * 4, "mlp grad")) ||
+        (rc = dev_alloc((void**)&g_mlp.W1t, KIN * HID * 2, "W1t")) ||
+        (rc = dev_alloc((void**)&g_mlp.W2t, HID * HID * 2, "W2t")) ||
+        (rc = dev_alloc((void**)&g_mlp.W3t, HID * KOUT * 2, "W3t")) ||
+        (rc = dev_alloc((void**)&g_mlp.W2n, HID * HID * 2, "W2n")) ||
+        (rc = dev_alloc((void**)&g_mlp.W3n, HID * KOUT * 2, "W3n"))) {
+        release_mlp();
+        return rc;
+    }
+    // Glorot-uniform weights on the used rows/columns, zero padding and biases
+    std::vector<float> w(P, 0.0f);
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto uni = [&]() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return ((double)(z >> 11) + 0.5) * 0x1p-53;
+    };
+    auto fill = [&](size_t off, int rows, int cols, int used_rows, int used_cols) {
+        const double lim = std::sqrt(6.0 / (used_rows + used_cols));
+        for (int r = 0; r < used_rows; ++r)
+            for (int c = 0; c < used_cols; ++c) w[off + (size_t)r * cols + c] = (float)((2.0 * uni() - 1.0) * lim);
+        (void)rows;
+    };
+    fill(OFF_W1, KIN, HID, NFEAT, HID);
+    fill(OFF_W2, HID, HID, HID, HID);
+    fill(OFF_W3, HID, KOUT, HID, NOUT);
+    Context& c = ctx();
+    CT2_HIP(hipMemcpyAsync(g_mlp.w, w.data(), P * 4, hipMemcpyHostToDevice, c.stream));
+    CT2_HIP(hipMemsetAsync(g_mlp.m, 0, P * 4, c.stream));
+    CT2_HIP(hipMemsetAsync(g_mlp.v, 0, P * 4, c.stream));
+    CT2_TRY(repack(c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    g_mlp.adam_t = 0;
+    g_mlp.ready = true;
+    return CT2_OK;
+}
+
+extern "C" int ct2_mlp_get_weights(float* w, size_t n) {
+    CT2_TRY(require_mlp());
+    if (!w || n != NPARAM) return set_error(CT2_ERR_INVALID, "weight buffer must hold %zu floats", NPARAM);
+    Context& c = ctx();
+    CT2_HIP(hipMemcpyAsync(w, g_mlp.w, NPARAM * 4, hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    return CT2_OK;
+}
+
+extern "C" int ct2_mlp_set_weights(const float* w, size_t n) {
+    CT2_TRY(require_mlp());
+    if (!w || n != NPARAM) return set_error(CT2_ERR_INVALID, "weight buffer must hold %zu floats", NPARAM);
+    Context& c = ctx();
+    CT2_HIP(hipMemcpyAsync(g_mlp.w, w, NPARAM * 4, hipMemcpyHostToDevice, c.stream));
+    CT2_HIP(hipMemsetAsync(g_mlp.m, 0, NPARAM * 4, c.stream));
+    CT2_HIP(hipMemsetAsync(g_mlp.v, 0, NPARAM * 4, c.stream));
+    CT2_TRY(repack(c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    g_mlp.adam_t = 0;
+    return CT2_OK;
+}
+
+// Host features -> outputs (the GPU forward on given inputs; for tests).
+extern "C" int ct2_mlp_forward(const float* x, size_t m, float* y) {
+    CT2_TRY(require_mlp());
+    if (!x || !y || m == 0 || m > (1u << 24)) return set_error(CT2_ERR_INVALID, "bad forward request");
+    CT2_TRY(ensure_work((int)m, false));
+    Context& c = ctx();
+    const int M = ((int)m + 127) / 128 * 128;
+    std::vector<bf16_t> hx((size_t)M * KIN, 0);
+    for (size_t i = 0; i < m * KIN; ++i) hx[i] = f2bf(x[i]);
+    CT2_HIP(hipMemcpyAsync(g_mlp.X, hx.data(), hx.size() * 2, hipMemcpyHostToDevice, c.stream));
+    CT2_TRY(forward(c.stream, M, false));
+    std::vector<float> hy((size_t)M * KOUT);
+    CT2_HIP(hipMemcpyAsync(hy.data(), g_mlp.Yo, hy.size() * 4, hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    for (size_t r = 0; r < m; ++r)
+        for (int o = 0; o < NOUT; ++o) y[r * NOUT + o] = hy[r * KOUT + o];
+    return CT2_OK;
+}
+
+// Loss and gradient over m given samples (no update); t[m][4] holds the
+// three targets and the sample weight.
+extern "C" int ct2_mlp_grad(const float* x, const float* t, size_t m, float* grad, float* loss) {
+    CT2_TRY(require_mlp());
+    if (!x || !t || !grad || m == 0 || m % 128 || m > (1u << 20))
+        return set_error(CT2_ERR_INVALID, "gradient batch must be a positive multiple of 128 (<= 2^20)");
+    CT2_TRY(ensure_work((int)m, true));
+    Context& c = ctx();
+    const int M = (int)m;
+    std::vector<bf16_t> hx((size_t)M * KIN), hxt((size_t)M * KIN);
+    for (int r = 0; r < M; ++r)
+        for (int k = 0; k < KIN; ++k) {
+            const bf16_t v = f2bf(x[(size_t)r * KIN + k]);
+            hx[(size_t)r * KIN + k] = v;
+            hxt[(size_t)k * M + r] = v;
+        }
+    std::vector<float> ht(t, t + (size_t)M * 4);
+    CT2_HIP(hipMemcpyAsync(g_mlp.X, hx.data(), hx.size() * 2, hipMemcpyHostToDevice, c.stream));
+    CT2_HIP(hipMemcpyAsync(g_mlp.XT, hxt.data(), hxt.size() * 2, hipMemcpyHostToDevice, c.stream));
+    CT2_HIP(hipMemcpyAsync(g_mlp.T, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, c.stream));
+    CT2_TRY(forward(c.stream, M, true));
+    CT2_TRY(backward(c.stream, M));
+    std::vector<float> lp((M + 7) / 8);
+    CT2_HIP(hipMemcpyAsync(grad, g_mlp.grad, NPARAM * 4, hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipMemcpyAsync(lp.data(), g_mlp.lpart, lp.size() * 4, hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    double L = 0.0;
+    for (float v : lp) L += v;
+    if (loss) *loss = (float)(L / M);
+    return CT2_OK;
+}
+
+// Features of sampled particles from the last PM evaluation (fp32, [m][128]).
+extern "C" int ct2_mlp_features(const int64_t* idx, size_t m, float* out) {
+    CT2_TRY(require_mlp());
+    CT2_TRY(pm_require());
+    Context& c = ctx();
+    if (!idx || !out || m == 0 || m > (1u << 22)) return set_error(CT2_ERR_INVALID, "bad feature request");
+    for (size_t s = 0; s < m; ++s)
+        if (idx[s] < 0 || (size_t)idx[s] >= c.n) return set_error(CT2_ERR_INVALID, "index out of range");
+    CT2_TRY(ensure_work((int)m, true));
+    CT2_HIP(hipMemcpyAsync(g_mlp.idx, idx, m * 8, hipMemcpyHostToDevice, c.stream));
+    features_kernel<<<((int)m + 255) / 256, 256, 0, c.stream>>>(c.x, c.y, c.z, g_mlp.idx, 0, (int)m, feat_grid(), g_mlp.X,
+                                                                g_mlp.Xf);
+    CT2_LAUNCH_CHECK();
+    CT2_HIP(hipMemcpyAsync(out, g_mlp.Xf, m * KIN * 4, hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    return CT2_OK;
+}
+
+extern "C" int ct2_mlp_data_reset(size_t capacity) {
+    CT2_TRY(require_mlp());
+    if (capacity == 0 || capacity > (1ul << 27)) return set_error(CT2_ERR_INVALID, "bad data capacity");
+    dev_free(g_mlp.DX);
+    dev_free(g_mlp.DT);
+    g_mlp.DX = nullptr;
+    g_mlp.DT = nullptr;
+    g_mlp.dcap = g_mlp.dsize = 0;
+    int rc;
+    if ((rc = dev_alloc((void**)&g_mlp.DX, capacity * KIN * 2, "training features")) ||
+        (rc = dev_alloc((void**)&g_mlp.DT, capacity * 16, "training targets")))
+        return rc;
+    g_mlp.dcap = (long)capacity;
+    return CT2_OK;
+}
+
+extern "C" size_t ct2_mlp_data_size(void) { return (size_t)g_mlp.dsize; }
+
+// Training samples for particles idx: features of the current PM state and
+// targets (periodic FP64 direct sum - PM force) / F_unit.  Runs the PM
+// force first.
+extern "C" int ct2_mlp_data_append(const int64_t* idx, size_t m) {
+    CT2_TRY(require_mlp());
+    CT2_TRY(pm_require());
+    Context& c = ctx();
+    if (!g_mlp.DX) return set_error(CT2_ERR_STATE, "ct2_mlp_data_reset has not been called");
+    if (!idx || m == 0 || m > (1u << 20)) return set_error(CT2_ERR_INVALID, "append 1..2^20 samples at a time");
+    if (g_mlp.dsize + (long)m > g_mlp.dcap) return set_error(CT2_ERR_INVALID, "training set capacity exceeded");
+    for (size_t s = 0; s < m; ++s)
+        if (idx[s] < 0 || (size_t)idx[s] >= c.n) return set_error(CT2_ERR_INVALID, "index out of range");
+    CT2_TRY(ensure_work((int)m, true));
+    CT2_TRY(pm_forces());
+    CT2_HIP(hipMemcpyAsync(g_mlp.idx, idx, m * 8, hipMemcpyHostToDevice, c.stream));
+    CT2_TRY(ewald_judge(g_mlp.idx, (int)m, g_mlp.judge));
+    const long off = g_mlp.dsize;
+    features_kernel<<<((int)m + 255) / 256, 256, 0, c.stream>>>(c.x, c.y, c.z, g_mlp.idx, 0, (int)m, feat_grid(),
+                                                                g_mlp.DX + (size_t)off * KIN, nullptr);
+    CT2_LAUNCH_CHECK();
+    targets_kernel<<<((int)m + 255) / 256, 256, 0, c.stream>>>(g_mlp.judge, g_mlp.idx, (int)m, c.fx, c.fy, c.fz,
+                                                               1.0 / force_unit(), g_mlp.DT + (size_t)off * 4);
+    CT2_LAUNCH_CHECK();
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    g_mlp.dsize += (long)m;
+    return CT2_OK;
+}
+
+// `steps` Adam steps on batches of `batch` samples drawn from the training set.
+extern "C" int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, float* losses) {
+    CT2_TRY(require_mlp());
+    if (g_mlp.dsize == 0) return set_error(CT2_ERR_STATE, "empty training set");
+    if (steps <= 0 || batch <= 0 || batch % 128 || batch > (1 << 20) || !(lr > 0.0f))
+        return set_error(CT2_ERR_INVALID, "bad training parameters");
+    CT2_TRY(ensure_work(batch, true));
+    Context& c = ctx();
+    hipStream_t st = c.stream;
+    const float b1 = 0.9f, b2 = 0.999f;
+    std::vector<float> lp((batch + 7) / 8);
+    for (int s = 0; s < steps; ++s) {
+        const uint64_t key = (seed * 0xD1B54A32D192ED03ull) ^ ((uint64_t)s * 0x632BE59BD9B4E019ull);
+        batch_kernel<<<batch, KIN, 0, st>>>(g_mlp.DX, g_mlp.DT, g_mlp.dsize, key, batch, g_mlp.X, g_mlp.XT, g_mlp.T);
+        CT2_LAUNCH_CHECK();
+        CT2_TRY(forward(st, batch, true));
+        CT2_TRY(backward(st, batch));
+        ++g_mlp.adam_t;
+        const float c1 = 1.0f - std::pow(b1, (float)g_mlp.adam_t), c2 = 1.0f - std::pow(b2, (float)g_mlp.adam_t);
+        adam_step<<<(int)((NPARAM + 255) / 256), 256, 0, st>>>(g_mlp.w, g_mlp.m, g_mlp.v, g_mlp.grad, (int)NPARAM, lr, b1,
+                                                               b2, c1, c2, 1e-8f);
+        CT2_LAUNCH_CHECK();
+        CT2_TRY(repack(st));
+        if (losses) {
+            CT2_HIP(hipMemcpyAsync(lp.data(), g_mlp.lpart, lp.size() * 4, hipMemcpyDeviceToHost, st));
+            CT2_HIP(hipStreamSynchronize(st));
+            double L = 0.0;
+            for (float v : lp) L += v;
+            losses[s] = (float)(L / batch);
+        }
+    }
+    CT2_HIP(hipStreamSynchronize(st));
+    return CT2_OK;
+}
+
+// PM force plus the learned correction into the stored-force arrays.
+extern "C" int ct2_pm_mlp_forces(void) {
+    CT2_TRY(pm_forces());
+    return mlp_correct_forces();
+}
+
+extern "C" int ct2_kick_pm_mlp(const ct2_kick_params* p, unsigned flags, int* max_rung) {
+    CT2_TRY(pm_require());
+    CT2_TRY(require_mlp());
+    CT2_TRY(check_kick_params(p));
+    if (flags & ~(unsigned)CT2_KICK_STORE) return set_error(CT2_ERR_INVALID, "unknown flags 0x%x", flags);
+    CT2_TRY(pm_forces());
+    CT2_TRY(mlp_correct_forces());
+    return run_epilogue(p, max_rung);
+}

@@ -140,6 +140,39 @@ int ct2_kick_pm(const ct2_kick_params* params, unsigned flags, int* max_rung);
 int ct2_pm_density_download(uint64_t* host);
 int ct2_pm_grid_download(int which, float* host);
 
+/* ------------------------------------------------------------ learned correction */
+/* No ct2 counterpart (the reference's far field is L2P, kick.cpp:29).  A
+ * network 111 -> 256 -> 256 -> 3 (GELU, BF16 weights, FP32 accumulation on
+ * MFMA) maps local PM-grid features (27 nodes around the particle's nearest
+ * node x (density/mean, PM force/F_unit) + 3 offsets, F_unit = GM ng^2) to the
+ * force the PM misses.  Flat FP32 weight layout (padded):
+ *   W1[128][256] b1[256] W2[256][256] b2[256] W3[256][32] b3[32]. */
+size_t ct2_mlp_num_params(void);
+int ct2_mlp_init(uint64_t seed);
+int ct2_mlp_get_weights(float* w, size_t n);
+int ct2_mlp_set_weights(const float* w, size_t n);
+/* GPU forward of m given feature rows (x[m][128] -> y[m][3]) and the loss
+ * L = sum_r t[r][3] |y_r - t[r][0..2]|^2 / m with its gradient (t is m x 4,
+ * m a multiple of 128): the kernels the training and the inference use,
+ * exposed for verification.  Training sets weight each sample by
+ * 1/|F|^2 (relative force error). */
+int ct2_mlp_forward(const float* x, size_t m, float* y);
+int ct2_mlp_grad(const float* x, const float* t, size_t m, float* grad, float* loss);
+/* Features of sampled particles from the last PM evaluation ([m][128]). */
+int ct2_mlp_features(const int64_t* idx, size_t m, float* out);
+/* Training set on the device: ct2_mlp_data_append runs the PM force and the
+ * periodic FP64 judge for the sampled particles and stores their features
+ * and targets (judge - PM) / F_unit. */
+int ct2_mlp_data_reset(size_t capacity);
+size_t ct2_mlp_data_size(void);
+int ct2_mlp_data_append(const int64_t* idx, size_t m);
+/* `steps` Adam steps on random batches; losses[steps] optional. */
+int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, float* losses);
+/* PM force plus the learned correction into the stored-force arrays, and the
+ * kick() built on it (epilogue of kick.cpp:75-99). */
+int ct2_pm_mlp_forces(void);
+int ct2_kick_pm_mlp(const ct2_kick_params* params, unsigned flags, int* max_rung);
+
 /* ------------------------------------------------------------ drift */
 /* Replaces void drift(float dt, float scale) (drift.hpp:12, drift.cpp:4-18):
  * x += fixed32(v * (dt/scale)) per axis, with fixed32(s) defined as

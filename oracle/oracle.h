@@ -83,6 +83,13 @@ void orc_ewald_kick(size_t n, const uint32_t* x, const uint32_t* y, const uint32
                     const int64_t* sinks, double gm, double soft, double alpha, int nimg, int kmax,
                     double* ax, double* ay, double* az, double* pot);
 
+/* ---- learned short-range correction (no ct2 counterpart) ---- */
+size_t orc_mlp_num_params(void);
+double orc_bf16_round(double v);
+void orc_gelu(size_t n, const double* z, double* g, double* dg);
+void orc_mlp_forward(const double* w, size_t m, const double* X, int bf16, double* Y);
+double orc_mlp_grad(const double* w, size_t m, const double* X, const double* T, int bf16, double* grad);
+
 /* ---- cosmology (cosmology.cpp:11-102), explicit parameters ---- */
 typedef struct {
     double H;         /* Hubble constant in code time units */

@@ -76,6 +76,13 @@ def _declare(L):
     L.orc_pm_green_table.argtypes = [i, d, d, _P]
     L.orc_pm_forces.argtypes = [sz, _P, _P, _P, i, d, d, _P, _P, _P, _P]
     L.orc_ewald_kick.argtypes = [sz, _P, _P, _P, sz, _P, d, d, d, i, i, _P, _P, _P, _P]
+    L.orc_mlp_num_params.restype = sz
+    L.orc_bf16_round.argtypes = [d]
+    L.orc_bf16_round.restype = d
+    L.orc_gelu.argtypes = [sz, _P, _P, _P]
+    L.orc_mlp_forward.argtypes = [_P, sz, _P, i, _P]
+    L.orc_mlp_grad.argtypes = [_P, sz, _P, _P, i, _P]
+    L.orc_mlp_grad.restype = d
     L.orc_cosmos_update.argtypes = [C.POINTER(Cosmo), _P, _P, _P, d]
     L.orc_cosmos_dadt.argtypes = [C.POINTER(Cosmo), d]
     L.orc_cosmos_dadt.restype = d
@@ -240,6 +247,46 @@ def ewald_kick(x, y, z, gm, soft, sinks=None, alpha=2.5, nimg=2, kmax=5):
     out = [np.empty(m) for _ in range(4)]
     lib().orc_ewald_kick(x.size, _p(x), _p(y), _p(z), m, sp, gm, soft, alpha, nimg, kmax, *[_p(o) for o in out])
     return tuple(out)
+
+
+# ---------------------------------------------------------------- learned correction
+MLP_KIN, MLP_HID, MLP_KOUT = 128, 256, 32
+
+
+def mlp_num_params() -> int:
+    return int(lib().orc_mlp_num_params())
+
+
+def bf16_round(v: float) -> float:
+    return lib().orc_bf16_round(v)
+
+
+def gelu(z):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    g, dg = np.empty_like(z), np.empty_like(z)
+    lib().orc_gelu(z.size, _p(z), _p(g), _p(dg))
+    return g, dg
+
+
+def mlp_forward(w, X, bf16=True):
+    """FP64 forward of the correction network: X[m][128] -> Y[m][32]."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    m = X.shape[0]
+    Y = np.empty((m, MLP_KOUT))
+    lib().orc_mlp_forward(_p(w), m, _p(X), int(bf16), _p(Y))
+    return Y
+
+
+def mlp_grad(w, X, T, bf16=True):
+    """(loss, dL/dw) of L = sum_r T[r,3] |y[r,:3] - T[r,:3]|^2 / m (T is m x 4:
+    targets and a per-sample weight)."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    g = np.empty(mlp_num_params())
+    loss = lib().orc_mlp_grad(_p(w), X.shape[0], _p(X), _p(T), int(bf16), _p(g))
+    return loss, g
 
 
 # ---------------------------------------------------------------- cosmology

@@ -1,0 +1,126 @@
+"""GPU parity of the learned correction's BF16-MFMA kernels against the FP64
+oracle (forward, gradients, features) and the gain test: the trained
+network must reduce the PM force error against the periodic FP64 judge."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TWO32 = 2 ** 32
+
+
+def features(rng, m):
+    X = np.zeros((m, 128), np.float32)
+    X[:, :108] = rng.standard_normal((m, 108))
+    X[:, 108:111] = rng.uniform(-0.5, 0.5, (m, 3))
+    return X
+
+
+def test_mlp_forward_vs_oracle(ct2, orc):
+    ct2.mlp_init(7)
+    w = ct2.mlp_get_weights().astype(np.float64)
+    rng = np.random.default_rng(1)
+    X = features(rng, 384)
+    got = ct2.mlp_forward(X)
+    want = orc.mlp_forward(w, X, bf16=True)[:, :3]
+    # bf16 activations: an FP32 pre-activation that lands next to a bf16
+    # rounding boundary may round one ulp (2^-8) apart from the FP64 one
+    err = np.abs(got - want)
+    scale = np.sqrt(np.mean(want ** 2))
+    assert np.median(err) < 2e-3 * scale and np.max(err) < 3e-2 * scale, (np.median(err) / scale, err.max() / scale)
+    exact = orc.mlp_forward(w, X, bf16=False)[:, :3]
+    assert np.median(np.abs(got - exact)) < 2e-2 * scale  # bf16 network vs FP64 network
+
+
+def test_mlp_gradients_vs_oracle(ct2, orc):
+    ct2.mlp_init(11)
+    w = ct2.mlp_get_weights().astype(np.float64)
+    rng = np.random.default_rng(2)
+    X = features(rng, 256)
+    T = np.concatenate([rng.standard_normal((256, 3)), rng.uniform(0.5, 2.0, (256, 1))], axis=1).astype(np.float32)
+    loss, g = ct2.mlp_grad(X, T)
+    oloss, og = orc.mlp_grad(w, X, T, bf16=True)
+    assert loss == pytest.approx(oloss, rel=1e-2)
+    # per-matrix relative L2 error; bf16 output gradients bound it at ~1e-2
+    offs = [0, 32768, 33024, 98560, 98816, 107008, 107040]
+    for a, b in zip(offs[:-1], offs[1:]):
+        d = np.linalg.norm(g[a:b] - og[a:b]) / max(np.linalg.norm(og[a:b]), 1e-30)
+        assert d < 3e-2, (a, d)
+    cos = np.dot(g, og) / (np.linalg.norm(g) * np.linalg.norm(og))
+    assert cos > 0.999
+
+
+def test_mlp_features_from_pm_grids(ct2):
+    x, y, z = ct2.ic_plummer_mix(20000, 10000, 4, 0.02, 0.2, 8)
+    n = x.size
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z)
+    gm = float(np.float32(1.0 / n))
+    ct2.set_force_options(1e-3, gm, 0)
+    ng = 32
+    ct2.pm_init(ng, 1.25)
+    ct2.mlp_init(1)
+    ct2.pm_forces()
+    W = ct2.pm_density_download().astype(np.float64) * 2.0 ** -32
+    A = ct2.pm_grid_download(0)
+    B = ct2.pm_grid_download(1)
+    idx = np.array([0, 5, 999, 15000, 19999])
+    X = ct2.mlp_features(idx)
+    s = 32 - 5
+    funit = gm * ng * ng
+    for r, p in enumerate(idx):
+        c = [((int(v[p]) + (1 << (s - 1))) >> s) % ng for v in (x, y, z)]
+        off = [np.float32(np.int32(np.uint32(v[p]) - np.uint32(cc << s))) / np.float32(2 ** s) for v, cc in zip((x, y, z), c)]
+        assert np.allclose(X[r, 108:111], off, atol=1e-7)
+        q = 0
+        for a in (-1, 0, 1):
+            for b in (-1, 0, 1):
+                for cz in (-1, 0, 1):
+                    i, j, k = (c[0] + a) % ng, (c[1] + b) % ng, (c[2] + cz) % ng
+                    assert X[r, 4 * q] == pytest.approx(W[i, j, k] * ng ** 3 / n, rel=1e-6, abs=1e-6)
+                    assert X[r, 4 * q + 1] == pytest.approx(A[i, j, k].real / funit, rel=1e-5, abs=1e-6)
+                    assert X[r, 4 * q + 2] == pytest.approx(A[i, j, k].imag / funit, rel=1e-5, abs=1e-6)
+                    assert X[r, 4 * q + 3] == pytest.approx(B[i, j, k].real / funit, rel=1e-5, abs=1e-6)
+                    q += 1
+        assert not X[r, 111:].any()
+
+
+def rel_err(F, J):
+    return np.linalg.norm(F - J, axis=0) / np.linalg.norm(J, axis=0)
+
+
+def test_mlp_training_reduces_pm_error(ct2):
+    """Train on direct-sum (periodic FP64) labels of two lattice+Gaussian
+    sets (0.2 cell, as the GPU configuration), then on held-out particles of
+    a third seed the PM + MLP force must be closer to the judge than the PM
+    force alone."""
+    side, ng = 32, 64
+    n = side ** 3
+    gm = float(np.float32(1.0 / n))
+    ct2.set_force_options(1.0 / (50 * side), gm, 0)
+    ct2.pm_init(ng, 1.25)
+    ct2.mlp_init(3)
+    ct2.mlp_data_reset(1 << 16)
+    for seed in (1, 2):
+        x, y, z = ct2.ic_lattice(side, 0.2, seed)
+        ct2.particle_allocate(n)
+        ct2.particles_upload(x, y, z)
+        ct2.mlp_data_append(np.random.default_rng(seed).choice(n, 16384, replace=False))
+    assert ct2.mlp_data_size() == 32768
+    losses = ct2.mlp_train(2000, 1024, 1e-3, 5)
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-100:]) < 0.7 * np.mean(losses[:20])
+    # held-out set
+    x, y, z = ct2.ic_lattice(side, 0.2, 99)
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z)
+    idx = np.random.default_rng(9).choice(n, 2048, replace=False)
+    jx, jy, jz, _ = ct2.kick_sample_fp64(idx, periodic=True)
+    J = np.stack([jx, jy, jz])
+    fx, fy, fz, _ = ct2.pm_forces()
+    e_pm = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J)
+    fx, fy, fz, _ = ct2.pm_mlp_forces()
+    e_ml = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J)
+    print(f"held-out median rel err: PM {np.median(e_pm):.3e}  PM+MLP {np.median(e_ml):.3e}; "
+          f"p90 {np.percentile(e_pm, 90):.3e} -> {np.percentile(e_ml, 90):.3e}", flush=True)
+    assert np.median(e_ml) < 0.8 * np.median(e_pm)
