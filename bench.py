@@ -25,46 +25,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from ct2_amd.timing import HipEvents, time_calls  # noqa: E402
+
 METRIC = "kick interactions/s + 3D FFT GB/s; step time PM+MLP vs direct at ≤1e-3 force err"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
-
-
-class HipEvents:
-    """hipEvent timing on the library's stream (where every kernel runs)."""
-
-    def __init__(self, stream: int):
-        self.hip = C.CDLL("libamdhip64.so")
-        self.stream = C.c_void_p(stream)
-
-    def event(self):
-        e = C.c_void_p()
-        assert self.hip.hipEventCreate(C.byref(e)) == 0
-        return e
-
-    def record(self, e):
-        assert self.hip.hipEventRecord(e, self.stream) == 0
-
-    def elapsed_ms(self, a, b) -> float:
-        assert self.hip.hipEventSynchronize(b) == 0
-        ms = C.c_float()
-        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
-        return ms.value
-
-    def device_sync(self):
-        assert self.hip.hipDeviceSynchronize() == 0
-
-
-def time_calls(ev: HipEvents, fn, reps: int) -> list[float]:
-    out = []
-    for _ in range(reps):
-        a, b = ev.event(), ev.event()
-        ev.record(a)
-        fn()
-        ev.record(b)
-        out.append(ev.elapsed_ms(a, b))
-    return out
 
 
 def rel_err(F, J):

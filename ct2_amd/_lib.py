@@ -44,6 +44,10 @@ class KickParams(C.Structure):
                 ("max_dt", C.c_float), ("scale", C.c_float)]
 
 
+class Cosmology(C.Structure):
+    _fields_ = [("H", C.c_double), ("omega_m", C.c_double), ("omega_r", C.c_double), ("omega_lam", C.c_double)]
+
+
 class ParticlePtrs(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("vx", C.c_void_p),
                 ("vy", C.c_void_p), ("vz", C.c_void_p), ("rung", C.c_void_p), ("n", C.c_size_t)]
@@ -91,6 +95,13 @@ _SIGS = {
     "ct2_memcpy_h2d": ([_P, _P, C.c_size_t], C.c_int),
     "ct2_memcpy_d2h": ([_P, _P, C.c_size_t], C.c_int),
     "ct2_ic_lattice": ([C.c_int, C.c_double, C.c_uint64, _P, _P, _P], C.c_int),
+    "ct2_ic_grf": ([C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_double], C.c_int),
+    "ct2_cosmos_update": ([C.POINTER(Cosmology), _P, _P, _P, C.c_double], C.c_int),
+    "ct2_cosmos_dadt": ([C.POINTER(Cosmology), C.c_double], C.c_double),
+    "ct2_cosmos_conformal_time": ([C.POINTER(Cosmology), C.c_double, C.c_double], C.c_double),
+    "ct2_cosmos_time": ([C.POINTER(Cosmology), C.c_double, C.c_double], C.c_double),
+    "ct2_cosmos_kick_factor": ([C.POINTER(Cosmology), C.c_double, C.c_double], C.c_double),
+    "ct2_cosmos_drift_factor": ([C.POINTER(Cosmology), C.c_double, C.c_double], C.c_double),
     "ct2_ic_plummer_mix": ([C.c_size_t, C.c_size_t, C.c_int, C.c_double, C.c_double, C.c_uint64,
                             _P, _P, _P], C.c_int),
 }

@@ -313,3 +313,44 @@ def ic_plummer_mix(n: int, n_sph: int, nspheres: int, a: float, rmax: float, see
     out = [np.empty(n, np.uint32) for _ in range(3)]
     L.check(L.load().ct2_ic_plummer_mix(n, n_sph, nspheres, a, rmax, seed, *[_p(x) for x in out]))
     return tuple(out)
+
+
+def ic_grf(n: int, rms_cells: float, spectral_index: float = -2.0, seed: int = 1, vfac: float = 0.0) -> None:
+    """Zel'dovich lattice from a Gaussian random field, written into the
+    device container (which must hold n^3 particles)."""
+    L.check(L.load().ct2_ic_grf(n, rms_cells, spectral_index, seed, vfac))
+
+
+# ---------------------------------------------------------------- cosmology
+Cosmology = L.Cosmology
+
+
+def cosmology(H: float, omega_m: float = 1.0, omega_r: float = 0.0, omega_lam: float = 0.0):
+    return L.Cosmology(H, omega_m, omega_r, omega_lam)
+
+
+def cosmos_update(c, adotdot: float, adot: float, a: float, dt0: float):
+    """cosmos_update (cosmology.cpp:38-44): returns (adotdot, adot, a)."""
+    v = [C.c_double(adotdot), C.c_double(adot), C.c_double(a)]
+    L.check(L.load().ct2_cosmos_update(C.byref(c), C.byref(v[0]), C.byref(v[1]), C.byref(v[2]), dt0))
+    return v[0].value, v[1].value, v[2].value
+
+
+def cosmos_dadt(c, a: float) -> float:
+    return L.load().ct2_cosmos_dadt(C.byref(c), a)
+
+
+def cosmos_conformal_time(c, a0: float, a1: float) -> float:
+    return L.load().ct2_cosmos_conformal_time(C.byref(c), a0, a1)
+
+
+def cosmos_time(c, a0: float, a1: float) -> float:
+    return L.load().ct2_cosmos_time(C.byref(c), a0, a1)
+
+
+def cosmos_kick_factor(c, a0: float, a1: float) -> float:
+    return L.load().ct2_cosmos_kick_factor(C.byref(c), a0, a1)
+
+
+def cosmos_drift_factor(c, a0: float, a1: float) -> float:
+    return L.load().ct2_cosmos_drift_factor(C.byref(c), a0, a1)

@@ -59,6 +59,21 @@ def _compile(src: str, force: bool) -> str:
     return obj
 
 
+def build_compat_driver() -> str:
+    """The C++ program that drives ct2's kick()/drift() signatures through
+    ct2_amd/compat (tests/test_gpu_compat.py runs it on the GPU)."""
+    out_dir = os.path.join(ROOT, "tests", "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "compat_driver")
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(HERE, 'compat')}",
+           os.path.join(HERE, "compat", "ct2_gpu_binding.cpp"), os.path.join(ROOT, "tests", "compat_driver.cpp"),
+           f"-L{HERE}", "-lct2hip", "-Wl,-rpath,$ORIGIN/../../ct2_amd", "-o", exe]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compat driver build failed:\n{r.stdout}")
+    return exe
+
+
 def build(force: bool = False, verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
@@ -70,6 +85,7 @@ def build(force: bool = False, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}")
     os.replace(tmp, LIB)
+    build_compat_driver()
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
     return LIB

@@ -46,10 +46,14 @@ def build_training_set(api, kinds, seeds, side, samples, log):
     for seed in seeds:
         for kind in kinds:
             t0 = time.perf_counter()
-            x, y, z, soft = particle_set(api, kind, side, seed)
-            api.set_force_options(soft, 1.0 / n, api.KERNEL_PLUMMER)
             api.particle_allocate(n)
-            api.particles_upload(x, y, z)
+            if kind == "grf":  # the stepping configuration's initial conditions, built on the device
+                api.ic_grf(side, 0.3, -2.0, seed, 0.0)
+                soft = 1.0 / (50 * side)
+            else:
+                x, y, z, soft = particle_set(api, kind, side, seed)
+                api.particles_upload(x, y, z)
+            api.set_force_options(soft, 1.0 / n, api.KERNEL_PLUMMER)
             idx = np.random.default_rng(1000 + seed).choice(n, min(samples, n), replace=False)
             api.mlp_data_append(idx)
             api.synchronize()

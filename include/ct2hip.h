@@ -207,6 +207,33 @@ int ct2_ic_lattice(int n, double sigma_cells, uint64_t seed, uint32_t* x, uint32
 int ct2_ic_plummer_mix(size_t n, size_t n_sph, int nspheres, double a, double rmax,
                        uint64_t seed, uint32_t* x, uint32_t* y, uint32_t* z);
 
+/* Lattice of n^3 particles (n a power of two <= 512) displaced by a Gaussian
+ * random field in the Zel'dovich approximation, written straight into the
+ * device container (which must hold n^3 particles): density power
+ * P(k) ~ k^spectral_index up to the Nyquist sphere, displacement rms per
+ * component rms_cells lattice cells, velocities vfac * displacement.  FP64
+ * FFTs; seeded and deterministic. */
+int ct2_ic_grf(int n, double rms_cells, double spectral_index, uint64_t seed, double vfac);
+
+/* ------------------------------------------------------------ cosmology */
+/* FP64 scale-factor and time integrals of cosmology.hpp:13-16
+ * (cosmology.cpp:11-102) with explicit parameters instead of get_options():
+ * H is the Hubble constant in code time units. */
+typedef struct {
+    double H;
+    double omega_m;
+    double omega_r;
+    double omega_lam;
+} ct2_cosmology;
+int ct2_cosmos_update(const ct2_cosmology* c, double* adotdot, double* adot, double* a, double dt0);
+double ct2_cosmos_dadt(const ct2_cosmology* c, double a);
+double ct2_cosmos_conformal_time(const ct2_cosmology* c, double a0, double a1);
+double ct2_cosmos_time(const ct2_cosmology* c, double a0, double a1);
+/* Leapfrog factors between two scale factors: int dt/a (kick of the
+ * canonical momentum) and int dt/a^2 (drift), to 1e-12 relative. */
+double ct2_cosmos_kick_factor(const ct2_cosmology* c, double a0, double a1);
+double ct2_cosmos_drift_factor(const ct2_cosmology* c, double a0, double a1);
+
 #ifdef __cplusplus
 }
 #endif

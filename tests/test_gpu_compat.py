@@ -1,0 +1,48 @@
+"""The reference-side binding: ct2's own kick()/drift() signatures and
+particle globals (ct2_amd/compat) driven from C++ the way kick_step drives
+them (main.cpp:62-81), checked against the CPU oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "_bin", "compat_driver")
+
+
+def lcg_positions(n):
+    s = 12345
+    out = np.empty((n, 3), np.uint32)
+    for i in range(n):
+        for d in range(3):
+            s = (s * 1664525 + 1013904223) % 2 ** 32
+            out[i, d] = s
+    return out.T
+
+
+def test_kick_and_drift_through_ct2_signatures(orc):
+    n = 512
+    r = subprocess.run([EXE, str(n)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    max_rung = int(lines[0].split()[1])
+    rows = [l.split() for l in lines[1:]]
+    x, y, z = (np.array([int(v[a]) for v in rows], np.uint32) for a in range(3))
+    vx, vy, vz = (np.array([float(v[a]) for v in rows], np.float32) for a in range(3, 6))
+    rung = np.array([int(v[6]) for v in rows])
+    x0, y0, z0 = lcg_positions(n)
+    gm, soft = float(np.float32(1.0 / n)), float(np.float32(0.01))
+    ax, ay, az, _ = orc.kick(x0, y0, z0, gm, soft, orc.KERNEL_PLUMMER)
+    # ascending (not first) and descending both half-kick: v = 2 F dt/2
+    dtk = float(np.float32(0.5) * np.float32(0.01))
+    V = np.stack([vx, vy, vz]).astype(np.float64)
+    W = 2 * dtk * np.stack([ax, ay, az])
+    err = np.linalg.norm(V - W, axis=0) / np.linalg.norm(W, axis=0)  # FP32 force vs FP64, per particle
+    assert np.median(err) < 2e-6 and err.max() < 1e-4, (np.median(err), err.max())
+    # the drift is bit-exact given the velocities the kick produced
+    wx, wy, wz = orc.drift(x0, y0, z0, vx, vy, vz, 0.01, 1.0)
+    assert np.array_equal(x, wx) and np.array_equal(y, wy) and np.array_equal(z, wz)
+    assert max_rung == rung.max()
