@@ -142,6 +142,16 @@ def pm_init(ng: int, rs_cells: float = 1.25) -> None:
     _pm_ng = ng
 
 
+def pm_near(rcut_rs: float) -> None:
+    """Adds the near-field direct sum (pairs closer than rcut_rs * rs) to every
+    PM force evaluation; 0 switches it off (ct2_pm_near)."""
+    L.check(L.load().ct2_pm_near(rcut_rs))
+
+
+def pm_near_cells() -> int:
+    return int(L.load().ct2_pm_near_cells())
+
+
 def pm_forces():
     """PM force and potential of every particle: (fx, fy, fz, pot)."""
     L.check(L.load().ct2_pm_forces())

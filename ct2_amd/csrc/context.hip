@@ -116,6 +116,7 @@ int ct2_finalize(void) {
     release_pm();
     release_ewald();
     release_mlp();
+    release_near();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device = -1;

@@ -58,6 +58,14 @@ int pm_forces();
 int pm_require();
 void pm_grids(int* g, const unsigned long long** W, const float2** A, const float2** B);
 
+// Near field of the PM force (near.hip): direct pairs closer than
+// rcut_rs * rs with the softened kernel minus the mesh's filtered one, added
+// to the stored force.  rs in box units.
+int near_set(double rcut_rs);
+bool near_enabled();
+int near_forces(double rs);
+int near_cells();
+
 // Periodic FP64 judge of m sinks (device index list) into d_out[4][m] (ewald.hip).
 int ewald_judge(const long* d_idx, int m, double* d_out);
 
@@ -67,6 +75,7 @@ void release_fft_plans();
 void release_pm();
 void release_ewald();
 void release_mlp();
+void release_near();
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(long v) {

@@ -1,0 +1,445 @@
+// near.hip -- the near field of the PM force on gfx950: a direct pair sum over
+// cell neighbourhoods beside the mesh far field, the structure of ct2's kick()
+// (kick.cpp:105-125: neighbour list at :117, direct pairs through kick_cell
+// :36-67, far field from the mesh expansion at :118 and :29).
+//
+// The Gaussian-filtered PM force (pm.hip) already carries, for every pair, the
+// filtered Newtonian kernel phi_long = erf(u)/r, u = r/(2 rs).  For each
+// minimum-image pair closer than rcut the near field adds the softened kernel
+// minus that:
+//     F += GM dx (k_soft(r) - [erf(u) - (2u/sqrt(pi)) e^{-u^2}] / r^3)
+//     pot += GM (p_soft(r) - erf(u)/r)
+// and the potential drops the background integral GM N 4 pi rs^2, so that
+// PM + near field tends to the periodic direct sum.
+//
+// Pipeline (all on the library stream, deterministic):
+//   near_bin      cell of each particle, per-cell counts (atomics) and slots
+//   near_scan     exclusive scan of the counts -> cell starts
+//   near_rows     work items: chunks of NEAR_BLOCK sinks within one (x, y) row
+//   near_scatter  (x, y, z, index) into its cell's segment (slot order)
+//   near_order    each cell's segment sorted by particle index (rank by
+//                 counting), so the summation order never depends on atomics
+//   near_force    one workgroup per work item: sinks in registers, the 9
+//                 neighbouring rows' cell ranges streamed through LDS tiles,
+//                 FP32 pair terms with tile-level TwoSum; adds into the
+//                 stored force of each particle.
+// Cells have side >= rcut, so every partner of a sink lies in the 3x3x3 cells
+// around it; a work item's sinks span a run of cells along z in one row, and
+// the union of their neighbourhoods is 9 contiguous (or wrapped) cell ranges
+// of the sorted array.
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+namespace ct2 {
+namespace {
+
+constexpr int NEAR_BLOCK = 256;  // sinks per work item == sources per LDS tile
+constexpr int NEAR_MAX_NC = 64;  // cells per side at most
+constexpr int SCAN_THREADS = 1024;
+
+struct NearState {
+    double rcut_rs = 0.0;  // 0 = off
+    int nc = 0;
+    size_t cap = 0;                // particles the buffers hold
+    int* cell = nullptr;           // [cap]
+    int* slot = nullptr;           // [cap]
+    uint4* P0 = nullptr;           // [cap] scattered (x, y, z, index)
+    uint4* P = nullptr;            // [cap] sorted by (cell, index)
+    int ncell_cap = 0;
+    int* count = nullptr;          // [nc^3]
+    int* start = nullptr;          // [nc^3 + 1]
+    int* row_chunk = nullptr;      // [nc^2 + 1]
+} g_near;
+
+struct NearConst {
+    float rc2u;      // rcut^2 in 2^-64 units
+    float eps2u;     // Plummer eps^2
+    float h2u, hinvu, hinv3u;  // ct2 kernel
+    float inv4rs2u;  // 1 / (4 rs^2)
+    float inv2rsu;   // 1 / (2 rs)
+    float kl0;       // (2/sqrt(pi)) / (2 rs)^3
+    float pl0;       // (2/sqrt(pi)) / (2 rs)
+};
+
+__device__ __forceinline__ uint32_t cell_of(uint32_t x, uint32_t nc) {
+    return (uint32_t)(((uint64_t)x * nc) >> 32);
+}
+
+__global__ __launch_bounds__(256) void fill_int(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void near_bin(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                const uint32_t* __restrict__ Z, int n, uint32_t nc,
+                                                int* __restrict__ count, int* __restrict__ cell,
+                                                int* __restrict__ slot) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const int c = (int)((cell_of(X[p], nc) * nc + cell_of(Y[p], nc)) * nc + cell_of(Z[p], nc));
+    cell[p] = c;
+    slot[p] = atomicAdd(count + c, 1);
+}
+
+// Exclusive scan of len values (one workgroup): out[0..len], out[len] = total.
+// mode 0: values are in[i]; mode 1: values are ceil(row count / NEAR_BLOCK)
+// of row i, row count = in[(i+1)*nc] - in[i*nc] (in = cell starts).
+__global__ __launch_bounds__(SCAN_THREADS) void near_scan(const int* __restrict__ in, int len, int mode, int nc,
+                                                          int* __restrict__ out) {
+    __shared__ int part[SCAN_THREADS];
+    const int t = threadIdx.x;
+    const int per = (len + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int b = min(len, t * per), e = min(len, b + per);
+    auto value = [&](int i) -> int {
+        if (mode == 0) return in[i];
+        const int m = in[(i + 1) * nc] - in[i * nc];
+        return (m + NEAR_BLOCK - 1) / NEAR_BLOCK;
+    };
+    int s = 0;
+    for (int i = b; i < e; ++i) s += value(i);
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;  // exclusive prefix of this thread's chunk
+    for (int i = b; i < e; ++i) {
+        out[i] = run;
+        run += value(i);
+    }
+    if (t == SCAN_THREADS - 1) out[len] = part[t];
+}
+
+__global__ __launch_bounds__(256) void near_scatter(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                    const uint32_t* __restrict__ Z, int n,
+                                                    const int* __restrict__ cell, const int* __restrict__ slot,
+                                                    const int* __restrict__ start, uint4* __restrict__ P0) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    P0[start[cell[p]] + slot[p]] = make_uint4(X[p], Y[p], Z[p], (uint32_t)p);
+}
+
+// One workgroup per cell: element e goes to rank = #{f in cell : index_f < index_e}.
+__global__ __launch_bounds__(256) void near_order(const uint4* __restrict__ P0, const int* __restrict__ start,
+                                                  uint4* __restrict__ P) {
+    __shared__ uint32_t key[1024];
+    const int c = blockIdx.x;
+    const int base = start[c], m = start[c + 1] - base;
+    if (m == 0) return;
+    if (m == 1) {
+        if (threadIdx.x == 0) P[base] = P0[base];
+        return;
+    }
+    for (int e0 = 0; e0 < m; e0 += 256) {
+        const int e = e0 + threadIdx.x;
+        uint4 mine = make_uint4(0u, 0u, 0u, 0u);
+        if (e < m) mine = P0[base + e];
+        int rank = 0;
+        for (int f0 = 0; f0 < m; f0 += 1024) {
+            const int cnt = min(1024, m - f0);
+            __syncthreads();
+            for (int f = threadIdx.x; f < cnt; f += 256) key[f] = P0[base + f0 + f].w;
+            __syncthreads();
+            if (e < m)
+                for (int f = 0; f < cnt; ++f) rank += key[f] < mine.w;
+        }
+        if (e < m) P[base + rank] = mine;
+    }
+}
+
+__device__ __forceinline__ void two_sum(float& s, float& c, float v) {
+#pragma clang fp contract(off)
+    const float t = s + v;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (v - bp);
+    s = t;
+    c += e;
+}
+
+// Softened kernel minus the filtered Newtonian kernel for one pair, in
+// 2^-32 units: returns the force factor k (F += dx k) and potential p.
+template <int KERNEL>
+__device__ __forceinline__ void near_pair(float r2, const NearConst& q, float& k, float& p) {
+    float ks, ps;
+    if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+        const float w = __builtin_amdgcn_rsqf(r2 + q.eps2u);
+        ks = (w * w) * w;
+        ps = w;
+    } else {
+        if (r2 < q.h2u) {
+            ks = q.hinv3u;
+            ps = fmaf(-0.5f * q.hinv3u, r2, 1.5f * q.hinvu);
+        } else {
+            const float w = __builtin_amdgcn_rsqf(r2);
+            ks = (w * w) * w;
+            ps = w;
+        }
+    }
+    const float u2 = r2 * q.inv4rs2u;
+    float kl, pl;
+    if (u2 < 0.25f) {
+        // series of [erf(u) - (2u/sqrt(pi)) e^{-u^2}] / u^3 and erf(u) / u
+        // (2/sqrt(pi) factored out), exact limits at r = 0 (self pair)
+        float sk = fmaf(u2, 1.0f / 5400.0f, -1.0f / 780.0f);
+        sk = fmaf(u2, sk, 1.0f / 132.0f);
+        sk = fmaf(u2, sk, -1.0f / 27.0f);
+        sk = fmaf(u2, sk, 1.0f / 7.0f);
+        sk = fmaf(u2, sk, -2.0f / 5.0f);
+        sk = fmaf(u2, sk, 2.0f / 3.0f);
+        float sp = fmaf(u2, 1.0f / 9360.0f, -1.0f / 1320.0f);
+        sp = fmaf(u2, sp, 1.0f / 216.0f);
+        sp = fmaf(u2, sp, -1.0f / 42.0f);
+        sp = fmaf(u2, sp, 1.0f / 10.0f);
+        sp = fmaf(u2, sp, -1.0f / 3.0f);
+        sp = fmaf(u2, sp, 1.0f);
+        kl = q.kl0 * sk;
+        pl = q.pl0 * sp;
+    } else {
+        const float rn = __builtin_amdgcn_rsqf(r2);
+        const float u = (r2 * rn) * q.inv2rsu;
+        const float ec = erfcf(u);
+        const float ex = __expf(-u2);
+        const float rn3 = (rn * rn) * rn;
+        kl = rn3 * (1.0f - fmaf(1.1283791670955126f * u, ex, ec));
+        pl = rn * (1.0f - ec);
+    }
+    k = ks - kl;
+    p = ps - pl;
+}
+
+// Work item b: row r with row_chunk[r] <= b < row_chunk[r+1], chunk b - row_chunk[r].
+template <int KERNEL>
+__global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict__ P, const int* __restrict__ start,
+                                                         const int* __restrict__ row_chunk, int nc, NearConst q,
+                                                         double fscale, double pscale, double pconst,
+                                                         float* __restrict__ fx, float* __restrict__ fy,
+                                                         float* __restrict__ fz, float* __restrict__ pot) {
+    __shared__ uint4 tile[NEAR_BLOCK];
+    __shared__ int rng[18][2];
+    __shared__ int nrng, srow, s0s, s1s;
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    const int nrow = nc * nc;
+    if (tid == 0) {
+        int lo = 0, hi = nrow;  // find the last row with row_chunk[row] <= b
+        if (b >= row_chunk[nrow]) {
+            srow = -1;
+        } else {
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (row_chunk[mid] <= b) lo = mid;
+                else hi = mid;
+            }
+            srow = lo;
+            const int r0 = start[lo * nc], r1 = start[(lo + 1) * nc];
+            s0s = r0 + (b - row_chunk[lo]) * NEAR_BLOCK;
+            s1s = min(r1, s0s + NEAR_BLOCK);
+        }
+    }
+    __syncthreads();
+    const int row = srow;
+    if (row < 0) return;  // uniform across the workgroup
+    const int s0 = s0s, s1 = s1s;
+    const int cx = row / nc, cy = row % nc;
+    if (tid == 0) {
+        const int kz0 = (int)cell_of(P[s0].z, (uint32_t)nc), kz1 = (int)cell_of(P[s1 - 1].z, (uint32_t)nc);
+        int k = 0;
+        for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj) {
+                const int rr = ((cx + di + nc) % nc) * nc + (cy + dj + nc) % nc;
+                const int c0 = rr * nc;
+                if (kz1 - kz0 + 3 >= nc) {
+                    rng[k][0] = start[c0];
+                    rng[k][1] = start[c0 + nc];
+                    ++k;
+                } else {
+                    const int lo = kz0 - 1, hi = kz1 + 1;  // inclusive cell range along z
+                    if (lo < 0) {
+                        rng[k][0] = start[c0 + lo + nc];
+                        rng[k][1] = start[c0 + nc];
+                        ++k;
+                        rng[k][0] = start[c0];
+                        rng[k][1] = start[c0 + hi + 1];
+                        ++k;
+                    } else if (hi >= nc) {
+                        rng[k][0] = start[c0 + lo];
+                        rng[k][1] = start[c0 + nc];
+                        ++k;
+                        rng[k][0] = start[c0];
+                        rng[k][1] = start[c0 + hi - nc + 1];
+                        ++k;
+                    } else {
+                        rng[k][0] = start[c0 + lo];
+                        rng[k][1] = start[c0 + hi + 1];
+                        ++k;
+                    }
+                }
+            }
+        nrng = k;
+    }
+    __syncthreads();
+    const int i = s0 + tid;
+    const uint4 me = P[i < s1 ? i : s1 - 1];
+    float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f, cx_ = 0.f, cy_ = 0.f, cz_ = 0.f, cp_ = 0.f;
+    const int nr = nrng;
+    for (int r = 0; r < nr; ++r) {
+        const int a = rng[r][0], e = rng[r][1];
+        for (int j0 = a; j0 < e; j0 += NEAR_BLOCK) {
+            const int cnt = min(NEAR_BLOCK, e - j0);
+            __syncthreads();
+            if (tid < cnt) tile[tid] = P[j0 + tid];
+            __syncthreads();
+            float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
+            for (int j = 0; j < cnt; ++j) {
+                const uint4 s = tile[j];
+                const float dx = (float)(int32_t)(s.x - me.x);
+                const float dy = (float)(int32_t)(s.y - me.y);
+                const float dz = (float)(int32_t)(s.z - me.z);
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (r2 < q.rc2u) {
+                    float k, p;
+                    near_pair<KERNEL>(r2, q, k, p);
+                    tx = fmaf(dx, k, tx);
+                    ty = fmaf(dy, k, ty);
+                    tz = fmaf(dz, k, tz);
+                    tp += p;
+                }
+            }
+            two_sum(sx, cx_, tx);
+            two_sum(sy, cy_, ty);
+            two_sum(sz, cz_, tz);
+            two_sum(sp, cp_, tp);
+        }
+    }
+    if (i < s1) {
+        const uint32_t p = me.w;
+        fx[p] += (float)(((double)sx + (double)cx_) * fscale);
+        fy[p] += (float)(((double)sy + (double)cy_) * fscale);
+        fz[p] += (float)(((double)sz + (double)cz_) * fscale);
+        pot[p] += (float)(((double)sp + (double)cp_) * pscale - pconst);
+    }
+}
+
+void free_near_buffers() {
+    dev_free(g_near.cell);
+    dev_free(g_near.slot);
+    dev_free(g_near.P0);
+    dev_free(g_near.P);
+    dev_free(g_near.count);
+    dev_free(g_near.start);
+    dev_free(g_near.row_chunk);
+    g_near.cell = g_near.slot = nullptr;
+    g_near.P0 = g_near.P = nullptr;
+    g_near.count = g_near.start = g_near.row_chunk = nullptr;
+    g_near.cap = 0;
+    g_near.ncell_cap = 0;
+}
+
+int ensure_buffers(size_t n, int nc) {
+    if (n > g_near.cap) {
+        dev_free(g_near.cell);
+        dev_free(g_near.slot);
+        dev_free(g_near.P0);
+        dev_free(g_near.P);
+        g_near.cell = g_near.slot = nullptr;
+        g_near.P0 = g_near.P = nullptr;
+        g_near.cap = 0;
+        CT2_TRY(dev_alloc((void**)&g_near.cell, n * sizeof(int), "near-field cells"));
+        CT2_TRY(dev_alloc((void**)&g_near.slot, n * sizeof(int), "near-field slots"));
+        CT2_TRY(dev_alloc((void**)&g_near.P0, n * sizeof(uint4), "near-field scatter"));
+        CT2_TRY(dev_alloc((void**)&g_near.P, n * sizeof(uint4), "near-field sorted particles"));
+        g_near.cap = n;
+    }
+    if (nc > g_near.ncell_cap) {
+        dev_free(g_near.count);
+        dev_free(g_near.start);
+        dev_free(g_near.row_chunk);
+        g_near.count = g_near.start = g_near.row_chunk = nullptr;
+        g_near.ncell_cap = 0;
+        const size_t c3 = (size_t)nc * nc * nc;
+        CT2_TRY(dev_alloc((void**)&g_near.count, c3 * sizeof(int), "near-field counts"));
+        CT2_TRY(dev_alloc((void**)&g_near.start, (c3 + 1) * sizeof(int), "near-field cell starts"));
+        CT2_TRY(dev_alloc((void**)&g_near.row_chunk, ((size_t)nc * nc + 1) * sizeof(int), "near-field work items"));
+        g_near.ncell_cap = nc;
+    }
+    return CT2_OK;
+}
+
+}  // namespace
+
+bool near_enabled() { return g_near.rcut_rs > 0.0; }
+
+// Adds the near field to the stored PM force (rs in box units).
+int near_forces(double rs) {
+    Context& c = ctx();
+    const double rcut = g_near.rcut_rs * rs;
+    int nc = (int)std::floor(1.0 / (rcut * (1.0 + 1e-6)));
+    nc = std::min(nc, NEAR_MAX_NC);
+    if (nc < 3) return set_error(CT2_ERR_INVALID, "near-field cutoff %g exceeds a third of the box", rcut);
+    const int n = (int)c.n;
+    CT2_TRY(ensure_buffers(c.n, nc));
+    g_near.nc = nc;
+    const int c3 = nc * nc * nc, nrow = nc * nc;
+    const unsigned pb = (unsigned)((n + 255) / 256);
+    hipStream_t st = c.stream;
+    fill_int<<<(c3 + 255) / 256, 256, 0, st>>>(g_near.count, c3);
+    CT2_LAUNCH_CHECK();
+    near_bin<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, (uint32_t)nc, g_near.count, g_near.cell, g_near.slot);
+    CT2_LAUNCH_CHECK();
+    near_scan<<<1, SCAN_THREADS, 0, st>>>(g_near.count, c3, 0, nc, g_near.start);
+    CT2_LAUNCH_CHECK();
+    near_scan<<<1, SCAN_THREADS, 0, st>>>(g_near.start, nrow, 1, nc, g_near.row_chunk);
+    CT2_LAUNCH_CHECK();
+    near_scatter<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, g_near.cell, g_near.slot, g_near.start, g_near.P0);
+    CT2_LAUNCH_CHECK();
+    near_order<<<c3, 256, 0, st>>>(g_near.P0, g_near.start, g_near.P);
+    CT2_LAUNCH_CHECK();
+
+    const double u = 0x1p32;
+    const double rsu = rs * u, h = c.opts.hsoft, hu = h * u;
+    NearConst q;
+    q.rc2u = (float)(rcut * u * rcut * u);
+    q.eps2u = (float)(hu * hu);
+    q.h2u = (float)(hu * hu);
+    q.hinvu = (float)(1.0 / hu);
+    q.hinv3u = (float)(1.0 / (hu * hu * hu));
+    q.inv4rs2u = (float)(0.25 / (rsu * rsu));
+    q.inv2rsu = (float)(0.5 / rsu);
+    const double c2 = 2.0 / std::sqrt(M_PI);
+    q.kl0 = (float)(c2 * std::pow(0.5 / rsu, 3));
+    q.pl0 = (float)(c2 * 0.5 / rsu);
+    const double gm = c.opts.GM;
+    const double fscale = gm * 0x1p64, pscale = gm * 0x1p32;
+    const double pconst = gm * (double)n * 4.0 * M_PI * rs * rs;
+    // work items: at most one partial chunk per row beyond n / NEAR_BLOCK full ones
+    const unsigned items = (unsigned)((n + NEAR_BLOCK - 1) / NEAR_BLOCK + nrow);
+    if (c.opts.kernel == CT2_KERNEL_PLUMMER)
+        near_force<CT2_KERNEL_PLUMMER><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk, nc, q,
+                                                                     fscale, pscale, pconst, c.fx, c.fy, c.fz, c.pot);
+    else
+        near_force<CT2_KERNEL_CT2><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk, nc, q,
+                                                                 fscale, pscale, pconst, c.fx, c.fy, c.fz, c.pot);
+    CT2_LAUNCH_CHECK();
+    return CT2_OK;
+}
+
+int near_set(double rcut_rs) {
+    if (!(rcut_rs >= 0.0) || rcut_rs > 64.0) return set_error(CT2_ERR_INVALID, "bad near-field cutoff %g", rcut_rs);
+    g_near.rcut_rs = rcut_rs;
+    return CT2_OK;
+}
+
+int near_cells() { return g_near.nc; }
+
+void release_near() {
+    free_near_buffers();
+    g_near.rcut_rs = 0.0;
+    g_near.nc = 0;
+}
+
+}  // namespace ct2

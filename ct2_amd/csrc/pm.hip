@@ -211,6 +211,7 @@ int pm_forces() {
     CT2_TRY(fft3d(g_pm.B, g_pm.ng, -1, false));
     cic_interp<<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.A, g_pm.B, c.fx, c.fy, c.fz, c.pot);
     CT2_LAUNCH_CHECK();
+    if (near_enabled()) CT2_TRY(near_forces(g_pm.rs_cells / g_pm.ng));
     c.forces_valid = true;
     return CT2_OK;
 }
@@ -251,6 +252,18 @@ extern "C" int ct2_pm_init(int ng, double rs_cells) {
     g_pm.gm_table = 0.0f;
     return CT2_OK;
 }
+
+extern "C" int ct2_pm_near(double rcut_rs) {
+    CT2_TRY(require_init());
+    if (!g_pm.ng) return set_error(CT2_ERR_STATE, "ct2_pm_init has not been called");
+    if (rcut_rs > 0.0 && !(g_pm.rs_cells > 0.0))
+        return set_error(CT2_ERR_INVALID, "the near field needs a PM split radius rs > 0");
+    if (rcut_rs > 0.0 && rcut_rs * g_pm.rs_cells / g_pm.ng > 1.0 / 3.0)
+        return set_error(CT2_ERR_INVALID, "near-field cutoff %g rs exceeds a third of the box", rcut_rs);
+    return near_set(rcut_rs);
+}
+
+extern "C" int ct2_pm_near_cells(void) { return near_cells(); }
 
 extern "C" int ct2_pm_forces(void) { return pm_forces(); }
 

@@ -131,6 +131,22 @@ int ct2_pm_init(int ng, double rs_cells);
 /* PM force and potential of every particle into the stored-force arrays
  * (ct2_forces_download). */
 int ct2_pm_forces(void);
+/* Near field of the PM force.  ct2's kick() sums the near field directly over
+ * each cell's neighbour list and takes the far field from the mesh
+ * (kick.cpp:105-125: neighbour list :117, expansion :118, kick_cell :36-67).
+ * With rcut_rs > 0 every PM force evaluation here (ct2_pm_forces,
+ * ct2_kick_pm, ct2_pm_mlp_forces, ct2_kick_pm_mlp, ct2_mlp_data_append) adds,
+ * for each minimum-image pair closer than rcut = rcut_rs * rs (rs =
+ * rs_cells / ng of ct2_pm_init), the softened kernel minus the mesh's
+ * Gaussian-filtered Newtonian kernel, and removes the potential's background
+ * term GM N 4 pi rs^2, so that the result tends to the periodic direct sum.
+ * Particles are binned into cells of side >= rcut (at most 64 per side) and
+ * ordered by (cell, index): the sum is deterministic.  rcut_rs = 0 (the
+ * default) switches it off; the setting persists across ct2_pm_init.  rcut
+ * may not exceed a third of the box; rs must be > 0. */
+int ct2_pm_near(double rcut_rs);
+/* Cells per side of the last near-field evaluation (0 before the first). */
+int ct2_pm_near_cells(void);
 /* kick() with the PM force: forces as ct2_pm_forces, then the epilogue of
  * kick.cpp:75-99. */
 int ct2_kick_pm(const ct2_kick_params* params, unsigned flags, int* max_rung);

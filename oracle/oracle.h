@@ -78,6 +78,11 @@ void orc_cic_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint3
 void orc_pm_green_table(int ng, double gm, double rs_cells, double* table);
 void orc_pm_forces(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
                    double gm, double rs_cells, double* fx, double* fy, double* fz, double* pot);
+/* near-field complement of the Gaussian-filtered PM force (kick.cpp:105-125
+ * structure: direct pairs within rcut, mesh beyond); rs, rcut in box units */
+void orc_pm_near(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, size_t m,
+                 const int64_t* sinks, double gm, double soft, int kernel, double rs, double rcut,
+                 double* ax, double* ay, double* az, double* pot);
 /* periodic (Ewald) direct sum with Plummer softening of the nearest image */
 void orc_ewald_kick(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, size_t m,
                     const int64_t* sinks, double gm, double soft, double alpha, int nimg, int kmax,

@@ -76,6 +76,7 @@ def _declare(L):
     L.orc_pm_green_table.argtypes = [i, d, d, _P]
     L.orc_pm_forces.argtypes = [sz, _P, _P, _P, i, d, d, _P, _P, _P, _P]
     L.orc_ewald_kick.argtypes = [sz, _P, _P, _P, sz, _P, d, d, d, i, i, _P, _P, _P, _P]
+    L.orc_pm_near.argtypes = [sz, _P, _P, _P, sz, _P, d, d, i, d, d, _P, _P, _P, _P]
     L.orc_mlp_num_params.restype = sz
     L.orc_bf16_round.argtypes = [d]
     L.orc_bf16_round.restype = d
@@ -233,6 +234,20 @@ def pm_forces(x, y, z, g, gm, rs_cells=1.25):
     x, y, z = _u32(x), _u32(y), _u32(z)
     out = [np.empty(x.size) for _ in range(4)]
     lib().orc_pm_forces(x.size, _p(x), _p(y), _p(z), g, gm, rs_cells, *[_p(o) for o in out])
+    return tuple(out)
+
+
+def pm_near(x, y, z, gm, soft, rs, rcut, kernel=KERNEL_PLUMMER, sinks=None):
+    """FP64 near-field complement of the Gaussian-filtered PM force (pairs
+    closer than rcut; rs and rcut in box units): (ax, ay, az, pot)."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    if sinks is None:
+        m, sp = x.size, None
+    else:
+        sinks = np.ascontiguousarray(sinks, dtype=np.int64)
+        m, sp = sinks.size, _p(sinks)
+    out = [np.empty(m) for _ in range(4)]
+    lib().orc_pm_near(x.size, _p(x), _p(y), _p(z), m, sp, gm, soft, kernel, rs, rcut, *[_p(o) for o in out])
     return tuple(out)
 
 

@@ -219,3 +219,70 @@ void orc_ewald_kick(size_t n, const uint32_t* x, const uint32_t* y, const uint32
         pot[s] = gm * p;
     }
 }
+
+/*
+ * Near-field complement of the PM force, FP64 brute force.
+ *
+ * ct2's kick() sums the near field directly over each cell's neighbour list
+ * and takes the far field from the mesh (kick.cpp:105-125: neighbour list at
+ * :117, expansion at :118, direct pairs in kick_cell :36-67).  With the
+ * Gaussian-filtered PM force above, the mesh already holds, for every pair, the
+ * force and potential of the filtered Newtonian kernel
+ *     phi_long(r) = erf(u) / r,   u = r / (2 rs),
+ *     F_long(r)   = dx [erf(u) - (2u/sqrt(pi)) exp(-u^2)] / r^3,
+ * so the near field adds, for each minimum-image pair with r < rcut,
+ * the softened kernel (Plummer, or ct2's kick.cpp:52-60) minus that:
+ *     F_i += GM dx (k_soft(r) - [erf(u) - (2u/sqrt(pi)) e^{-u^2}] / r^3)
+ *     pot_i += GM (p_soft(r) - erf(u) / r)
+ * The self pair adds GM (p_soft(0) - 1/(rs sqrt(pi))) to the potential.  The
+ * PM + near-field potential then differs from the periodic (Ewald) one by the
+ * integral of erfc(u)/r over the box, GM N 4 pi rs^2 per particle, which is
+ * subtracted here, so that PM + near field converges to orc_ewald_kick as
+ * rcut grows and the mesh is refined.  rs and rcut are in box units.
+ */
+void orc_pm_near(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, size_t m,
+                 const int64_t* sinks, double gm, double soft, int kernel, double rs, double rcut,
+                 double* ax, double* ay, double* az, double* pot) {
+    const double e2 = soft * soft, rc2 = rcut * rcut;
+    const double c2 = 2.0 / sqrt(M_PI), inv2rs = 0.5 / rs;
+    const double background = 4.0 * M_PI * rs * rs * (double)n;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (ptrdiff_t s = 0; s < (ptrdiff_t)m; s++) {
+        const size_t i = sinks ? (size_t)sinks[s] : (size_t)s;
+        double f[3] = {0.0, 0.0, 0.0}, p = 0.0;
+        for (size_t j = 0; j < n; j++) {
+            const double d[3] = {(double)(int32_t)(x[j] - x[i]) * 0x1p-32, (double)(int32_t)(y[j] - y[i]) * 0x1p-32,
+                                 (double)(int32_t)(z[j] - z[i]) * 0x1p-32};
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 >= rc2) continue;
+            double ks, ps;
+            if (kernel == ORC_KERNEL_PLUMMER) {
+                const double w = 1.0 / sqrt(r2 + e2);
+                ks = w * w * w;
+                ps = w;
+            } else if (r2 < e2) {
+                ks = 1.0 / (soft * e2);
+                ps = 1.5 / soft - 0.5 * r2 / (soft * e2);
+            } else {
+                const double w = 1.0 / sqrt(r2);
+                ks = w * w * w;
+                ps = w;
+            }
+            const double r = sqrt(r2), u = r * inv2rs;
+            double kl, pl;
+            if (u < 1e-3) { /* leading terms of the series; r -> 0 limits */
+                kl = c2 * (2.0 / 3.0 - 0.4 * u * u) * inv2rs * inv2rs * inv2rs;
+                pl = c2 * (1.0 - u * u / 3.0) * inv2rs;
+            } else {
+                kl = (erf(u) - c2 * u * exp(-u * u)) / (r2 * r);
+                pl = erf(u) / r;
+            }
+            for (int q = 0; q < 3; q++) f[q] += d[q] * (ks - kl);
+            p += ps - pl;
+        }
+        ax[s] = gm * f[0];
+        ay[s] = gm * f[1];
+        az[s] = gm * f[2];
+        pot[s] = gm * (p - background);
+    }
+}

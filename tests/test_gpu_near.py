@@ -1,0 +1,122 @@
+"""GPU near field of the PM force (near.hip) against the FP64 oracle and the
+periodic FP64 judge: ragged and clustered particle sets (cells holding more
+than one tile and more than one LDS key chunk), both pair kernels, a
+neighbourhood that covers whole rows (3 cells per side), determinism and the
+argument checks."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TWO32 = 2 ** 32
+
+
+@pytest.fixture(autouse=True)
+def near_off(ct2):
+    """The near-field setting is library state: leave it off for other tests."""
+    yield
+    ct2.pm_near(0.0)
+
+
+def f32(v):
+    return float(np.float32(v))
+
+
+def rel(F, J):
+    return np.linalg.norm(F - J, axis=0) / np.linalg.norm(J, axis=0)
+
+
+def load(ct2, x, y, z):
+    ct2.particle_allocate(x.size)
+    ct2.particles_upload(x, y, z)
+
+
+def near_only(ct2, rcut_rs):
+    """(PM + near) and its near-field part (PM + near - PM), FP64."""
+    ct2.pm_near(0.0)
+    pm = np.stack(ct2.pm_forces()).astype(np.float64)
+    ct2.pm_near(rcut_rs)
+    tot = np.stack(ct2.pm_forces()).astype(np.float64)
+    return tot, tot - pm
+
+
+CASES = [
+    # name, builder, ng, rs_cells, rcut_rs, kernel, expected cells per side
+    ("uniform_ragged", "uniform", 64, 2.0, 6.0, 0, 5),
+    ("clustered_3cells", "clustered", 32, 2.0, 5.0, 0, 3),
+    ("clustered_ct2_kernel", "clustered", 64, 1.5, 6.0, 1, 7),
+]
+
+
+def build(ct2, kind):
+    if kind == "uniform":
+        rng = np.random.default_rng(11)
+        return tuple(rng.integers(0, TWO32, 4099, dtype=np.uint64).astype(np.uint32) for _ in range(3))
+    return ct2.ic_plummer_mix(24000, 16000, 4, 0.01, 0.1, 5)
+
+
+@pytest.mark.parametrize("name,kind,ng,rs_cells,rcut_rs,kernel,ncell", CASES)
+def test_near_field_vs_fp64_oracle(ct2, orc, name, kind, ng, rs_cells, rcut_rs, kernel, ncell):
+    x, y, z = build(ct2, kind)
+    n = x.size
+    load(ct2, x, y, z)
+    gm, soft = f32(1.0 / n), f32(2e-3)
+    ct2.set_force_options(soft, gm, kernel)
+    ct2.pm_init(ng, rs_cells)
+    tot, near = near_only(ct2, rcut_rs)
+    assert ct2.pm_near_cells() == ncell
+    rs = rs_cells / ng
+    sinks = np.random.default_rng(2).choice(n, 2000, replace=False)
+    want = np.stack(orc.pm_near(x, y, z, gm, soft, rs, rcut_rs * rs, kernel, sinks=sinks))
+    # the FP32 near-field sum against FP64, on the scale of the total force
+    e = np.linalg.norm(near[:3, sinks] - want[:3], axis=0) / np.linalg.norm(tot[:3, sinks], axis=0)
+    assert np.median(e) < 1e-5 and np.percentile(e, 99) < 2e-4 and e.max() < 2e-3, (np.median(e), e.max())
+    ep = np.abs(near[3, sinks] - want[3]) / np.abs(tot[3, sinks])
+    assert np.median(ep) < 1e-5 and ep.max() < 1e-3, (np.median(ep), ep.max())
+
+
+def test_near_field_is_deterministic(ct2):
+    x, y, z = ct2.ic_plummer_mix(30000, 20000, 3, 0.01, 0.1, 9)
+    load(ct2, x, y, z)
+    ct2.set_force_options(f32(1e-3), f32(1.0 / x.size), 0)
+    ct2.pm_init(64, 2.0)
+    ct2.pm_near(6.0)
+    a = np.stack(ct2.pm_forces())
+    b = np.stack(ct2.pm_forces())
+    assert np.array_equal(a, b)
+
+
+def test_pm_plus_near_field_vs_periodic_judge(ct2):
+    """Lattice + Gaussian(0.2 cell): PM alone is off by ~100 %; PM + near field
+    reaches the periodic FP64 direct sum to the mesh's discretisation error."""
+    side = 32
+    x, y, z = ct2.ic_lattice(side, 0.2, 1)
+    n = x.size
+    load(ct2, x, y, z)
+    ct2.set_force_options(f32(1.0 / (50 * side)), f32(1.0 / n), 0)
+    ct2.pm_init(2 * side, 3.0)
+    idx = np.random.default_rng(5).choice(n, 1024, replace=False)
+    J = np.stack(ct2.kick_sample_fp64(idx, periodic=True))
+    ct2.pm_near(0.0)
+    pm = np.stack(ct2.pm_forces())[:, idx]
+    ct2.pm_near(6.0)
+    tot = np.stack(ct2.pm_forces())[:, idx]
+    assert np.median(rel(pm[:3], J[:3])) > 0.5
+    e = rel(tot[:3], J[:3])
+    assert np.median(e) < 2e-3 and np.percentile(e, 90) < 5e-3, (np.median(e), np.percentile(e, 90))
+    assert np.max(np.abs(tot[3] - J[3])) < 1e-3 * np.abs(J[3]).max()
+
+
+def test_near_field_argument_checks(ct2):
+    from ct2_amd._lib import Ct2Error
+    x, y, z = ct2.ic_lattice(8, 0.1, 1)
+    load(ct2, x, y, z)
+    ct2.pm_init(16, 2.0)
+    with pytest.raises(Ct2Error):
+        ct2.pm_near(3.0)  # rcut = 3 * 2/16 = 0.375 box > 1/3
+    with pytest.raises(Ct2Error):
+        ct2.pm_near(-1.0)
+    ct2.pm_init(16, 0.0)
+    with pytest.raises(Ct2Error):
+        ct2.pm_near(2.0)  # no Gaussian split
+    ct2.pm_near(0.0)
