@@ -37,7 +37,7 @@ def rel_err(F, J):
     return np.linalg.norm(F - J, axis=0) / np.linalg.norm(J, axis=0)
 
 
-def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, ng: int, reps: int) -> dict:
+def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, J, ng: int, reps: int) -> dict:
     """Times the PM force and the PM + learned-correction force of the current
     particles and measures both against the periodic FP64 judge."""
     from ct2_amd import _lib
@@ -49,9 +49,7 @@ def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, ng: int, 
     _lib.check(lib.ct2_pm_forces())
     pm_ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), reps)))
     fx, fy, fz, _ = api.forces_download()
-    jx, jy, jz, _ = api.kick_sample_fp64(idx, periodic=True)
-    J = np.stack([jx, jy, jz])
-    e_pm = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J)
+    e_pm = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
     out.update({"pm_ms": round(pm_ms, 3), "pm_median_rel_err": float(np.median(e_pm)),
                 "pm_p90_rel_err": float(np.percentile(e_pm, 90))})
     trained = load_weights(DEFAULT_OUT)
@@ -64,7 +62,7 @@ def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, ng: int, 
     _lib.check(lib.ct2_pm_mlp_forces())
     ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_mlp_forces()), reps)))
     fx, fy, fz, _ = api.forces_download()
-    e_ml = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J)
+    e_ml = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
     out.update({"pm_mlp_ms": round(ms, 3), "pm_mlp_median_rel_err": float(np.median(e_ml)),
                 "pm_mlp_p90_rel_err": float(np.percentile(e_ml, 90)),
                 "speedup_pm_mlp_vs_direct": round(kick_ms / ms, 1),
@@ -72,6 +70,28 @@ def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, ng: int, 
                 if ms > pm_ms else None,
                 "target_median_1e-3_met": bool(np.median(e_ml) <= 1e-3),
                 "model": {k: meta[k] for k in ("ng", "rs_cells", "kinds", "seeds", "samples_per_set", "steps")}})
+    return out
+
+
+def near_leg(api, ev: HipEvents, side: int, kick_ms: float, idx, J, reps: int, ng: int, rs: float,
+             rcut_rs: float) -> dict:
+    """PM + direct near field (ct2's kick() structure: neighbour-cell pairs
+    directly, the rest from the mesh) timed and judged like the PM leg."""
+    from ct2_amd import _lib
+    lib = _lib.load()
+    api.pm_init(ng, rs)
+    api.pm_near(rcut_rs)
+    _lib.check(lib.ct2_pm_forces())
+    ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), reps)))
+    fx, fy, fz, pot = api.forces_download()
+    e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
+    out = {"pm_grid": ng, "rs_cells": rs, "rcut_rs": rcut_rs, "cells_per_side": api.pm_near_cells(),
+           "ms": round(ms, 3), "median_rel_err": float(np.median(e)), "p90_rel_err": float(np.percentile(e, 90)),
+           "p99_rel_err": float(np.percentile(e, 99)),
+           "pot_max_rel_err": float(np.max(np.abs(pot[idx] - J[3])) / np.abs(J[3]).max()),
+           "speedup_vs_direct": round(kick_ms / ms, 1),
+           "target_median_1e-3_and_10x_met": bool(np.median(e) <= 1e-3 and kick_ms / ms >= 10.0)}
+    api.pm_near(0.0)
     return out
 
 
@@ -108,6 +128,8 @@ def main() -> int:
     ap.add_argument("--fft-reps", type=int, default=10)
     ap.add_argument("--reps", type=int, default=5, help="repetitions of the PM timings")
     ap.add_argument("--pm-ng", type=int, default=256, help="PM grid of the PM + MLP leg")
+    ap.add_argument("--near-rs", type=float, default=4.0, help="split radius (cells of the 4*side grid), near leg")
+    ap.add_argument("--near-rcut", type=float, default=5.5, help="near-field cutoff in units of rs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for how "
@@ -182,7 +204,10 @@ def main() -> int:
 
     # PM and PM + learned correction on the same particles, against the
     # periodic FP64 direct sum (Ewald) on the same sinks
-    pmres = pm_leg(api, ev, n, side, kick_avg, idx, args.pm_ng, args.reps)
+    J = np.stack(api.kick_sample_fp64(idx, periodic=True))
+    pmres = pm_leg(api, ev, n, side, kick_avg, idx, J, args.pm_ng, args.reps)
+    print("[bench] PM + near field ...", file=sys.stderr, flush=True)
+    nearres = near_leg(api, ev, side, kick_avg, idx, J, args.reps, 4 * side, args.near_rs, args.near_rcut)
     pmres["direct_fp32_vs_fp64_min_image"] = {"median_rel_err": float(np.median(e_direct)),
                                               "p90": float(np.percentile(e_direct, 90)),
                                               "p99": float(np.percentile(e_direct, 99))}
@@ -211,6 +236,7 @@ def main() -> int:
         "drift": {"ms": round(drift_avg, 4), "GB/s": round(36 * n / drift_avg / 1e6, 1),
                   "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},
         "pm_mlp_vs_direct": pmres,
+        "pm_near_vs_direct": nearres,
     }
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

@@ -92,6 +92,15 @@ _SIGS = {
     "ct2_kick_pm_mlp": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_fft3d_c2c": ([_P, C.c_int, C.c_int], C.c_int),
     "ct2_fft3d_c2c_f64": ([_P, C.c_int, C.c_int], C.c_int),
+    "ct2_fft_complex_1d": ([_P, C.c_int, C.c_int, C.c_int, C.c_size_t], C.c_int),
+    "ct2_fft_real_1d": ([_P, C.c_int, C.c_int, C.c_int, C.c_size_t], C.c_int),
+    "ct2_scramble": ([_P, C.c_int, C.c_int, C.c_int], C.c_int),
+    "ct2_transpose_yz": ([_P, C.c_int], C.c_int),
+    "ct2_transpose_r2c": ([_P, _P, C.c_int], C.c_int),
+    "ct2_transpose_c2r": ([_P, _P, C.c_int], C.c_int),
+    "ct2_fft_r2c_3d": ([_P, _P, C.c_int], C.c_int),
+    "ct2_fft_c2r_3d": ([_P, _P, C.c_int], C.c_int),
+    "ct2_twiddles": ([C.c_int, _P, _P, _P], C.c_int),
     "ct2_dev_alloc": ([C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
     "ct2_dev_free": ([_P], C.c_int),
     "ct2_memcpy_h2d": ([_P, _P, C.c_size_t], C.c_int),

@@ -311,6 +311,74 @@ def fft3d(a: np.ndarray, dir: int = 1) -> np.ndarray:
         buf.close()
 
 
+# ---------------------------------------------------------------- ct2's FFT API on the device
+def _through_device(arrays, call):
+    """Uploads float64 host arrays, runs call(*pointers), synchronises and
+    returns the downloaded arrays."""
+    bufs = [DeviceBuffer(max(8, a.nbytes)) for a in arrays]
+    try:
+        for b, a in zip(bufs, arrays):
+            b.upload(np.ascontiguousarray(a, np.float64))
+        L.check(call(*[b.ptr for b in bufs]))
+        synchronize()
+        return [b.download(np.empty_like(np.ascontiguousarray(a, np.float64))) for b, a in zip(bufs, arrays)]
+    finally:
+        for b in bufs:
+            b.close()
+
+
+def fft_complex(X: np.ndarray, N: int, dir: int = 1, lanes: int = 1) -> np.ndarray:
+    """fft_complex(complex<simd_f64>*, N, dir) (fft_complex_vector.cpp:9-65) on
+    the GPU: X holds batches of 2*lanes*N doubles, bit-reversed input."""
+    batch = X.size // (2 * lanes * N)
+    return _through_device([X], lambda p: L.load().ct2_fft_complex_1d(p, N, dir, lanes, batch))[0]
+
+
+def fft_real(X: np.ndarray, N: int, dir: int = 1, lanes: int = 1) -> np.ndarray:
+    """fft_real(double*/simd_f64*, N, dir) (fft_real_*.cpp) on the GPU:
+    bit-reversed real input -> half-complex output, batches of lanes*N."""
+    batch = X.size // (lanes * N)
+    return _through_device([X], lambda p: L.load().ct2_fft_real_1d(p, N, dir, lanes, batch))[0]
+
+
+def scramble(X: np.ndarray, N1: int, N2: int, N3: int) -> np.ndarray:
+    return _through_device([X], lambda p: L.load().ct2_scramble(p, N1, N2, N3))[0]
+
+
+def transpose_yz(X: np.ndarray, N: int) -> np.ndarray:
+    return _through_device([X], lambda p: L.load().ct2_transpose_yz(p, N))[0]
+
+
+def transpose_r2c(X: np.ndarray, N: int) -> np.ndarray:
+    out = np.zeros(2 * N * N * (N // 2 + 1))
+    return _through_device([X, out], lambda a, b: L.load().ct2_transpose_r2c(a, b, N))[1]
+
+
+def transpose_c2r(Y: np.ndarray, N: int) -> np.ndarray:
+    out = np.zeros(N ** 3)
+    return _through_device([Y, out], lambda a, b: L.load().ct2_transpose_c2r(a, b, N))[1]
+
+
+def fft_r2c_3d(X: np.ndarray) -> np.ndarray:
+    """fft_r2c_3d(in, out, N): real N^3 -> planar half spectrum (2 N^2 (N/2+1))."""
+    N = round(X.size ** (1 / 3))
+    out = np.zeros(2 * N * N * (N // 2 + 1))
+    return _through_device([X, out], lambda a, b: L.load().ct2_fft_r2c_3d(a, b, N))[1]
+
+
+def fft_c2r_3d(Y: np.ndarray, N: int) -> np.ndarray:
+    """fft_c2r_3d(in, out, N): planar half spectrum -> real N^3, unnormalised."""
+    out = np.zeros(N ** 3)
+    return _through_device([Y, out], lambda a, b: L.load().ct2_fft_c2r_3d(a, b, N))[1]
+
+
+def twiddles(N: int):
+    """cos_twiddles, sin_twiddles, inv_sin_twiddles (twiddles.cpp:6-54)."""
+    out = [np.empty(N) for _ in range(3)]
+    L.check(L.load().ct2_twiddles(N, *[_p(a) for a in out]))
+    return tuple(out)
+
+
 # ---------------------------------------------------------------- initial conditions
 def ic_lattice(n: int, sigma_cells: float, seed: int):
     N = n ** 3

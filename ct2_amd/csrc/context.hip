@@ -117,6 +117,7 @@ int ct2_finalize(void) {
     release_ewald();
     release_mlp();
     release_near();
+    release_fft_ct2();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device = -1;

@@ -76,6 +76,7 @@ void release_pm();
 void release_ewald();
 void release_mlp();
 void release_near();
+void release_fft_ct2();
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(long v) {

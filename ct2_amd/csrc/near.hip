@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "near_erfc.h"
 
 namespace ct2 {
 namespace {
@@ -200,13 +201,18 @@ __device__ __forceinline__ void near_pair(float r2, const NearConst& q, float& k
         kl = q.kl0 * sk;
         pl = q.pl0 * sp;
     } else {
+        // erfc(u) = e^{-u^2} g(u), g a degree-10 fit on u in [0.5, 4.5]
+        // (near_erfc.h); beyond 4.5, erfc < 2e-10 and s is clamped
         const float rn = __builtin_amdgcn_rsqf(r2);
         const float u = (r2 * rn) * q.inv2rsu;
-        const float ec = erfcf(u);
         const float ex = __expf(-u2);
+        const float s = fminf(fmaf(u, 2.0f / (NEAR_ERFC_U1 - NEAR_ERFC_U0), -(NEAR_ERFC_U0 + NEAR_ERFC_U1) / (NEAR_ERFC_U1 - NEAR_ERFC_U0)), 1.0f);
+        float g = NEAR_ERFC_C[NEAR_ERFC_DEG];
+#pragma unroll
+        for (int k = NEAR_ERFC_DEG - 1; k >= 0; --k) g = fmaf(g, s, NEAR_ERFC_C[k]);
         const float rn3 = (rn * rn) * rn;
-        kl = rn3 * (1.0f - fmaf(1.1283791670955126f * u, ex, ec));
-        pl = rn * (1.0f - ec);
+        kl = rn3 * fmaf(-ex, fmaf(1.1283791670955126f, u, g), 1.0f);  // 1 - erfc(u) - (2u/sqrt(pi)) e^{-u^2}
+        pl = rn * fmaf(-ex, g, 1.0f);                                  // 1 - erfc(u)
     }
     k = ks - kl;
     p = ps - pl;
@@ -295,13 +301,12 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
             if (tid < cnt) tile[tid] = P[j0 + tid];
             __syncthreads();
             float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
-            for (int j = 0; j < cnt; ++j) {
-                const uint4 s = tile[j];
+            auto pair = [&](const uint4 s) {
                 const float dx = (float)(int32_t)(s.x - me.x);
                 const float dy = (float)(int32_t)(s.y - me.y);
                 const float dz = (float)(int32_t)(s.z - me.z);
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (r2 < q.rc2u) {
+                if (r2 < q.rc2u) {  // a source out of range of the whole wave is skipped
                     float k, p;
                     near_pair<KERNEL>(r2, q, k, p);
                     tx = fmaf(dx, k, tx);
@@ -309,6 +314,12 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
                     tz = fmaf(dz, k, tz);
                     tp += p;
                 }
+            };
+            if (cnt == NEAR_BLOCK) {
+#pragma unroll 4
+                for (int j = 0; j < NEAR_BLOCK; ++j) pair(tile[j]);
+            } else {
+                for (int j = 0; j < cnt; ++j) pair(tile[j]);
             }
             two_sum(sx, cx_, tx);
             two_sum(sy, cy_, ty);

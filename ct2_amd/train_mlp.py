@@ -6,7 +6,9 @@ library computes the periodic FP64 direct sum (Ewald) and stores
 (judge - PM) / F_unit with the PM-grid features; training runs the
 library's BF16-MFMA forward/backward kernels and Adam.
 
-Default: N = 128^3, PM grid 256^3 (split 1.25 cells); training sets from the
+Default: N = 128^3, PM grid 256^3 (split 1.25 cells), no near field
+(`--near` adds the direct near-field sum to the base force, and the model
+then learns the mesh's remaining error); training sets from the
 lattice+Gaussian(0.2 cell) configuration (eps = 1/(50*128)) and the
 clustered configuration (2^21 particles, half in 64 Plummer spheres of scale
 radius 0.01, eps = 2e-4), 5 seeds each, `--samples` labels per set.
@@ -83,6 +85,8 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=128)
     ap.add_argument("--ng", type=int, default=256)
     ap.add_argument("--rs", type=float, default=1.25)
+    ap.add_argument("--near", type=float, default=0.0,
+                    help="near-field cutoff in units of rs (0: PM alone); labels are then judge - (PM + near field)")
     ap.add_argument("--samples", type=int, default=1_000_000, help="labels per particle set")
     ap.add_argument("--seeds", type=int, default=5)
     ap.add_argument("--kinds", default="lattice,clustered")
@@ -102,12 +106,13 @@ def main() -> int:
     kinds = args.kinds.split(",")
     n = args.side ** 3
     api.pm_init(args.ng, args.rs)
+    api.pm_near(args.near)
     api.mlp_init(args.init_seed)
     api.mlp_data_reset(args.samples * args.seeds * len(kinds))
     build_training_set(api, kinds, range(1, args.seeds + 1), args.side, args.samples, log)
     losses = train(api, args.steps, args.batch, args.lr, args.init_seed, log)
     w = api.mlp_get_weights()
-    meta = {"side": args.side, "n": n, "ng": args.ng, "rs_cells": args.rs, "kinds": kinds, "seeds": args.seeds,
+    meta = {"side": args.side, "n": n, "ng": args.ng, "rs_cells": args.rs, "near_rcut_rs": args.near, "kinds": kinds, "seeds": args.seeds,
             "samples_per_set": args.samples, "steps": args.steps, "batch": args.batch, "lr": args.lr,
             "final_loss": float(np.mean(losses[-200:]))}
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)

@@ -204,6 +204,38 @@ int ct2_drift(float dt, float scale);
 int ct2_fft3d_c2c(void* data, int n, int dir);
 int ct2_fft3d_c2c_f64(void* data, int n, int dir);
 
+/* ct2's own FFT API (fft.hpp:18-29) on device memory, in ct2's conventions
+ * (FP64; synchronise with ct2_synchronize before reading results on the host):
+ *   ct2_fft_complex_1d  replaces fft_complex(complex<simd_f64>*, N, dir)
+ *       (fft_complex_vector.cpp:9-65): `batch` groups of `lanes` (1 or 4)
+ *       interleaved transforms, element n lane l at re X[2 lanes n + l],
+ *       im X[2 lanes n + lanes + l], group stride 2 lanes N doubles.
+ *       Input bit-reversed, output natural; dir = +1 is FFTW_FORWARD,
+ *       dir = -1 the conjugate transform; unnormalised.  N in [2, 4096].
+ *   ct2_fft_real_1d  replaces fft_real(double*, N, dir) and
+ *       fft_real(simd_f64*, N, dir) (fft_real_scalar.cpp:9-180,
+ *       fft_real_vector.cpp:9-183): real bit-reversed input [n][lane],
+ *       half-complex output X[k] = Re F_k, X[N-k] = Im F_k.
+ *   ct2_scramble  replaces scramble(X, N1, N2, N3) (scramble.cpp:8-43).
+ *   ct2_transpose_yz / _r2c / _c2r  replace transpose.cpp:7-128 (intended
+ *       index maps; planar out[x][z][ky], imaginary plane at N^2 (N/2+1)).
+ *   ct2_fft_r2c_3d / ct2_fft_c2r_3d  replace fft_r2c_3d / fft_c2r_3d
+ *       (fft_r2c_3d.cpp:4-31, fft_c2r_3d.cpp:4-40, stubs in ct2): real N^3
+ *       <-> planar half spectrum out[(N/2+1)(N i + j) + k] = Re,
+ *       out[N^2 (N/2+1) + same] = Im; unnormalised.  N in [8, 512].
+ *   ct2_twiddles  replaces cos_twiddles/sin_twiddles/inv_sin_twiddles
+ *       (twiddles.cpp:6-54): host arrays cos(-2 pi n/N), sin(-2 pi n/N),
+ *       sin(2 pi n/N). */
+int ct2_fft_complex_1d(double* X, int N, int dir, int lanes, size_t batch);
+int ct2_fft_real_1d(double* X, int N, int dir, int lanes, size_t batch);
+int ct2_scramble(double* X, int N1, int N2, int N3);
+int ct2_transpose_yz(double* X, int N);
+int ct2_transpose_r2c(const double* in, double* out, int N);
+int ct2_transpose_c2r(const double* in, double* out, int N);
+int ct2_fft_r2c_3d(const double* in, double* out, int N);
+int ct2_fft_c2r_3d(const double* in, double* out, int N);
+int ct2_twiddles(int N, double* cosw, double* sinw, double* invsinw);
+
 /* Device memory for callers without HIP code of their own. */
 int ct2_dev_alloc(size_t bytes, void** ptr);
 int ct2_dev_free(void* ptr);
