@@ -31,6 +31,9 @@ METRIC = "kick interactions/s + 3D FFT GB/s; step time PM+MLP vs direct at ≤1e
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
+# HBM bytes per kick_direct launch at N = 128^3 from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled
+# by a plain 1 GiB copy (profiles/kick_direct_pmc_n128cubed.md); None at other sizes
+KICK_TRAFFIC_BYTES = {128: 6.32e9}
 
 
 def rel_err(F, J):
@@ -130,11 +133,24 @@ def main() -> int:
     ap.add_argument("--pm-ng", type=int, default=256, help="PM grid of the PM + MLP leg")
     ap.add_argument("--near-rs", type=float, default=4.0, help="split radius (cells of the 4*side grid), near leg")
     ap.add_argument("--near-rcut", type=float, default=5.5, help="near-field cutoff in units of rs")
+    ap.add_argument("--kick-only", action="store_true",
+                    help="time the kick alone (for counter runs under rocprofv3 --pmc)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for how "
               "the kick and the FFT would shard)", file=sys.stderr)
         return 2
+
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: 3 x {args.copy_probe} MiB read + write", file=sys.stderr, flush=True)
 
     from ct2_amd import api
     api.init(0)
@@ -166,6 +182,12 @@ def main() -> int:
     value = inter * args.steps / wall
     achieved_tflops = FLOP_PER_INTERACTION * inter / (kick_avg * 1e-3) / 1e12
     print(f"[bench] kick {kick_avg:.1f} ms/step, {inter / kick_avg * 1e3:.3e} int/s", file=sys.stderr, flush=True)
+    if args.kick_only:
+        api.finalize()
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "interactions/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+                          "note": "kick only"}), flush=True)
+        return 0
 
     # drift: 36 B/particle
     drift_ms = time_calls(ev, lambda: api.drift(1e-9, 1.0), 20)
@@ -229,7 +251,7 @@ def main() -> int:
                                f"Plummer eps=1/(50*{side}), FP32 compensated, with half-kick/rung epilogue",
                    "n_particles": n, "interactions_per_step": inter},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4), "traffic": KICK_TRAFFIC_BYTES.get(side),
                      "kernel": "kick_direct (+kick_finalize)", "flop_per_interaction": FLOP_PER_INTERACTION,
                      "avg_launch_ms": round(kick_avg, 3)},
         "fft_fp32": fft,
