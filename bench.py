@@ -77,25 +77,50 @@ def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, J, ng: in
 
 
 def near_leg(api, ev: HipEvents, side: int, kick_ms: float, idx, J, reps: int, ng: int, rs: float,
-             rcut_rs: float) -> dict:
+             rcut_rs: float, order: int) -> dict:
     """PM + direct near field (ct2's kick() structure: neighbour-cell pairs
     directly, the rest from the mesh) timed and judged like the PM leg."""
     from ct2_amd import _lib
     lib = _lib.load()
+    api.pm_assignment(order)
     api.pm_init(ng, rs)
     api.pm_near(rcut_rs)
     _lib.check(lib.ct2_pm_forces())
     ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), reps)))
     fx, fy, fz, pot = api.forces_download()
     e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
-    out = {"pm_grid": ng, "rs_cells": rs, "rcut_rs": rcut_rs, "cells_per_side": api.pm_near_cells(),
-           "ms": round(ms, 3), "median_rel_err": float(np.median(e)), "p90_rel_err": float(np.percentile(e, 90)),
-           "p99_rel_err": float(np.percentile(e, 99)),
+    out = {"assignment": {1: "cic", 2: "tsc"}[order], "pm_grid": ng, "rs_cells": rs, "rcut_rs": rcut_rs,
+           "cells_per_side": api.pm_near_cells(), "ms": round(ms, 3), "median_rel_err": float(np.median(e)),
+           "p90_rel_err": float(np.percentile(e, 90)), "p99_rel_err": float(np.percentile(e, 99)),
            "pot_max_rel_err": float(np.max(np.abs(pot[idx] - J[3])) / np.abs(J[3]).max()),
            "speedup_vs_direct": round(kick_ms / ms, 1),
            "target_median_1e-3_and_10x_met": bool(np.median(e) <= 1e-3 and kick_ms / ms >= 10.0)}
     api.pm_near(0.0)
+    api.pm_assignment(1)
     return out
+
+
+def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: int) -> dict:
+    """The stepping configuration: side^3 particles from a Gaussian random
+    field, 512^3 grid, KDK leapfrog a = 0.02 -> 0.03 with FP64 factors, force
+    = TSC PM + near field, the periodic FP64 direct sum judging `judge`
+    particles each step.  The direct kick at this N is not run (N^2 / rate at
+    the measured rate would take minutes per step); its time is derived."""
+    from ct2_amd import stepping
+    rec = stepping.run(side, 2 * side, steps=steps, judge=judge, method="pm", timer=ev.time,
+                       log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=2.5, near=5.5, order=2)
+    n = side ** 3
+    ms = [r["force_kick_ms"] for r in rec["records"]][1:]  # the first call includes one-time set-up
+    med = [r["median_rel_err"] for r in rec["records"]]
+    direct_ms = n * float(n) / rate * 1e3
+    return {"n": n, "pm_grid": 2 * side, "assignment": "tsc", "rs_cells": 2.5, "rcut_rs": 5.5, "steps": steps,
+            "a": [rec["a_start"], rec["a_end"]], "force_kick_ms_median": round(float(np.median(ms)), 2),
+            "drift_ms_median": round(float(np.median([r["drift_ms"] for r in rec["records"][:-1]])), 3),
+            "median_rel_err_per_step": [float(f"{v:.3e}") for v in med],
+            "p90_rel_err_max": float(max(r["p90_rel_err"] for r in rec["records"])),
+            "judge_sinks_per_step": judge, "direct_kick_ms_derived": round(direct_ms, 1),
+            "speedup_vs_direct": round(direct_ms / float(np.median(ms)), 1),
+            "target_median_1e-3_and_10x_met": bool(max(med) <= 1e-3 and direct_ms / float(np.median(ms)) >= 10)}
 
 
 def cpu_baseline(x, y, z, soft, gm, target_s: float):
@@ -131,8 +156,8 @@ def main() -> int:
     ap.add_argument("--fft-reps", type=int, default=10)
     ap.add_argument("--reps", type=int, default=5, help="repetitions of the PM timings")
     ap.add_argument("--pm-ng", type=int, default=256, help="PM grid of the PM + MLP leg")
-    ap.add_argument("--near-rs", type=float, default=4.0, help="split radius (cells of the 4*side grid), near leg")
-    ap.add_argument("--near-rcut", type=float, default=5.5, help="near-field cutoff in units of rs")
+    ap.add_argument("--stepping-side", type=int, default=256, help="particles^(1/3) of the stepping leg (0: skip)")
+    ap.add_argument("--stepping-steps", type=int, default=10)
     ap.add_argument("--kick-only", action="store_true",
                     help="time the kick alone (for counter runs under rocprofv3 --pmc)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -229,7 +254,12 @@ def main() -> int:
     J = np.stack(api.kick_sample_fp64(idx, periodic=True))
     pmres = pm_leg(api, ev, n, side, kick_avg, idx, J, args.pm_ng, args.reps)
     print("[bench] PM + near field ...", file=sys.stderr, flush=True)
-    nearres = near_leg(api, ev, side, kick_avg, idx, J, args.reps, 4 * side, args.near_rs, args.near_rcut)
+    nearres = [near_leg(api, ev, side, kick_avg, idx, J, args.reps, ng, rs, rc, order)
+               for ng, rs, rc, order in ((2 * side, 1.5, 5.5, 2), (2 * side, 2.0, 5.5, 2), (4 * side, 4.0, 5.5, 1))]
+    steps_res = None
+    if args.stepping_side:
+        print("[bench] stepping configuration ...", file=sys.stderr, flush=True)
+        steps_res = stepping_leg(api, ev, inter / (kick_avg * 1e-3), args.stepping_side, args.stepping_steps, 16384)
     pmres["direct_fp32_vs_fp64_min_image"] = {"median_rel_err": float(np.median(e_direct)),
                                               "p90": float(np.percentile(e_direct, 90)),
                                               "p99": float(np.percentile(e_direct, 99))}
@@ -259,6 +289,7 @@ def main() -> int:
                   "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},
         "pm_mlp_vs_direct": pmres,
         "pm_near_vs_direct": nearres,
+        "stepping": steps_res,
     }
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

@@ -74,6 +74,7 @@ _SIGS = {
     "ct2_pm_forces": ([], C.c_int),
     "ct2_pm_near": ([C.c_double], C.c_int),
     "ct2_pm_near_cells": ([], C.c_int),
+    "ct2_pm_assignment": ([C.c_int], C.c_int),
     "ct2_kick_pm": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_pm_density_download": ([_P], C.c_int),
     "ct2_pm_grid_download": ([C.c_int, _P], C.c_int),

@@ -148,6 +148,11 @@ def pm_near(rcut_rs: float) -> None:
     L.check(L.load().ct2_pm_near(rcut_rs))
 
 
+def pm_assignment(order: int) -> None:
+    """Mass assignment of the PM force: 1 CIC (default), 2 TSC (ct2_pm_assignment)."""
+    L.check(L.load().ct2_pm_assignment(order))
+
+
 def pm_near_cells() -> int:
     return int(L.load().ct2_pm_near_cells())
 

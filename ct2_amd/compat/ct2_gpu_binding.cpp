@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <vector>
 
 #include "ct2hip.h"
@@ -82,3 +83,108 @@ void drift(float dt, float scale) {
     check(ct2_particles_download(particle_x_ptr, particle_y_ptr, particle_z_ptr, nullptr, nullptr, nullptr, nullptr),
           "download");
 }
+
+// ---- FFT API: host arrays in and out, the transform on the device.
+namespace {
+struct DeviceCopy {  // a device copy of a host array of n doubles
+    void* p = nullptr;
+    size_t bytes = 0;
+    DeviceCopy(const double* host, size_t n) : bytes(n * sizeof(double)) {
+        check(ct2_dev_alloc(bytes, &p), "ct2_dev_alloc");
+        if (host) check(ct2_memcpy_h2d(p, host, bytes), "ct2_memcpy_h2d");
+    }
+    void to_host(double* host) const { check(ct2_memcpy_d2h(host, p, bytes), "ct2_memcpy_d2h"); }
+    double* ptr() const { return (double*)p; }
+    ~DeviceCopy() { ct2_dev_free(p); }
+};
+
+void real_1d(double* X, int N, int dir, int lanes) {
+    ensure_init();
+    DeviceCopy d(X, (size_t)N * lanes);
+    check(ct2_fft_real_1d(d.ptr(), N, dir, lanes, 1), "ct2_fft_real_1d");
+    check(ct2_synchronize(), "ct2_synchronize");
+    d.to_host(X);
+}
+}  // namespace
+
+void fft_real(fft_float* X, int N, int dir) { real_1d(X, N, dir, 1); }
+void ct2_gpu_fft_real_lanes4(double* X, int N, int dir) { real_1d(X, N, dir, 4); }
+
+void ct2_gpu_fft_complex_lanes4(double* X, int N, int dir) {
+    ensure_init();
+    DeviceCopy d(X, (size_t)N * 8);
+    check(ct2_fft_complex_1d(d.ptr(), N, dir, 4, 1), "ct2_fft_complex_1d");
+    check(ct2_synchronize(), "ct2_synchronize");
+    d.to_host(X);
+}
+
+void fft_r2c_3d(const fft_float* in, fft_float* out, int N) {
+    ensure_init();
+    const size_t n3 = (size_t)N * N * N, half = (size_t)N * N * (N / 2 + 1);
+    DeviceCopy a(in, n3), b(nullptr, 2 * half);
+    check(ct2_fft_r2c_3d(a.ptr(), b.ptr(), N), "ct2_fft_r2c_3d");
+    check(ct2_synchronize(), "ct2_synchronize");
+    b.to_host(out);
+}
+
+void fft_c2r_3d(const fft_float* in, fft_float* out, int N) {
+    ensure_init();
+    const size_t n3 = (size_t)N * N * N, half = (size_t)N * N * (N / 2 + 1);
+    DeviceCopy a(in, 2 * half), b(nullptr, n3);
+    check(ct2_fft_c2r_3d(a.ptr(), b.ptr(), N), "ct2_fft_c2r_3d");
+    check(ct2_synchronize(), "ct2_synchronize");
+    b.to_host(out);
+}
+
+void scramble(fft_float* X, int N1, int N2, int N3) {
+    ensure_init();
+    DeviceCopy d(X, (size_t)N1 * N2 * N3);
+    check(ct2_scramble(d.ptr(), N1, N2, N3), "ct2_scramble");
+    check(ct2_synchronize(), "ct2_synchronize");
+    d.to_host(X);
+}
+
+void transpose_yz(fft_float* X, int N) {
+    ensure_init();
+    DeviceCopy d(X, (size_t)N * N * N);
+    check(ct2_transpose_yz(d.ptr(), N), "ct2_transpose_yz");
+    check(ct2_synchronize(), "ct2_synchronize");
+    d.to_host(X);
+}
+
+void transpose_r2c(const fft_float* in, fft_float* out, int N) {
+    ensure_init();
+    DeviceCopy a(in, (size_t)N * N * N), b(nullptr, 2 * (size_t)N * N * (N / 2 + 1));
+    check(ct2_transpose_r2c(a.ptr(), b.ptr(), N), "ct2_transpose_r2c");
+    check(ct2_synchronize(), "ct2_synchronize");
+    b.to_host(out);
+}
+
+void transpose_c2r(const fft_float* in, fft_float* out, int N) {
+    ensure_init();
+    DeviceCopy a(in, 2 * (size_t)N * N * (N / 2 + 1)), b(out, (size_t)N * N * N);
+    check(ct2_transpose_c2r(a.ptr(), b.ptr(), N), "ct2_transpose_c2r");
+    check(ct2_synchronize(), "ct2_synchronize");
+    b.to_host(out);
+}
+
+namespace {
+// twiddles.cpp keeps one table per N; so does this (not thread-safe either)
+const std::vector<double>& twiddle_table(int N, int which) {
+    static std::map<int, std::vector<double>> tables[3];
+    auto it = tables[which].find(N);
+    if (it == tables[which].end()) {
+        std::vector<double> c(N), s(N), i(N);
+        check(ct2_twiddles(N, c.data(), s.data(), i.data()), "ct2_twiddles");
+        tables[0][N] = c;
+        tables[1][N] = s;
+        tables[2][N] = i;
+        it = tables[which].find(N);
+    }
+    return it->second;
+}
+}  // namespace
+
+const std::vector<double>& cos_twiddles(int N) { return twiddle_table(N, 0); }
+const std::vector<double>& sin_twiddles(int N) { return twiddle_table(N, 1); }
+const std::vector<double>& inv_sin_twiddles(int N) { return twiddle_table(N, 2); }

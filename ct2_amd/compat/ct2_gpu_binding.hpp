@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 // sfmm::fixed32 is a 32-bit unsigned fixed-point position (cosmictiger.hpp:22).
 using ct2_pos_fixed = uint32_t;
@@ -42,3 +43,22 @@ void drift(float dt, float scale);     // drift.hpp:12
 
 // Options the force path reads from get_options() (options.cpp:155,190).
 void ct2_gpu_set_options(float hsoft, float GM);
+
+// ---- the FFT API of fft.hpp:18-29 (fft_float is double, cosmictiger.hpp:18-23)
+using fft_float = double;
+void fft_real(fft_float* X, int N, int dir = 1);                    // fft_real_scalar.cpp:9
+void fft_r2c_3d(const fft_float* in, fft_float* out, int N);        // fft_r2c_3d.cpp:4
+void fft_c2r_3d(const fft_float* in, fft_float* out, int N);        // fft_c2r_3d.cpp:4
+void scramble(fft_float* X, int N1, int N2, int N3);                // scramble.cpp:26
+void transpose_yz(fft_float* X, int N);                             // transpose.cpp:7
+void transpose_r2c(const fft_float* in, fft_float* out, int N);     // transpose.cpp:33
+void transpose_c2r(const fft_float* in, fft_float* out, int N);     // transpose.cpp:75
+const std::vector<double>& cos_twiddles(int N);                     // twiddles.cpp:6
+const std::vector<double>& sin_twiddles(int N);                     // twiddles.cpp:24
+const std::vector<double>& inv_sin_twiddles(int N);                 // twiddles.cpp:42
+// The 4-lane overloads fft_real(simd_f64*, N, dir) and
+// fft_complex(complex<simd_f64>*, N, dir) take sfmm's SIMD types, which this
+// file cannot name; a ct2 build forwards them to these (the layouts are the
+// same: [n][lane] reals, {re[4], im[4]} complex elements):
+void ct2_gpu_fft_real_lanes4(double* X, int N, int dir);
+void ct2_gpu_fft_complex_lanes4(double* X, int N, int dir);

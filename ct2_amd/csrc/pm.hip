@@ -3,15 +3,15 @@
 // Replaces the reference's multipole-mesh far field (particle_compute_multipoles
 // and particle_compute_expansions, particle.cpp:180-271, evaluated by L2P at
 // kick.cpp:29) with cloud-in-cell mass assignment, keeping its pipeline:
-//   cic_deposit   fixed-point CIC weights, uint64 atomics: deterministic,
-//                 bit-identical to the CPU oracle (12 B/particle read)
+//   mesh_deposit  fixed-point CIC (default) or TSC weights, uint64 atomics:
+//                 deterministic, bit-identical to the CPU oracle (12 B/particle read)
 //   to_complex    density grid -> FP32 complex grid
 //   fft3d         forward FP32 transform (fft.hip)
 //   green_mul     rho_hat * G (FP64 Green's table, octant-symmetric), then
 //                 the spectral gradient; two real fields share one complex
 //                 grid: A = Fx + i Fy, B = Fz + i pot
 //   fft3d x2      inverse transforms (two instead of four)
-//   cic_interp    force and potential at the particles (same CIC weights)
+//   mesh_interp   force and potential at the particles (same weights)
 // Grid layout: (i*ng + j)*ng + k with i along x; node c at position c/ng.
 #include <cmath>
 #include <vector>
@@ -28,18 +28,38 @@ constexpr int PM_BLOCK = 256;
 struct PmState {
     int ng = 0, g = 0;
     double rs_cells = 0.0;
+    int order_table = 0;              // assignment order the table was built for
     float gm_table = 0.0f;  // GM the table was built for
     unsigned long long* W = nullptr;  // density, 2^-32 particle units
     float2* A = nullptr;              // rho_hat -> Fx + i Fy
     float2* B = nullptr;              // Fz + i pot
     double* G = nullptr;              // Green's table on the octant, (ng/2+1)^3
 } g_pm;
+int g_assign = 1;  // mass assignment: 1 CIC, 2 TSC (kept across ct2_pm_init)
 
-__device__ __forceinline__ void cic_axis(uint32_t x, int s, uint32_t& i0, uint64_t& a0, uint64_t& a1) {
-    i0 = x >> s;
-    const uint32_t t = x & (uint32_t)((1ull << s) - 1ull);
-    a1 = (uint64_t)(t >> (s - CIC_BITS));
-    a0 = (1ull << CIC_BITS) - a1;
+// Per-axis weights, 21-bit fixed point summing to 2^21, and the first node:
+//   ORDER 1 (CIC): nodes lo, lo+1 with lo = x >> s, a1 = fraction >> (s-21);
+//   ORDER 2 (TSC): nearest node c = (x + 2^(s-1)) >> s, signed offset D in
+//     [-2^20, 2^20) (2^-21 cells); nodes c-1, c, c+1 get round((2^20-D)^2/2^22),
+//     the rest, round((2^20+D)^2/2^22): the quadratic spline (1/2)(1/2 -+ d)^2.
+template <int ORDER>
+__device__ __forceinline__ void assign_axis(uint32_t x, int s, uint32_t& lo, uint64_t (&w)[ORDER + 1]) {
+    if constexpr (ORDER == 1) {
+        lo = x >> s;
+        const uint32_t t = x & ((1u << s) - 1u);
+        w[1] = (uint64_t)(t >> (s - CIC_BITS));
+        w[0] = (1ull << CIC_BITS) - w[1];
+    } else {
+        const uint32_t half = 1u << (s - 1);
+        const uint32_t xs = x + half;
+        const int32_t d = (int32_t)(xs & ((1u << s) - 1u)) - (int32_t)half;
+        const int64_t D = (int64_t)(d >> (s - CIC_BITS));
+        const uint64_t a = (uint64_t)((1 << 20) - D), b = (uint64_t)((1 << 20) + D);
+        w[0] = (a * a + (1ull << 21)) >> 22;
+        w[2] = (b * b + (1ull << 21)) >> 22;
+        w[1] = (1ull << CIC_BITS) - w[0] - w[2];
+        lo = (xs >> s) - 1u;
+    }
 }
 
 __global__ __launch_bounds__(PM_BLOCK) void fill_u64(unsigned long long* __restrict__ p, long n) {
@@ -47,26 +67,27 @@ __global__ __launch_bounds__(PM_BLOCK) void fill_u64(unsigned long long* __restr
     if (i < n) p[i] = 0ull;
 }
 
-__global__ __launch_bounds__(PM_BLOCK) void cic_deposit(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
-                                                       const uint32_t* __restrict__ Z, int n, int g,
-                                                       unsigned long long* __restrict__ W) {
+template <int ORDER>
+__global__ __launch_bounds__(PM_BLOCK) void mesh_deposit(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                        const uint32_t* __restrict__ Z, int n, int g,
+                                                        unsigned long long* __restrict__ W) {
     const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
     if (p >= n) return;
     const int s = 32 - g;
     const uint32_t ng = 1u << g, mask = ng - 1;
     uint32_t i0, j0, k0;
-    uint64_t ax[2], ay[2], az[2];
-    cic_axis(X[p], s, i0, ax[0], ax[1]);
-    cic_axis(Y[p], s, j0, ay[0], ay[1]);
-    cic_axis(Z[p], s, k0, az[0], az[1]);
+    uint64_t ax[ORDER + 1], ay[ORDER + 1], az[ORDER + 1];
+    assign_axis<ORDER>(X[p], s, i0, ax);
+    assign_axis<ORDER>(Y[p], s, j0, ay);
+    assign_axis<ORDER>(Z[p], s, k0, az);
 #pragma unroll
-    for (int dx = 0; dx < 2; ++dx)
+    for (int dx = 0; dx <= ORDER; ++dx)
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy) {
+        for (int dy = 0; dy <= ORDER; ++dy) {
             const uint64_t axy = ax[dx] * ay[dy];
             const size_t row = ((size_t)((i0 + dx) & mask) * ng + ((j0 + dy) & mask)) * ng;
 #pragma unroll
-            for (int dz = 0; dz < 2; ++dz)
+            for (int dz = 0; dz <= ORDER; ++dz)
                 atomicAdd(W + row + ((k0 + dz) & mask), (unsigned long long)((axy * az[dz]) >> 31));
         }
 }
@@ -98,30 +119,31 @@ __global__ __launch_bounds__(PM_BLOCK) void green_mul(float2* __restrict__ A, fl
     B[c] = make_float2((float)((kz + 1.0) * pi), (float)(-(kz + 1.0) * pr));
 }
 
-__global__ __launch_bounds__(PM_BLOCK) void cic_interp(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
-                                                      const uint32_t* __restrict__ Z, int n, int g,
-                                                      const float2* __restrict__ A, const float2* __restrict__ B,
-                                                      float* __restrict__ fx, float* __restrict__ fy,
-                                                      float* __restrict__ fz, float* __restrict__ pot) {
+template <int ORDER>
+__global__ __launch_bounds__(PM_BLOCK) void mesh_interp(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                       const uint32_t* __restrict__ Z, int n, int g,
+                                                       const float2* __restrict__ A, const float2* __restrict__ B,
+                                                       float* __restrict__ fx, float* __restrict__ fy,
+                                                       float* __restrict__ fz, float* __restrict__ pot) {
     const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
     if (p >= n) return;
     const int s = 32 - g;
     const uint32_t ng = 1u << g, mask = ng - 1;
     uint32_t i0, j0, k0;
-    uint64_t ax[2], ay[2], az[2];
-    cic_axis(X[p], s, i0, ax[0], ax[1]);
-    cic_axis(Y[p], s, j0, ay[0], ay[1]);
-    cic_axis(Z[p], s, k0, az[0], az[1]);
+    uint64_t ax[ORDER + 1], ay[ORDER + 1], az[ORDER + 1];
+    assign_axis<ORDER>(X[p], s, i0, ax);
+    assign_axis<ORDER>(Y[p], s, j0, ay);
+    assign_axis<ORDER>(Z[p], s, k0, az);
     const float q = 0x1p-21f;
     float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
 #pragma unroll
-    for (int dx = 0; dx < 2; ++dx)
+    for (int dx = 0; dx <= ORDER; ++dx)
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy) {
+        for (int dy = 0; dy <= ORDER; ++dy) {
             const float wxy = ((float)ax[dx] * q) * ((float)ay[dy] * q);
             const size_t row = ((size_t)((i0 + dx) & mask) * ng + ((j0 + dy) & mask)) * ng;
 #pragma unroll
-            for (int dz = 0; dz < 2; ++dz) {
+            for (int dz = 0; dz <= ORDER; ++dz) {
                 const size_t cc = row + ((k0 + dz) & mask);
                 const float w = wxy * ((float)az[dz] * q);
                 const float2 a = A[cc], b = B[cc];
@@ -146,8 +168,9 @@ void free_pm() {
 }
 
 // FP64 Green's table on the octant |m_d| <= ng/2 (host, libm):
-//   G(m) = -4 pi GM exp(-k^2 rs^2) / (k^2 W(m)^2), W = prod_d sinc^2(pi m_d/ng), G(0) = 0.
-std::vector<double> green_table(int ng, double gm, double rs_cells) {
+//   G(m) = -4 pi GM exp(-k^2 rs^2) / (k^2 W(m)^2), W = prod_d sinc^(order+1)(pi m_d/ng),
+//   G(0) = 0.
+std::vector<double> green_table(int ng, double gm, double rs_cells, int order) {
     const int h = ng / 2 + 1;
     const double rs = rs_cells / ng;
     std::vector<double> t((size_t)h * h * h);
@@ -155,7 +178,7 @@ std::vector<double> green_table(int ng, double gm, double rs_cells) {
     for (int m = 0; m < h; ++m) {
         const double u = M_PI * m / ng;
         const double sc = m ? std::sin(u) / u : 1.0;
-        sinc2[m] = sc * sc;
+        sinc2[m] = order == 1 ? sc * sc : sc * sc * sc;
         k2[m] = (2.0 * M_PI * m) * (2.0 * M_PI * m);
     }
     for (int a = 0; a < h; ++a)
@@ -174,11 +197,12 @@ std::vector<double> green_table(int ng, double gm, double rs_cells) {
 
 int ensure_table() {
     Context& c = ctx();
-    if (g_pm.gm_table == c.opts.GM) return CT2_OK;
-    const std::vector<double> t = green_table(g_pm.ng, (double)c.opts.GM, g_pm.rs_cells);
+    if (g_pm.gm_table == c.opts.GM && g_pm.order_table == g_assign) return CT2_OK;
+    const std::vector<double> t = green_table(g_pm.ng, (double)c.opts.GM, g_pm.rs_cells, g_assign);
     CT2_HIP(hipMemcpyAsync(g_pm.G, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     CT2_HIP(hipStreamSynchronize(c.stream));
     g_pm.gm_table = c.opts.GM;
+    g_pm.order_table = g_assign;
     return CT2_OK;
 }
 
@@ -200,7 +224,10 @@ int pm_forces() {
     const unsigned gb = (unsigned)((n3 + PM_BLOCK - 1) / PM_BLOCK), pb = (unsigned)((n + PM_BLOCK - 1) / PM_BLOCK);
     fill_u64<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, n3);
     CT2_LAUNCH_CHECK();
-    cic_deposit<<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.W);
+    if (g_assign == 1)
+        mesh_deposit<1><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.W);
+    else
+        mesh_deposit<2><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.W);
     CT2_LAUNCH_CHECK();
     to_complex<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, g_pm.A, n3);
     CT2_LAUNCH_CHECK();
@@ -209,7 +236,10 @@ int pm_forces() {
     CT2_LAUNCH_CHECK();
     CT2_TRY(fft3d(g_pm.A, g_pm.ng, -1, false));
     CT2_TRY(fft3d(g_pm.B, g_pm.ng, -1, false));
-    cic_interp<<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.A, g_pm.B, c.fx, c.fy, c.fz, c.pot);
+    if (g_assign == 1)
+        mesh_interp<1><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.A, g_pm.B, c.fx, c.fy, c.fz, c.pot);
+    else
+        mesh_interp<2><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.A, g_pm.B, c.fx, c.fy, c.fz, c.pot);
     CT2_LAUNCH_CHECK();
     if (near_enabled()) CT2_TRY(near_forces(g_pm.rs_cells / g_pm.ng));
     c.forces_valid = true;
@@ -264,6 +294,13 @@ extern "C" int ct2_pm_near(double rcut_rs) {
 }
 
 extern "C" int ct2_pm_near_cells(void) { return near_cells(); }
+
+extern "C" int ct2_pm_assignment(int order) {
+    CT2_TRY(require_init());
+    if (order != 1 && order != 2) return set_error(CT2_ERR_INVALID, "assignment order %d is not 1 (CIC) or 2 (TSC)", order);
+    g_assign = order;
+    return CT2_OK;
+}
 
 extern "C" int ct2_pm_forces(void) { return pm_forces(); }
 

@@ -39,7 +39,7 @@ def rel_err(F, J):
 
 
 def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, judge=16384, method="pm_mlp",
-        weights=None, timer=None, log=print, rs_cells=1.25, near=0.0):
+        weights=None, timer=None, log=print, rs_cells=1.25, near=0.0, order=1):
     """Runs the loop; returns a record with per-step force/step times and the
     judged force errors."""
     from ct2_amd import _lib, api
@@ -52,6 +52,7 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
     api.ic_grf(side, rms, -2.0, seed, H * a_start ** 0.5)
     api.set_force_options(1.0 / (50 * side), 1.0 / n, api.KERNEL_PLUMMER)
     if method in ("pm", "pm_mlp"):
+        api.pm_assignment(order)
         api.pm_init(ng, rs_cells)
         api.pm_near(near)
     if method == "pm_mlp":
@@ -106,7 +107,9 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
     p = api.particles_download()
     if method in ("pm", "pm_mlp"):
         api.pm_near(0.0)
+        api.pm_assignment(1)
     return {"side": side, "n": n, "ng": ng, "method": method, "rs_cells": rs_cells, "near_rcut_rs": near,
+            "assignment": {1: "cic", 2: "tsc"}[order],
             "steps": steps, "a_start": a_start, "a_end": a_end,
             "records": records, "v_rms": float(np.sqrt(np.mean(p.vx.astype(np.float64) ** 2)))}
 
@@ -121,6 +124,7 @@ def main() -> int:
     ap.add_argument("--weights", default=os.path.join(ROOT, "ct2_amd", "weights", "mlp_grf_ng2x.npz"))
     ap.add_argument("--rs", type=float, default=1.25, help="PM split radius in cells")
     ap.add_argument("--near", type=float, default=0.0, help="near-field cutoff in units of rs (0: off)")
+    ap.add_argument("--assign", type=int, default=1, choices=[1, 2], help="mass assignment: 1 CIC, 2 TSC")
     args = ap.parse_args()
     from ct2_amd import api
     from ct2_amd.timing import HipEvents
@@ -136,7 +140,7 @@ def main() -> int:
         w = t[0]
     ev = HipEvents(api.stream())
     rec = run(args.side, args.ng, steps=args.steps, judge=args.judge, method=args.method, weights=w, timer=ev.time,
-              log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=args.rs, near=args.near)
+              log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=args.rs, near=args.near, order=args.assign)
     print(json.dumps(rec))
     api.finalize()
     return 0

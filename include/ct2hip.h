@@ -147,6 +147,12 @@ int ct2_pm_forces(void);
 int ct2_pm_near(double rcut_rs);
 /* Cells per side of the last near-field evaluation (0 before the first). */
 int ct2_pm_near_cells(void);
+/* Mass assignment of the PM force: 1 = cloud in cell (default), 2 =
+ * triangular-shaped cloud (27 nodes, quadratic spline; the Green's table
+ * deconvolves sinc^6 instead of sinc^4).  TSC lowers the mesh's
+ * discretisation error about tenfold at a given split radius.  Kept across
+ * ct2_pm_init.  (ct2's own mesh assigns order-8 multipoles, particle.cpp:180-209.) */
+int ct2_pm_assignment(int order);
 /* kick() with the PM force: forces as ct2_pm_forces, then the epilogue of
  * kick.cpp:75-99. */
 int ct2_kick_pm(const ct2_kick_params* params, unsigned flags, int* max_rung);

@@ -75,9 +75,12 @@ void orc_fft_c2r_3d(const double* in, double* out, int N); /* fft_c2r_3d.cpp int
 /* ---- particle mesh (particle.cpp:116-271 shape, CIC assignment) ---- */
 void orc_cic_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
                      uint64_t* grid);
-void orc_pm_green_table(int ng, double gm, double rs_cells, double* table);
+/* order 1 = cloud in cell, 2 = triangular-shaped cloud */
+void orc_mesh_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g, int order,
+                      uint64_t* grid);
+void orc_pm_green_table(int ng, double gm, double rs_cells, int order, double* table);
 void orc_pm_forces(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
-                   double gm, double rs_cells, double* fx, double* fy, double* fz, double* pot);
+                   double gm, double rs_cells, int order, double* fx, double* fy, double* fz, double* pot);
 /* near-field complement of the Gaussian-filtered PM force (kick.cpp:105-125
  * structure: direct pairs within rcut, mesh beyond); rs, rcut in box units */
 void orc_pm_near(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, size_t m,

@@ -73,8 +73,9 @@ def _declare(L):
     L.orc_fft_r2c_3d.argtypes = [_P, _P, i]
     L.orc_fft_c2r_3d.argtypes = [_P, _P, i]
     L.orc_cic_deposit.argtypes = [sz, _P, _P, _P, i, _P]
-    L.orc_pm_green_table.argtypes = [i, d, d, _P]
-    L.orc_pm_forces.argtypes = [sz, _P, _P, _P, i, d, d, _P, _P, _P, _P]
+    L.orc_mesh_deposit.argtypes = [sz, _P, _P, _P, i, i, _P]
+    L.orc_pm_green_table.argtypes = [i, d, d, i, _P]
+    L.orc_pm_forces.argtypes = [sz, _P, _P, _P, i, d, d, i, _P, _P, _P, _P]
     L.orc_ewald_kick.argtypes = [sz, _P, _P, _P, sz, _P, d, d, d, i, i, _P, _P, _P, _P]
     L.orc_pm_near.argtypes = [sz, _P, _P, _P, sz, _P, d, d, i, d, d, _P, _P, _P, _P]
     L.orc_mlp_num_params.restype = sz
@@ -222,18 +223,27 @@ def cic_deposit(x, y, z, g):
     return grid.reshape(ng, ng, ng)
 
 
-def pm_green_table(ng, gm, rs_cells=1.25):
+def mesh_deposit(x, y, z, g, order=1):
+    """Mass assignment of `order` (1 CIC, 2 TSC) in 2^-32 particle units."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    ng = 1 << g
+    grid = np.empty(ng ** 3, np.uint64)
+    lib().orc_mesh_deposit(x.size, _p(x), _p(y), _p(z), g, order, _p(grid))
+    return grid.reshape(ng, ng, ng)
+
+
+def pm_green_table(ng, gm, rs_cells=1.25, order=1):
     h = ng // 2 + 1
     t = np.empty(h ** 3)
-    lib().orc_pm_green_table(ng, gm, rs_cells, _p(t))
+    lib().orc_pm_green_table(ng, gm, rs_cells, order, _p(t))
     return t.reshape(h, h, h)
 
 
-def pm_forces(x, y, z, g, gm, rs_cells=1.25):
+def pm_forces(x, y, z, g, gm, rs_cells=1.25, order=1):
     """FP64 PM force and potential at the particles: (fx, fy, fz, pot)."""
     x, y, z = _u32(x), _u32(y), _u32(z)
     out = [np.empty(x.size) for _ in range(4)]
-    lib().orc_pm_forces(x.size, _p(x), _p(y), _p(z), g, gm, rs_cells, *[_p(o) for o in out])
+    lib().orc_pm_forces(x.size, _p(x), _p(y), _p(z), g, gm, rs_cells, order, *[_p(o) for o in out])
     return tuple(out)
 
 

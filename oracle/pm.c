@@ -11,12 +11,17 @@
  * (order 0) and keeps the same pipeline shape: deposit -> FFT -> multiply by
  * an FP64 Green's table -> inverse FFT -> interpolate at the particles.
  *
- * CIC definition (shared bit for bit with the GPU):
- *   grid ng = 2^g, node c at c/ng; s = 32 - g; for a fixed32 coordinate x:
- *   cell i0 = x >> s, fraction t = x & (2^s - 1), a1 = t >> (s - 21),
- *   a0 = 2^21 - a1 (21-bit weights); the node (i0+dx, j0+dy, k0+dz) gets
- *   W = (ax * ay * az) >> 31 with a = a1 for offset 1, a0 for offset 0,
- *   accumulated in uint64 (units of 2^-32 particles).
+ * Mass-assignment definitions (shared bit for bit with the GPU), grid
+ * ng = 2^g, node c at c/ng, s = 32 - g, 21-bit weights per axis summing to 2^21:
+ *   CIC (order 1): cell i0 = x >> s, fraction t = x & (2^s - 1),
+ *     a1 = t >> (s - 21) for node i0 + 1, a0 = 2^21 - a1 for node i0;
+ *   TSC (order 2): nearest node i0 = (x + 2^(s-1)) >> s, signed offset
+ *     D = ((x + 2^(s-1)) & (2^s - 1)) - 2^(s-1), shifted right (arithmetic) by
+ *     s - 21 into [-2^20, 2^20); node i0 - 1 gets round((2^20 - D)^2 / 2^22),
+ *     node i0 + 1 gets round((2^20 + D)^2 / 2^22) (the quadratic spline
+ *     (1/2)(1/2 -+ d)^2), node i0 the rest of 2^21.
+ *   A node gets W = (wx * wy * wz) >> 31, accumulated in uint64 (units of
+ *   2^-32 particles).
  */
 #include <math.h>
 #include <stdint.h>
@@ -27,27 +32,41 @@
 
 #define CIC_BITS 21
 
-static void cic_weights(uint32_t x, int g, uint32_t* i0, uint64_t a[2]) {
+/* Per-axis weights of `order` (1 CIC, 2 TSC): nodes lo, lo+1[, lo+2]. */
+static int assign_weights(uint32_t x, int g, int order, uint32_t* lo, uint64_t w[3]) {
     const int s = 32 - g;
-    *i0 = x >> s;
-    const uint32_t t = x & (uint32_t)((1ull << s) - 1ull);
-    a[1] = (uint64_t)(t >> (s - CIC_BITS));
-    a[0] = (1ull << CIC_BITS) - a[1];
+    if (order == 1) {
+        *lo = x >> s;
+        const uint32_t t = x & (uint32_t)((1ull << s) - 1ull);
+        w[1] = (uint64_t)(t >> (s - CIC_BITS));
+        w[0] = (1ull << CIC_BITS) - w[1];
+        return 2;
+    }
+    const uint32_t half = 1u << (s - 1);
+    const uint32_t xs = x + half;
+    const int32_t d = (int32_t)(xs & (uint32_t)((1ull << s) - 1ull)) - (int32_t)half;
+    const int64_t D = (int64_t)(d >> (s - CIC_BITS));
+    const uint64_t a = (uint64_t)((1 << 20) - D), b = (uint64_t)((1 << 20) + D);
+    w[0] = (a * a + (1ull << 21)) >> 22;
+    w[2] = (b * b + (1ull << 21)) >> 22;
+    w[1] = (1ull << CIC_BITS) - w[0] - w[2];
+    *lo = (xs >> s) - 1u;
+    return 3;
 }
 
-void orc_cic_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
-                     uint64_t* grid) {
+void orc_mesh_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g, int order,
+                      uint64_t* grid) {
     const uint32_t ng = 1u << g, mask = ng - 1;
     memset(grid, 0, sizeof(uint64_t) * (size_t)ng * ng * ng);
     for (size_t p = 0; p < n; p++) {
         uint32_t i0, j0, k0;
-        uint64_t ax[2], ay[2], az[2];
-        cic_weights(x[p], g, &i0, ax);
-        cic_weights(y[p], g, &j0, ay);
-        cic_weights(z[p], g, &k0, az);
-        for (int dx = 0; dx < 2; dx++)
-            for (int dy = 0; dy < 2; dy++)
-                for (int dz = 0; dz < 2; dz++) {
+        uint64_t ax[3], ay[3], az[3];
+        const int nw = assign_weights(x[p], g, order, &i0, ax);
+        assign_weights(y[p], g, order, &j0, ay);
+        assign_weights(z[p], g, order, &k0, az);
+        for (int dx = 0; dx < nw; dx++)
+            for (int dy = 0; dy < nw; dy++)
+                for (int dz = 0; dz < nw; dz++) {
                     const size_t c = ((size_t)((i0 + dx) & mask) * ng + ((j0 + dy) & mask)) * ng +
                                      ((k0 + dz) & mask);
                     grid[c] += (ax[dx] * ay[dy] * az[dz]) >> 31;
@@ -55,15 +74,20 @@ void orc_cic_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint3
     }
 }
 
+void orc_cic_deposit(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
+                     uint64_t* grid) {
+    orc_mesh_deposit(n, x, y, z, g, 1, grid);
+}
+
 /* Green's function of the PM force, FP64: for k = 2 pi m != 0,
- *   G(m) = -4 pi GM exp(-k^2 rs^2) / (k^2 W(m)^2),  W = prod_d sinc^2(pi m_d / ng),
- * rs = rs_cells / ng.  W is the CIC window, deconvolved once for the deposit
+ *   G(m) = -4 pi GM exp(-k^2 rs^2) / (k^2 W(m)^2),  W = prod_d sinc^(order+1)(pi m_d / ng),
+ * rs = rs_cells / ng.  W is the CIC (TSC) window, deconvolved once for the deposit
  * and once for the interpolation; the Gaussian keeps the long-range part of
  * the force (the split of TreePM/P3M codes) so that the grid-scale modes that
  * the deconvolution amplifies are cut off.  G(0) = 0 (mean density removed:
  * the periodic, Ewald-consistent solution).
  * Tabulated on the octant |m_d| in [0, ng/2]: table[(a*h + b)*h + c], h = ng/2+1. */
-void orc_pm_green_table(int ng, double gm, double rs_cells, double* table) {
+void orc_pm_green_table(int ng, double gm, double rs_cells, int order, double* table) {
     const double rs = rs_cells / ng;
     const int h = ng / 2 + 1;
     for (int a = 0; a < h; a++)
@@ -76,7 +100,7 @@ void orc_pm_green_table(int ng, double gm, double rs_cells, double* table) {
                     for (int d = 0; d < 3; d++) {
                         const double u = M_PI * m[d] / ng;
                         const double sc = m[d] ? sin(u) / u : 1.0;
-                        w *= sc * sc;
+                        w *= order == 1 ? sc * sc : sc * sc * sc;
                         k2 += (2.0 * M_PI * m[d]) * (2.0 * M_PI * m[d]);
                     }
                     v = -4.0 * M_PI * gm * exp(-k2 * rs * rs) / (k2 * w * w);
@@ -87,14 +111,14 @@ void orc_pm_green_table(int ng, double gm, double rs_cells, double* table) {
 
 /* Full FP64 PM force and potential at the particles:
  * deposit -> forward FFT -> phi = G rho -> F = -i k phi (Nyquist component of
- * the gradient zeroed), pot = -phi -> inverse FFTs -> CIC interpolation with
- * the same weights as the deposit. */
+ * the gradient zeroed), pot = -phi -> inverse FFTs -> interpolation with
+ * the same weights as the deposit (order 1 CIC, 2 TSC). */
 void orc_pm_forces(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int g,
-                   double gm, double rs_cells, double* fx, double* fy, double* fz, double* pot) {
+                   double gm, double rs_cells, int order, double* fx, double* fy, double* fz, double* pot) {
     const int ng = 1 << g, h = ng / 2 + 1;
     const size_t n3 = (size_t)ng * ng * ng;
     uint64_t* W = (uint64_t*)malloc(sizeof(uint64_t) * n3);
-    orc_cic_deposit(n, x, y, z, g, W);
+    orc_mesh_deposit(n, x, y, z, g, order, W);
     double* rho = (double*)malloc(sizeof(double) * 2 * n3);
     for (size_t c = 0; c < n3; c++) {
         rho[2 * c] = (double)W[c] * 0x1p-32;
@@ -103,7 +127,7 @@ void orc_pm_forces(size_t n, const uint32_t* x, const uint32_t* y, const uint32_
     free(W);
     orc_fft3d(rho, ng, 1);
     double* G = (double*)malloc(sizeof(double) * (size_t)h * h * h);
-    orc_pm_green_table(ng, gm, rs_cells, G);
+    orc_pm_green_table(ng, gm, rs_cells, order, G);
     double* out[4];
     for (int q = 0; q < 4; q++) out[q] = (double*)malloc(sizeof(double) * 2 * n3);
     for (int i = 0; i < ng; i++)
@@ -130,14 +154,14 @@ void orc_pm_forces(size_t n, const uint32_t* x, const uint32_t* y, const uint32_
     double* dst[4] = {fx, fy, fz, pot};
     for (size_t p = 0; p < n; p++) {
         uint32_t i0, j0, k0;
-        uint64_t ax[2], ay[2], az[2];
-        cic_weights(x[p], g, &i0, ax);
-        cic_weights(y[p], g, &j0, ay);
-        cic_weights(z[p], g, &k0, az);
+        uint64_t ax[3], ay[3], az[3];
+        const int nw = assign_weights(x[p], g, order, &i0, ax);
+        assign_weights(y[p], g, order, &j0, ay);
+        assign_weights(z[p], g, order, &k0, az);
         double acc[4] = {0, 0, 0, 0};
-        for (int dx = 0; dx < 2; dx++)
-            for (int dy = 0; dy < 2; dy++)
-                for (int dz = 0; dz < 2; dz++) {
+        for (int dx = 0; dx < nw; dx++)
+            for (int dy = 0; dy < nw; dy++)
+                for (int dz = 0; dz < nw; dz++) {
                     const size_t c = ((size_t)((i0 + dx) & mask) * ng + ((j0 + dy) & mask)) * ng +
                                      ((k0 + dz) & mask);
                     const double w = (double)ax[dx] * (double)ay[dy] * (double)az[dz] * 0x1p-63;

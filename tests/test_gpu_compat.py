@@ -46,3 +46,29 @@ def test_kick_and_drift_through_ct2_signatures(orc):
     wx, wy, wz = orc.drift(x0, y0, z0, vx, vy, vz, 0.01, 1.0)
     assert np.array_equal(x, wx) and np.array_equal(y, wy) and np.array_equal(z, wz)
     assert max_rung == rung.max()
+
+
+def test_fft_api_through_ct2_signatures(orc):
+    """scramble + fft_real and fft_r2c_3d / fft_c2r_3d through ct2's own
+    signatures (fft.hpp:18-29), host arrays in and out, as fft_test.cpp calls
+    them; checked against the oracle and the round trip."""
+    r = subprocess.run([EXE, "fft"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    v = np.array([float(t) for t in r.stdout.split()])
+    N, M = 1024, 16
+    h = M // 2 + 1
+    s = 777
+    vals = []
+    for _ in range(N + M ** 3):
+        s = (s * 1664525 + 1013904223) % 2 ** 32
+        vals.append(s / 2 ** 32 - 0.5)
+    x, cube = np.array(vals[:N]), np.array(vals[N:]).reshape(M, M, M)
+    want = orc.fft_real(orc.scramble(x, 1, N, 1), N)
+    got, v = v[:N], v[N:]
+    assert np.linalg.norm(got - want) < 1e-14 * np.linalg.norm(want)
+    spec, v = v[:2 * M * M * h], v[2 * M * M * h:]
+    ws = orc.fft_r2c_3d(cube)
+    assert np.linalg.norm(spec - ws) < 1e-14 * np.linalg.norm(ws)
+    back, v = v[:M ** 3], v[M ** 3:]
+    assert np.allclose(back / M ** 3, cube.ravel(), atol=1e-14)
+    assert np.allclose(v, orc.twiddles(8)[0], atol=1e-16)

@@ -134,3 +134,37 @@ def test_pm_512_cubed_runs_and_conserves_momentum(ct2):
     for a in (fx, fy, fz):
         assert np.isfinite(a).all()
         assert abs(a.astype(np.float64).sum()) < 1e-4 * np.abs(a).sum()
+
+
+@pytest.fixture
+def tsc(ct2):
+    ct2.pm_assignment(2)
+    yield
+    ct2.pm_assignment(1)
+
+
+@pytest.mark.parametrize("ng", [16, 128])
+def test_tsc_deposit_bit_exact(ct2, orc, tsc, ng):
+    x, y, z = ct2.ic_plummer_mix(50000, 30000, 3, 0.005, 0.05, 22)
+    load(ct2, x, y, z)
+    ct2.set_force_options(1e-3, 1.0 / x.size, 0)
+    ct2.pm_init(ng, 1.25)
+    ct2.pm_forces()
+    assert np.array_equal(ct2.pm_density_download(), orc.mesh_deposit(x, y, z, ng.bit_length() - 1, 2))
+
+
+@pytest.mark.parametrize("ng,rs", [(32, 1.25), (64, 2.0)])
+def test_tsc_pm_forces_vs_fp64_oracle(ct2, orc, tsc, ng, rs):
+    x, y, z = ct2.ic_plummer_mix(30000, 15000, 8, 0.02, 0.2, 4)
+    n = x.size
+    load(ct2, x, y, z)
+    gm = f32(1.0 / n)
+    ct2.set_force_options(1e-3, gm, 0)
+    ct2.pm_init(ng, rs)
+    fx, fy, fz, pot = ct2.pm_forces()
+    ox, oy, oz, op = orc.pm_forces(x, y, z, ng.bit_length() - 1, gm, rs, order=2)
+    F = np.stack([fx, fy, fz])
+    O = np.stack([ox, oy, oz])
+    rms = np.sqrt(np.mean(O ** 2))
+    assert np.max(np.abs(F - O)) < 2e-5 * rms * np.sqrt(ng), np.max(np.abs(F - O)) / rms
+    assert np.max(np.abs(pot - op)) < 1e-5 * np.abs(op).max()

@@ -121,3 +121,19 @@ def test_series_branch_is_continuous(orc):
         vals.append((plummer - ax[0]) / rr)  # the filtered kernel divided by r
     assert vals[0] == pytest.approx(vals[1], rel=1e-3)
     assert vals[0] == pytest.approx(2 / np.sqrt(np.pi) * (2 / 3) / (2 * rs) ** 3, rel=1e-5)
+
+
+def test_tsc_mesh_is_ten_times_more_accurate(orc, case):
+    """With the quadratic-spline assignment the mesh's discretisation error at
+    a given split radius falls by about an order of magnitude."""
+    J = case["J"]
+    rs_cells = 2.0
+    rs = rs_cells / case["ng"]
+    nr = orc.pm_near(case["x"], case["y"], case["z"], case["gm"], case["soft"], rs, 5.5 * rs, sinks=case["sinks"])
+    errs = []
+    for order in (1, 2):
+        pm = orc.pm_forces(case["x"], case["y"], case["z"], case["ng"].bit_length() - 1, case["gm"], rs_cells,
+                           order=order)
+        F = np.stack([pm[a][case["sinks"]] + nr[a] for a in range(3)])
+        errs.append(np.median(rel(F, J[:3])))
+    assert errs[1] < 1e-3 and errs[1] < errs[0] / 5, errs
