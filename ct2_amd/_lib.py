@@ -73,6 +73,12 @@ _SIGS = {
     "ct2_pm_init": ([C.c_int, C.c_double], C.c_int),
     "ct2_pm_forces": ([], C.c_int),
     "ct2_pm_near": ([C.c_double], C.c_int),
+    "ct2_particle_sort_by_rung": ([C.c_int], C.c_int),
+    "ct2_particle_push_rungs": ([], C.c_int),
+    "ct2_particle_pop_rungs": ([], C.c_int),
+    "ct2_particle_current_range": ([_P, _P], C.c_int),
+    "ct2_particle_rung_counts": ([_P, C.c_int, _P], C.c_int),
+    "ct2_particle_set_minrung": ([C.c_int], C.c_int),
     "ct2_pm_near_cells": ([], C.c_int),
     "ct2_pm_assignment": ([C.c_int], C.c_int),
     "ct2_kick_pm": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),

@@ -131,6 +131,37 @@ def drift(dt: float, scale: float) -> None:
     L.check(L.load().ct2_drift(dt, scale))
 
 
+# ---------------------------------------------------------------- rungs
+def particle_sort_by_rung(minrung: int) -> None:
+    """particle_sort_by_rung (particle.cpp:320-340), stable."""
+    L.check(L.load().ct2_particle_sort_by_rung(minrung))
+
+
+def particle_push_rungs() -> None:
+    L.check(L.load().ct2_particle_push_rungs())
+
+
+def particle_pop_rungs() -> None:
+    L.check(L.load().ct2_particle_pop_rungs())
+
+
+def particle_current_range() -> tuple[int, int]:
+    a, b = C.c_size_t(0), C.c_size_t(0)
+    L.check(L.load().ct2_particle_current_range(C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def particle_rung_counts() -> list[int]:
+    counts = np.zeros(128, np.uint64)
+    nr = C.c_int(0)
+    L.check(L.load().ct2_particle_rung_counts(_p(counts), 128, C.byref(nr)))
+    return [int(v) for v in counts[:nr.value]]
+
+
+def particle_set_minrung(minrung: int) -> None:
+    L.check(L.load().ct2_particle_set_minrung(minrung))
+
+
 # ---------------------------------------------------------------- particle mesh
 _pm_ng = 0
 

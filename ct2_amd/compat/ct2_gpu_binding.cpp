@@ -19,6 +19,9 @@ char* particle_rung_ptr = nullptr;
 namespace {
 size_t g_count = 0;
 std::vector<char> g_store;
+float g_eta = 0.15f;
+int g_mesh_ng = 0;  // 0: direct-sum kick
+std::FILE* g_trace = nullptr;
 
 // ct2 aborts on allocation failure (memory.cpp:7-10); the binding does the
 // same on any library error, with the library's message.
@@ -38,11 +41,24 @@ void ensure_init() {
 }
 }  // namespace
 
-void ct2_gpu_set_options(float hsoft, float GM) {
+void ct2_gpu_set_options(float hsoft, float GM, float eta) {
     ensure_init();
     ct2_force_options o{hsoft, GM, CT2_KERNEL_PLUMMER};
     check(ct2_set_force_options(&o), "ct2_set_force_options");
+    g_eta = eta;
 }
+
+void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order) {
+    ensure_init();
+    g_mesh_ng = ng;
+    if (ng > 0) {
+        check(ct2_pm_assignment(order), "ct2_pm_assignment");
+        check(ct2_pm_init(ng, rs_cells), "ct2_pm_init");
+        check(ct2_pm_near(rcut_rs), "ct2_pm_near");
+    }
+}
+
+void ct2_gpu_trace(std::FILE* f) { g_trace = f; }
 
 void particle_allocate(size_t count) {
     ensure_init();
@@ -68,7 +84,16 @@ int kick(const kick_params& p) {
           "upload");
     const ct2_kick_params q{p.ascending, p.descending, p.top, p.first, p.rung, p.t0, p.eta, p.max_dt, p.scale};
     int max_rung = 0;
-    check(ct2_kick(&q, CT2_KICK_STORE, &max_rung), "ct2_kick");
+    if (g_mesh_ng > 0)
+        check(ct2_kick_pm(&q, CT2_KICK_STORE, &max_rung), "ct2_kick_pm");
+    else
+        check(ct2_kick(&q, CT2_KICK_STORE, &max_rung), "ct2_kick");
+    if (g_trace) {
+        size_t first = 0, last = 0;
+        check(ct2_particle_current_range(&first, &last), "ct2_particle_current_range");
+        std::fprintf(g_trace, "kick rung %d asc %d desc %d top %d first %d range %zu -> max_rung %d\n", p.rung,
+                     (int)p.ascending, (int)p.descending, (int)p.top, (int)p.first, first, max_rung);
+    }
     check(ct2_particles_download(nullptr, nullptr, nullptr, particle_vx_ptr, particle_vy_ptr, particle_vz_ptr,
                                  (int8_t*)particle_rung_ptr),
           "download");
@@ -76,12 +101,126 @@ int kick(const kick_params& p) {
 }
 
 void drift(float dt, float scale) {
+    if (g_trace) std::fprintf(g_trace, "drift %.9e %.9e\n", dt, scale);
     check(ct2_particles_upload(particle_x_ptr, particle_y_ptr, particle_z_ptr, particle_vx_ptr, particle_vy_ptr,
                                particle_vz_ptr, nullptr),
           "upload");
     check(ct2_drift(dt, scale), "ct2_drift");
     check(ct2_particles_download(particle_x_ptr, particle_y_ptr, particle_z_ptr, nullptr, nullptr, nullptr, nullptr),
           "download");
+}
+
+// ---- rung bookkeeping: the host arrays stay authoritative, as in ct2
+namespace {
+void upload_all() {
+    check(ct2_particles_upload(particle_x_ptr, particle_y_ptr, particle_z_ptr, particle_vx_ptr, particle_vy_ptr,
+                               particle_vz_ptr, (const int8_t*)particle_rung_ptr),
+          "upload");
+}
+}  // namespace
+
+void particle_sort_by_rung(int minrung) {
+    upload_all();
+    check(ct2_particle_sort_by_rung(minrung), "ct2_particle_sort_by_rung");
+    check(ct2_particles_download(particle_x_ptr, particle_y_ptr, particle_z_ptr, particle_vx_ptr, particle_vy_ptr,
+                                 particle_vz_ptr, (int8_t*)particle_rung_ptr),
+          "download");
+    if (g_trace) std::fprintf(g_trace, "sort_by_rung %d\n", minrung);
+}
+
+std::vector<size_t> particle_rung_counts() {
+    check(ct2_particles_upload(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               (const int8_t*)particle_rung_ptr),
+          "upload");
+    uint64_t c[128];
+    int nr = 0;
+    check(ct2_particle_rung_counts(c, 128, &nr), "ct2_particle_rung_counts");
+    if (g_trace) {
+        std::fprintf(g_trace, "rung_counts");
+        for (int r = 0; r < nr; ++r) std::fprintf(g_trace, " %llu", (unsigned long long)c[r]);
+        std::fprintf(g_trace, "\n");
+    }
+    return std::vector<size_t>(c, c + nr);
+}
+
+std::pair<size_t, size_t> particle_current_range() {
+    size_t first = 0, last = 0;
+    check(ct2_particle_current_range(&first, &last), "ct2_particle_current_range");
+    return {first, last};
+}
+
+void particle_to_grid() {}
+
+void particle_pop_rungs() {
+    check(ct2_particle_pop_rungs(), "ct2_particle_pop_rungs");
+    if (g_trace) std::fprintf(g_trace, "pop\n");
+}
+
+void particle_push_rungs() {
+    check(ct2_particle_push_rungs(), "ct2_particle_push_rungs");
+    if (g_trace) std::fprintf(g_trace, "push\n");
+}
+
+void particle_set_minrung(int minrung) {
+    check(ct2_particles_upload(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               (const int8_t*)particle_rung_ptr),
+          "upload");
+    check(ct2_particle_set_minrung(minrung), "ct2_particle_set_minrung");
+    check(ct2_particles_download(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, (int8_t*)particle_rung_ptr),
+          "download");
+    if (g_trace) std::fprintf(g_trace, "set_minrung %d\n", minrung);
+}
+
+// The level schedule of kick_step (main.cpp:8-92): levels max_rung .. minrung+1
+// kick the ascending half only; minrung is the top level (both halves, sort by
+// rung); a kick that reports faster particles queues the next finer level;
+// drifts follow each descending kick with that level's dt; when the rungs
+// above minrung0 hold more than a third of the slow particles, the top level is
+// clipped (nocrop off) and minrung rises by one.
+int ct2_gpu_kick_step(int& minrung, int max_rung, double scale, double adot, double tau, double t0, int minrung0,
+                      bool nocrop) {
+    std::vector<int> levels;
+    for (int r = max_rung; r > minrung; --r) levels.push_back(r);
+    levels.push_back(minrung);
+    bool ascending = true, clip_top = false;
+    for (size_t li = 0; li < levels.size(); ++li) {
+        const int level = levels[li];
+        const bool top = level == minrung;
+        if (top) ascending = false;
+        if (!ascending && !top) particle_push_rungs();
+        if (!ascending) particle_sort_by_rung(level);
+        if (top && minrung == minrung0) {
+            const std::vector<size_t> counts = particle_rung_counts();
+            if (counts.size() > (size_t)minrung0 + 1) {
+                size_t fast = 0;
+                for (size_t r = minrung0 + 1; r < counts.size(); ++r) fast += counts[r];
+                if (3 * fast > counts[minrung0]) clip_top = !nocrop;
+            }
+        }
+        particle_to_grid();
+        const bool clipped_top = clip_top && top;
+        kick_params kp;
+        kp.top = (clip_top && level == minrung0 + 1) || top;
+        kp.ascending = ascending || top;
+        kp.descending = clipped_top ? false : (!ascending || top);
+        kp.eta = g_eta;
+        kp.scale = (float)scale;
+        kp.first = tau == 0.0;
+        kp.rung = level;
+        kp.t0 = (float)t0;
+        kp.max_dt = (float)(0.01f / adot);
+        const int reached = kick(kp);
+        if (clipped_top) {
+            particle_set_minrung(minrung0 + 1);
+            levels.push_back(minrung0 + 1);
+        } else if ((!ascending || top) && reached > level) {
+            levels.push_back(level + 1);
+        }
+        if ((!ascending || top) && !clipped_top) drift((float)(t0 / (double)(1 << level)), (float)scale);
+        if (ascending && !top) particle_pop_rungs();
+    }
+    if (clip_top) ++minrung;
+    return max_rung;
 }
 
 // ---- FFT API: host arrays in and out, the transform on the device.

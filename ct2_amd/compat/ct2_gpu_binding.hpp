@@ -10,6 +10,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <utility>
 #include <vector>
 
 // sfmm::fixed32 is a 32-bit unsigned fixed-point position (cosmictiger.hpp:22).
@@ -41,8 +43,29 @@ size_t particle_count();               // particle.cpp:27
 int kick(const kick_params& params);   // kick.hpp:24
 void drift(float dt, float scale);     // drift.hpp:12
 
-// Options the force path reads from get_options() (options.cpp:155,190).
-void ct2_gpu_set_options(float hsoft, float GM);
+// Rung bookkeeping of particle.hpp:57-66 (the primitives ct2's kick_step,
+// main.cpp:8-92, calls); the sort permutes the particle arrays.
+void particle_sort_by_rung(int minrung);
+std::vector<size_t> particle_rung_counts();
+std::pair<size_t, size_t> particle_current_range();
+void particle_to_grid();  // no work here: the library bins cells itself
+void particle_pop_rungs();
+void particle_push_rungs();
+void particle_set_minrung(int minrung);
+
+// Options the force path reads from get_options() (options.cpp:58,155,190).
+void ct2_gpu_set_options(float hsoft, float GM, float eta = 0.15f);
+// kick() takes the direct sum by default; this switches it to PM + near field
+// (ct2_pm_assignment, ct2_pm_init, ct2_pm_near); ng = 0 switches back.
+void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order);
+
+// kick_step (kick.hpp:23, main.cpp:8-92) over the functions above, for
+// callers that do not link ct2's main.cpp (which runs unchanged on them).
+// Returns its max_rung argument, as the reference does.
+int ct2_gpu_kick_step(int& minrung, int max_rung, double scale, double adot, double tau, double t0, int minrung0,
+                      bool nocrop);
+// Optional trace of kick_step's calls (one line per kick / drift / rung call).
+void ct2_gpu_trace(std::FILE* f);
 
 // ---- the FFT API of fft.hpp:18-29 (fft_float is double, cosmictiger.hpp:18-23)
 using fft_float = double;

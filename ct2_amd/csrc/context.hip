@@ -118,6 +118,7 @@ int ct2_finalize(void) {
     release_mlp();
     release_near();
     release_fft_ct2();
+    release_rungs();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device = -1;
@@ -166,6 +167,8 @@ int ct2_particles_alloc(size_t n) {
         return rc;
     }
     c.n = n;
+    c.rung_begin = 0;
+    c.rung_stack.clear();
     for (void* p : {(void*)c.x, (void*)c.y, (void*)c.z, (void*)c.vx, (void*)c.vy, (void*)c.vz,
                     (void*)c.fx, (void*)c.fy, (void*)c.fz, (void*)c.pot})
         CT2_HIP(hipMemsetAsync(p, 0, b4, c.stream));

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "ct2hip.h"
 
@@ -34,6 +35,11 @@ struct Context {
     double* partial = nullptr;
     size_t partial_bytes = 0;
     int* maxrung = nullptr;
+
+    // rung bookkeeping (particle.cpp:20-58): kicks act on the sinks
+    // [rung_begin, n); push/pop save and restore rung_begin
+    size_t rung_begin = 0;
+    std::vector<size_t> rung_stack;
 };
 
 Context& ctx();
@@ -77,6 +83,7 @@ void release_ewald();
 void release_mlp();
 void release_near();
 void release_fft_ct2();
+void release_rungs();
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(long v) {

@@ -71,21 +71,22 @@ __device__ __forceinline__ void interact(const uint4 src, uint32_t xi, uint32_t 
     }
 }
 
-// grid: (ceil(nsink/KICK_BLOCK), nsplit); block y sums sources
-// [y*jchunk, min(n, (y+1)*jchunk)).  Output: partial[(a*nsplit + y)*nsink + i]
-// for a in {x,y,z,pot}, in units of 2^-64 (force) and 2^-32... see scaling
-// in kick_finalize.
+// grid: (ceil(nsink/KICK_BLOCK), nsplit), sinks i0 .. n-1 (the active rung
+// range); block y sums sources [y*jchunk, min(n, (y+1)*jchunk)) of all n.
+// Output: partial[(a*nsplit + y)*nsink + i] for a in {x,y,z,pot}, in units
+// of 2^-64 (force) and 2^-32 (potential); see the scaling in kick_finalize.
 template <int KERNEL, bool NAIVE>
 __global__ __launch_bounds__(KICK_BLOCK) void kick_direct(const uint32_t* __restrict__ X,
                                                           const uint32_t* __restrict__ Y,
-                                                          const uint32_t* __restrict__ Z, int n,
+                                                          const uint32_t* __restrict__ Z, int n, int i0,
                                                           int jchunk, KickConst kc,
                                                           double* __restrict__ partial,
                                                           int* __restrict__ maxrung) {
     __shared__ uint4 tile[2][KICK_TILE];
     const int tid = threadIdx.x;
+    const int ns = n - i0;  // sinks i0 .. n-1
     const int i = blockIdx.x * KICK_BLOCK + tid;
-    const int ic = i < n ? i : n - 1;
+    const int ic = i < ns ? i0 + i : n - 1;
     const uint32_t xi = X[ic], yi = Y[ic], zi = Z[ic];
     if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) *maxrung = 0;
 
@@ -130,30 +131,32 @@ __global__ __launch_bounds__(KICK_BLOCK) void kick_direct(const uint32_t* __rest
         }
         buf ^= 1;
     }
-    if (i < n) {
-        const long ns = gridDim.y, y = blockIdx.y;
-        partial[(0 * ns + y) * n + i] = (double)sx + (double)cx;
-        partial[(1 * ns + y) * n + i] = (double)sy + (double)cy;
-        partial[(2 * ns + y) * n + i] = (double)sz + (double)cz;
-        partial[(3 * ns + y) * n + i] = (double)sp + (double)cp;
+    if (i < ns) {
+        const long nsp = gridDim.y, y = blockIdx.y;
+        partial[(0 * nsp + y) * ns + i] = (double)sx + (double)cx;
+        partial[(1 * nsp + y) * ns + i] = (double)sy + (double)cy;
+        partial[(2 * nsp + y) * ns + i] = (double)sz + (double)cz;
+        partial[(3 * nsp + y) * ns + i] = (double)sp + (double)cp;
     }
 }
 
 __global__ __launch_bounds__(256) void kick_finalize(const double* __restrict__ partial, int nsplit,
-                                                     int n, double fscale, double pscale, EpiConst e,
+                                                     int n, int i0, double fscale, double pscale, EpiConst e,
                                                      float* __restrict__ vx, float* __restrict__ vy,
                                                      float* __restrict__ vz, int8_t* __restrict__ rung,
                                                      float* __restrict__ fx, float* __restrict__ fy,
                                                      float* __restrict__ fz, float* __restrict__ pot,
                                                      int* __restrict__ maxrung) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int ns = n - i0;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = i0 + t;
     int rmax = 0;
-    if (i < n) {
+    if (t < ns) {
         double a[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             double s = 0.0;
-            for (int y = 0; y < nsplit; ++y) s += partial[((long)c * nsplit + y) * n + i];
+            for (int y = 0; y < nsplit; ++y) s += partial[((long)c * nsplit + y) * ns + t];
             a[c] = s;
         }
         const float f0 = (float)(a[0] * fscale), f1 = (float)(a[1] * fscale), f2 = (float)(a[2] * fscale);
@@ -178,11 +181,11 @@ __global__ __launch_bounds__(256) void kick_finalize(const double* __restrict__ 
 
 // Epilogue over stored forces (mesh-based kicks).
 __global__ __launch_bounds__(256) void epilogue_kernel(const float* __restrict__ fx, const float* __restrict__ fy,
-                                                       const float* __restrict__ fz, int n, EpiConst e,
+                                                       const float* __restrict__ fz, int n, int i0, EpiConst e,
                                                        float* __restrict__ vx, float* __restrict__ vy,
                                                        float* __restrict__ vz, int8_t* __restrict__ rung,
                                                        int* __restrict__ maxrung) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = i0 + blockIdx.x * 256 + threadIdx.x;
     int rmax = 0;
     if (i < n) {
         float u = vx[i], v = vy[i], w = vz[i];
@@ -296,10 +299,12 @@ int run_epilogue(const ct2_kick_params* p, int* max_rung) {
     CT2_LAUNCH_CHECK();
     if (epilogue_active(p)) {
         const EpiConst e = make_epilogue(p, c.opts.hsoft);
-        const int n = (int)c.n;
-        epilogue_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(c.fx, c.fy, c.fz, n, e, c.vx, c.vy, c.vz, c.rung,
-                                                               c.maxrung);
-        CT2_LAUNCH_CHECK();
+        const int n = (int)c.n, i0 = (int)c.rung_begin;
+        if (n > i0) {
+            epilogue_kernel<<<(n - i0 + 255) / 256, 256, 0, c.stream>>>(c.fx, c.fy, c.fz, n, i0, e, c.vx, c.vy, c.vz,
+                                                                         c.rung, c.maxrung);
+            CT2_LAUNCH_CHECK();
+        }
     }
     if (max_rung) {
         CT2_HIP(hipMemcpyAsync(max_rung, c.maxrung, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -319,6 +324,13 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
         return set_error(CT2_ERR_INVALID, "unknown kick flags 0x%x", flags);
     Context& c = ctx();
     const int n = (int)c.n;
+    const int i0 = (int)c.rung_begin;  // sinks: the active rung range [i0, n)
+    const int ns = n - i0;
+    if (!c.maxrung) CT2_TRY(dev_alloc((void**)&c.maxrung, sizeof(int), "max rung"));
+    if (ns <= 0) {  // no active sink: nothing to kick
+        if (max_rung) *max_rung = 0;
+        return CT2_OK;
+    }
     const double two32 = 4294967296.0;
     const double h = c.opts.hsoft;
     KickConst kc;
@@ -328,29 +340,28 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
     kc.hinv3u = (float)(1.0 / ((h * two32) * (h * two32) * (h * two32)));
 
     // sink blocks and source chunks: enough workgroups to fill 256 CUs
-    const int nb = (n + KICK_BLOCK - 1) / KICK_BLOCK;
+    const int nb = (ns + KICK_BLOCK - 1) / KICK_BLOCK;
     const int max_split = std::max(1, n / (4 * KICK_TILE));
     const int nsplit = std::min(max_split, std::max(1, (4096 + nb - 1) / nb));
     int jchunk = (n + nsplit - 1) / nsplit;
     jchunk = (jchunk + KICK_TILE - 1) / KICK_TILE * KICK_TILE;
     const int nsplit_eff = (n + jchunk - 1) / jchunk;
 
-    CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, sizeof(double) * 4 * (size_t)nsplit_eff * n,
+    CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, sizeof(double) * 4 * (size_t)nsplit_eff * ns,
                            "kick partial sums"));
-    if (!c.maxrung) CT2_TRY(dev_alloc((void**)&c.maxrung, sizeof(int), "max rung"));
 
     const dim3 grid(nb, nsplit_eff);
     const bool naive = flags & CT2_KICK_NAIVE;
     if (c.opts.kernel == CT2_KERNEL_PLUMMER) {
         if (naive)
-            kick_direct<CT2_KERNEL_PLUMMER, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+            kick_direct<CT2_KERNEL_PLUMMER, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
         else
-            kick_direct<CT2_KERNEL_PLUMMER, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+            kick_direct<CT2_KERNEL_PLUMMER, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
     } else {
         if (naive)
-            kick_direct<CT2_KERNEL_CT2, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+            kick_direct<CT2_KERNEL_CT2, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
         else
-            kick_direct<CT2_KERNEL_CT2, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, jchunk, kc, c.partial, c.maxrung);
+            kick_direct<CT2_KERNEL_CT2, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
     }
     CT2_LAUNCH_CHECK();
 
@@ -358,7 +369,7 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
     // force in box units: sum(dxu * w3u) * 2^64 * GM; potential: sum(wu) * 2^32 * GM
     const double fscale = (double)c.opts.GM * two32 * two32;
     const double pscale = (double)c.opts.GM * two32;
-    kick_finalize<<<(n + 255) / 256, 256, 0, c.stream>>>(c.partial, nsplit_eff, n, fscale, pscale, e, c.vx, c.vy,
+    kick_finalize<<<(ns + 255) / 256, 256, 0, c.stream>>>(c.partial, nsplit_eff, n, i0, fscale, pscale, e, c.vx, c.vy,
                                                           c.vz, c.rung, c.fx, c.fy, c.fz, c.pot, c.maxrung);
     CT2_LAUNCH_CHECK();
     c.forces_valid = true;

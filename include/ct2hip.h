@@ -195,6 +195,32 @@ int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, float* losses);
 int ct2_pm_mlp_forces(void);
 int ct2_kick_pm_mlp(const ct2_kick_params* params, unsigned flags, int* max_rung);
 
+/* ------------------------------------------------------------ rungs */
+/* ct2's rung bookkeeping (particle.cpp:20-58, 320-347), the primitives of
+ * kick_step (main.cpp:8-92).  Every kick (ct2_kick, ct2_kick_pm,
+ * ct2_kick_pm_mlp) acts on the sinks of the current range [first, n) --
+ * kick() grids only that range (particle.cpp:349-362) -- with every particle
+ * as a source; the PM kicks compute the mesh force for all particles and
+ * apply the epilogue to the range.  After ct2_particles_alloc the range is
+ * [0, n).
+ *   ct2_particle_sort_by_rung  particle_sort_by_rung (particle.cpp:320-340):
+ *       partitions [first, n) so that rungs < minrung come first and moves
+ *       first to the first particle with rung >= minrung; minrung 0 resets
+ *       first to 0.  The partition is stable (the reference's swap loop is
+ *       not); it permutes the container, and the stored forces are invalid
+ *       until the next kick.
+ *   ct2_particle_push_rungs / ct2_particle_pop_rungs  particle.cpp:31-37.
+ *   ct2_particle_current_range  particle_current_range (particle.cpp:53-58).
+ *   ct2_particle_rung_counts  particle_rung_counts (particle.cpp:39-51):
+ *       counts[r] for r < *nrungs = 1 + the highest rung present.
+ *   ct2_particle_set_minrung  particle_set_minrung (particle.cpp:342-347). */
+int ct2_particle_sort_by_rung(int minrung);
+int ct2_particle_push_rungs(void);
+int ct2_particle_pop_rungs(void);
+int ct2_particle_current_range(size_t* first, size_t* last);
+int ct2_particle_rung_counts(uint64_t* counts, int max_rungs, int* nrungs);
+int ct2_particle_set_minrung(int minrung);
+
 /* ------------------------------------------------------------ drift */
 /* Replaces void drift(float dt, float scale) (drift.hpp:12, drift.cpp:4-18):
  * x += fixed32(v * (dt/scale)) per axis, with fixed32(s) defined as

@@ -35,8 +35,48 @@ static int fft_mode() {
     return 0;
 }
 
+// Random particles plus tight pairs (fast rungs); kick_step five times with
+// the kick_step trace on stdout, then the final rungs.
+static int rungs_mode() {
+    const int n = 1024;
+    particle_allocate(n);
+    ct2_gpu_set_options(2e-3f, 1.0f / n, 0.15f);
+    unsigned s = 4242u;
+    for (int i = 0; i < n; i++) {
+        s = s * 1664525u + 1013904223u;
+        particle_x_ptr[i] = s;
+        s = s * 1664525u + 1013904223u;
+        particle_y_ptr[i] = s;
+        s = s * 1664525u + 1013904223u;
+        particle_z_ptr[i] = s;
+        particle_vx_ptr[i] = particle_vy_ptr[i] = particle_vz_ptr[i] = 0.0f;
+        particle_rung_ptr[i] = 0;
+    }
+    for (int k = 0; k < 48; k++) {  // partner 2k+1 sits 2^-12 .. 2^-8 box from 2k
+        const unsigned d = 1u << (20 + k % 5);
+        particle_x_ptr[2 * k + 1] = particle_x_ptr[2 * k] + d;
+        particle_y_ptr[2 * k + 1] = particle_y_ptr[2 * k];
+        particle_z_ptr[2 * k + 1] = particle_z_ptr[2 * k];
+    }
+    ct2_gpu_trace(stdout);
+    int minrung = 0, max_rung = 0;
+    double tau = 0.0;
+    const double t0 = 0.01;
+    for (int step = 0; step < 5; step++) {
+        std::printf("step %d minrung %d max_rung %d\n", step, minrung, max_rung);
+        ct2_gpu_kick_step(minrung, max_rung, 1.0, 1.0, tau, t0, 0, false);
+        tau += t0;
+        ct2_gpu_trace(nullptr);
+        max_rung = (int)particle_rung_counts().size() - 1;
+        ct2_gpu_trace(stdout);
+    }
+    std::printf("end minrung %d max_rung %d\n", minrung, max_rung);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::strcmp(argv[1], "fft") == 0) return fft_mode();
+    if (argc > 1 && std::strcmp(argv[1], "rungs") == 0) return rungs_mode();
     const int n = argc > 1 ? std::atoi(argv[1]) : 512;
     particle_allocate(n);
     ct2_gpu_set_options(0.01f, 1.0f / n);
