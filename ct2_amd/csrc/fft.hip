@@ -20,6 +20,10 @@
 
 #include "internal.h"
 
+#ifndef COL_K512
+#define COL_K512 2
+#endif
+
 namespace ct2 {
 namespace {
 
@@ -111,15 +115,24 @@ __host__ __device__ constexpr int ns_of(int N, int s) {
     return m;
 }
 
-// Lines per workgroup for the row kernel and columns per workgroup for the
-// column kernel, sized for <= 64 KB of LDS and >= 256 lanes where possible.
+// Butterflies per lane and stage of the column kernel: 2 for the FP32 512-point
+// columns, which then take 32 adjacent columns (256-byte segments per row, 128 KB
+// of LDS of gfx950's 160 KB), 1 elsewhere.
 template <typename T, int N>
+constexpr int col_k() {
+    return (sizeof(T) == 4 && N == 512) ? COL_K512 : 1;
+}
+
+// Lines per workgroup for the row kernel and columns per workgroup for the
+// column kernel, sized for <= 64 KB of LDS (128 KB for the 2-butterfly
+// columns) and >= 256 lanes where possible.  K butterflies per lane and stage.
+template <typename T, int N, int K = 1>
 struct Cfg {
     static constexpr int RM = rmax_of(N);
-    static constexpr int P = N / RM;  // lanes per line
+    static constexpr int P = N / (RM * K);  // lanes per line
     static constexpr int ELEM = (int)sizeof(T) * 2;
-    static constexpr int want_cols = (256 / P) > 16 ? (256 / P) : 16;
-    static constexpr int lds_cols = (65536 / (N * ELEM)) > 0 ? (65536 / (N * ELEM)) : 1;
+    static constexpr int want_cols = (256 / P) > 16 * K ? (256 / P) : 16 * K;
+    static constexpr int lds_cols = (65536 * K / (N * ELEM)) > 0 ? (65536 * K / (N * ELEM)) : 1;
     static constexpr int C0 = want_cols < lds_cols ? want_cols : lds_cols;
     static constexpr int COLS = C0 < N ? C0 : N;
     static constexpr int want_rows = (256 / P) > 1 ? (256 / P) : 1;
@@ -129,23 +142,24 @@ struct Cfg {
 
 template <bool ROWS, typename T, int N>
 struct Lanes {
-    static constexpr int value = ROWS ? Cfg<T, N>::P * Cfg<T, N>::ROWS : Cfg<T, N>::P * Cfg<T, N>::COLS;
+    static constexpr int value =
+        ROWS ? Cfg<T, N>::P * Cfg<T, N>::ROWS : Cfg<T, N, col_k<T, N>()>::P * Cfg<T, N, col_k<T, N>()>::COLS;
 };
 
 // Shared kernel body.  Element i of batch line c lives at gmem(i) in HBM
 // and at lds(i) in LDS.
-template <typename T, int N, bool ROWS, int DIR, int S>
+template <typename T, int N, bool ROWS, int DIR, int S, int K = 1>
 __device__ __forceinline__ void run_stages(typename V2<T>::t* __restrict__ data, typename V2<T>::t* sm,
                                            const typename V2<T>::t* tw, long base, long stride_i, int p,
                                            int c) {
     using T2 = typename V2<T>::t;
-    using CF = Cfg<T, N>;
+    using CF = Cfg<T, N, K>;
     constexpr int NS = nstages_of(N);
     if constexpr (S < NS) {
         constexpr int R = radix_of(N, S);
         constexpr int Ns = ns_of(N, S);
         constexpr int L = N / R;
-        constexpr int NB = CF::RM / R;  // butterflies per lane in this stage
+        constexpr int NB = CF::RM * K / R;  // butterflies per lane in this stage
         constexpr bool FIRST = S == 0, LAST = S == NS - 1;
         auto lds = [&](int i) -> int {
             if constexpr (ROWS)
@@ -153,7 +167,7 @@ __device__ __forceinline__ void run_stages(typename V2<T>::t* __restrict__ data,
             else
                 return i * CF::COLS + c;
         };
-        T2 v[CF::RM];
+        T2 v[CF::RM * K];
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             const int b = p + u * CF::P;
@@ -190,7 +204,7 @@ __device__ __forceinline__ void run_stages(typename V2<T>::t* __restrict__ data,
                 for (int r = 0; r < R; ++r) sm[lds(o + r * Ns)] = v[u * R + r];
             }
             __syncthreads();
-            run_stages<T, N, ROWS, DIR, S + 1>(data, sm, tw, base, stride_i, p, c);
+            run_stages<T, N, ROWS, DIR, S + 1, K>(data, sm, tw, base, stride_i, p, c);
         }
     }
 }
@@ -228,7 +242,8 @@ __global__ __launch_bounds__((Lanes<false, T, N>::value)) void fft_cols(typename
                                                                       const typename V2<T>::t* __restrict__ twg,
                                                                       long stride_i, long stride_outer, int inner) {
     using T2 = typename V2<T>::t;
-    using CF = Cfg<T, N>;
+    constexpr int K = col_k<T, N>();
+    using CF = Cfg<T, N, K>;
     __shared__ T2 sm[N * CF::COLS];
     __shared__ T2 tw[N];
     load_twiddles<T, N, DIR>(tw, twg, Lanes<false, T, N>::value);
@@ -237,7 +252,7 @@ __global__ __launch_bounds__((Lanes<false, T, N>::value)) void fft_cols(typename
     const long outer = blockIdx.x / inner, in = blockIdx.x % inner;
     const long base = outer * stride_outer + in * CF::COLS + c;
     if constexpr (nstages_of(N) == 1) __syncthreads();
-    run_stages<T, N, false, DIR, 0>(data, sm, tw, base, stride_i, p, c);
+    run_stages<T, N, false, DIR, 0, K>(data, sm, tw, base, stride_i, p, c);
 }
 
 // ---- twiddle tables (device, per precision and size) ----
@@ -272,7 +287,7 @@ int fft3d_launch(typename V2<T>::t* d, hipStream_t st) {
     // z
     fft_rows<T, N, DIR><<<(unsigned)(n2 / CF::ROWS), Lanes<true, T, N>::value, 0, st>>>(d, tw);
     CT2_LAUNCH_CHECK();
-    const int inner = N / CF::COLS;
+    const int inner = N / Cfg<T, N, col_k<T, N>()>::COLS;
     // y: columns z, slabs x
     fft_cols<T, N, DIR><<<(unsigned)(n * inner), Lanes<false, T, N>::value, 0, st>>>(d, tw, n, n2, inner);
     CT2_LAUNCH_CHECK();
