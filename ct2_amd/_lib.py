@@ -24,6 +24,7 @@ KERNEL_PLUMMER = 0
 KERNEL_CT2 = 1
 KICK_NAIVE = 0x1
 KICK_STORE = 0x2
+KICK_ORDERED = 0x4
 
 
 class Ct2Error(RuntimeError):

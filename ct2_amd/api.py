@@ -19,6 +19,7 @@ KERNEL_PLUMMER = L.KERNEL_PLUMMER
 KERNEL_CT2 = L.KERNEL_CT2
 KICK_NAIVE = L.KICK_NAIVE
 KICK_STORE = L.KICK_STORE
+KICK_ORDERED = L.KICK_ORDERED
 
 
 def _p(a: np.ndarray | None):

@@ -202,6 +202,245 @@ __global__ __launch_bounds__(256) void epilogue_kernel(const float* __restrict__
 
 __global__ void zero_int(int* p) { *p = 0; }
 
+// ---- symmetric all-pairs kick (Newton's third law): each pair evaluated once.
+//
+// Superblocks of SYM_BLOCK = 1024 particles, one per workgroup of 16 waves
+// (64 sinks per wave, positions in registers).  A superblock pair (I, J) is a
+// grid of 64x64 tiles: the wave's 64 sinks against 64 sources of J in LDS.  In
+// a tile, at step t lane l pairs with source (l + t) & 63; the sink-side sums
+// stay in the lane, and the source-side sums (minus the same force, plus the
+// same potential) rotate one lane per step (DPP wave_rol:1) so that each lane
+// always holds the running sums of the source it is paired with.  After the
+// tile, lane l holds the sums of source (l + t_last) & 63.  Per chunk of 64
+// sources the 16 waves' source-side sums are added in FP64 in a fixed order.
+//
+// Schedule (deterministic, race free): a diagonal launch computes each
+// superblock with itself (tile (w, c) for c > w in full, c == w over steps
+// 1..32, the last one on half the lanes, plus the self potential) and
+// initialises plane 0 of the partial sums; then launches over rounds r:
+// workgroup g pairs superblock g with J = (g + r) mod Gs (r = Gs/2 only for
+// g < Gs/2 when Gs is even), keeps its sink-side sums in registers across the
+// launch's rounds (plane 0, one read-modify-write per launch) and adds the
+// source-side sums of round r into plane 1 + (r - r0): within a launch every
+// (plane, superblock) slot has exactly one writer.  kick_finalize then adds the
+// planes in order.
+constexpr int SYM_WAVES = 16;  // chunks of 64 per superblock (and waves of the diagonal kernel)
+constexpr int SYM_BLOCK = 64 * SYM_WAVES;
+#ifndef SYM_NS
+#define SYM_NS 4  // sinks per lane in the off-diagonal rounds
+#endif
+#ifndef SYM_UNROLL
+#define SYM_UNROLL 1  // measured fastest: more unrolling costs occupancy
+#endif
+constexpr int SYM_THREADS = SYM_BLOCK / SYM_NS;
+static_assert(SYM_THREADS >= 256, "the chunk reduction uses 256 threads (4 components x 64 sources)");
+
+__device__ __forceinline__ float rol1(float v) {  // lane l <- lane l+1 (mod 64)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, true));
+}
+
+// Separation (2^-32 units) and kernel factors of one pair: F = d k, pot p.
+template <int KERNEL>
+__device__ __forceinline__ void pair_k(const uint4 s, uint32_t xi, uint32_t yi, uint32_t zi, const KickConst& kc,
+                                       float& dx, float& dy, float& dz, float& k, float& p) {
+    dx = (float)(int32_t)(s.x - xi);
+    dy = (float)(int32_t)(s.y - yi);
+    dz = (float)(int32_t)(s.z - zi);
+    if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, kc.eps2u)));
+        const float w = __builtin_amdgcn_rsqf(r2);
+        k = (w * w) * w;
+        p = w;
+    } else {
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const bool near = r2 < kc.h2u;
+        const float w = __builtin_amdgcn_rsqf(near ? kc.h2u : r2);
+        k = near ? kc.hinv3u : (w * w) * w;
+        p = near ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : w;
+    }
+}
+
+template <int KERNEL>
+__device__ __forceinline__ void pair_terms(const uint4 s, uint32_t xi, uint32_t yi, uint32_t zi, const KickConst& kc,
+                                           float& fx, float& fy, float& fz, float& p) {
+    float dx, dy, dz, k;
+    pair_k<KERNEL>(s, xi, yi, zi, kc, dx, dy, dz, k, p);
+    fx = dx * k;
+    fy = dy * k;
+    fz = dz * k;
+}
+
+// Steps [T0, T1) of a tile; HALF: the last step only on lanes < 32.  The
+// chunk is stored twice in a row in LDS (128 entries), so lane l reads
+// Sl[t] = chunk[(l + t) & 63] at an immediate offset from Sl = chunk + l.
+template <int KERNEL, int T0, int T1, bool HALF>
+__device__ __forceinline__ void sym_tile(const uint4* Sl, int lane, uint32_t xi, uint32_t yi, uint32_t zi,
+                                         const KickConst& kc, float& tx, float& ty, float& tz, float& tp, float& ax,
+                                         float& ay, float& az, float& ap) {
+#pragma unroll 8
+    for (int t = T0; t < T1; ++t) {
+        const uint4 s = Sl[t];
+        float fx, fy, fz, p;
+        pair_terms<KERNEL>(s, xi, yi, zi, kc, fx, fy, fz, p);
+        if (HALF && t == T1 - 1 && lane >= 32) fx = fy = fz = p = 0.0f;
+        tx += fx;
+        ty += fy;
+        tz += fz;
+        tp += p;
+        ax = rol1(ax) - fx;
+        ay = rol1(ay) - fy;
+        az = rol1(az) - fz;
+        ap = rol1(ap) + p;
+    }
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(SYM_BLOCK) void kick_sym_diag(const uint4* __restrict__ P, int n, int ny, KickConst kc,
+                                                           double* __restrict__ part, int* __restrict__ maxrung) {
+    __shared__ uint4 S[SYM_WAVES][128];
+    __shared__ float4 jr[SYM_WAVES][64];
+    __shared__ double jacc[4][SYM_BLOCK];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long i = (long)blockIdx.x * SYM_BLOCK + tid;
+    if (blockIdx.x == 0 && tid == 0) *maxrung = 0;
+    const uint4 me = P[i];
+    S[w][lane] = me;
+    S[w][lane + 64] = me;
+    __syncthreads();
+    float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, cp = 0.f;
+    for (int c = 0; c < SYM_WAVES; ++c) {
+        float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f, ax = 0.f, ay = 0.f, az = 0.f, ap = 0.f;
+        if (w < c) {  // wave-uniform
+            sym_tile<KERNEL, 0, 64, false>(S[c] + lane, lane, me.x, me.y, me.z, kc, tx, ty, tz, tp, ax, ay, az, ap);
+            jr[w][(lane + 63) & 63] = make_float4(ax, ay, az, ap);
+        } else if (w == c) {
+            sym_tile<KERNEL, 1, 33, true>(S[c] + lane, lane, me.x, me.y, me.z, kc, tx, ty, tz, tp, ax, ay, az, ap);
+            jr[w][(lane + 32) & 63] = make_float4(ax, ay, az, ap);
+        }
+        two_sum(sx, cx, tx);
+        two_sum(sy, cy, ty);
+        two_sum(sz, cz, tz);
+        two_sum(sp, cp, tp);
+        __syncthreads();
+        if (tid < 256) {
+            const int comp = tid >> 6, jj = tid & 63;
+            double acc = 0.0;
+            for (int v = 0; v <= c; ++v) acc += (double)((const float*)&jr[v][jj])[comp];
+            jacc[comp][c * 64 + jj] = acc;
+        }
+        __syncthreads();
+    }
+    // the self pair: potential only
+    float fx, fy, fz, pself;
+    pair_terms<KERNEL>(me, me.x, me.y, me.z, kc, fx, fy, fz, pself);
+    two_sum(sp, cp, pself);
+    part[(0L * ny) * n + i] = (double)sx + (double)cx + jacc[0][tid];
+    part[(1L * ny) * n + i] = (double)sy + (double)cy + jacc[1][tid];
+    part[(2L * ny) * n + i] = (double)sz + (double)cz + jacc[2][tid];
+    part[(3L * ny) * n + i] = (double)sp + (double)cp + jacc[3][tid];
+}
+
+// Off-diagonal rounds.  Each lane carries NS sinks (the wave 64*NS of the
+// superblock's 1024), all paired with the same source at every step, so one
+// rotation of the source-side sums serves NS pairs.
+template <int KERNEL>
+__global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __restrict__ P, int n, int ny, int gs,
+                                                              int r0, int r1, KickConst kc,
+                                                              double* __restrict__ part) {
+    constexpr int NWAVE = SYM_THREADS / 64;
+    __shared__ uint4 S[SYM_WAVES][128];
+    __shared__ float4 jr[2][NWAVE][64];  // double-buffered: one barrier per chunk
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = blockIdx.x;
+    uint32_t xi[SYM_NS], yi[SYM_NS], zi[SYM_NS];
+    float sx[SYM_NS], sy[SYM_NS], sz[SYM_NS], sp[SYM_NS], cx[SYM_NS], cy[SYM_NS], cz[SYM_NS], cp[SYM_NS];
+#pragma unroll
+    for (int q = 0; q < SYM_NS; ++q) {
+        const uint4 me = P[(long)g * SYM_BLOCK + (w * SYM_NS + q) * 64 + lane];
+        xi[q] = me.x;
+        yi[q] = me.y;
+        zi[q] = me.z;
+        sx[q] = sy[q] = sz[q] = sp[q] = cx[q] = cy[q] = cz[q] = cp[q] = 0.f;
+    }
+    for (int r = r0; r < r1; ++r) {
+        if ((gs & 1) == 0 && r == gs / 2 && g >= gs / 2) continue;  // that pair belongs to g - gs/2
+        const long J = (g + r) % gs;
+        __syncthreads();
+        for (int e = tid; e < SYM_BLOCK; e += SYM_THREADS) {
+            const uint4 src = P[J * SYM_BLOCK + e];
+            S[e >> 6][e & 63] = src;
+            S[e >> 6][(e & 63) + 64] = src;
+        }
+        __syncthreads();
+        for (int c = 0; c < SYM_WAVES; ++c) {
+            float tx[SYM_NS], ty[SYM_NS], tz[SYM_NS], tp[SYM_NS];
+#pragma unroll
+            for (int q = 0; q < SYM_NS; ++q) tx[q] = ty[q] = tz[q] = tp[q] = 0.f;
+            float ax = 0.f, ay = 0.f, az = 0.f, ap = 0.f;
+            const uint4* Sl = S[c] + lane;
+#pragma unroll SYM_UNROLL
+            for (int t = 0; t < 64; ++t) {
+                const uint4 src = Sl[t];
+                float gx = 0.f, gy = 0.f, gz = 0.f, gp = 0.f;
+#pragma unroll
+                for (int q = 0; q < SYM_NS; ++q) {
+                    float dx, dy, dz, k, p;
+                    pair_k<KERNEL>(src, xi[q], yi[q], zi[q], kc, dx, dy, dz, k, p);
+                    tx[q] = fmaf(dx, k, tx[q]);
+                    ty[q] = fmaf(dy, k, ty[q]);
+                    tz[q] = fmaf(dz, k, tz[q]);
+                    tp[q] += p;
+                    gx = fmaf(dx, k, gx);
+                    gy = fmaf(dy, k, gy);
+                    gz = fmaf(dz, k, gz);
+                    gp += p;
+                }
+                ax = rol1(ax) - gx;
+                ay = rol1(ay) - gy;
+                az = rol1(az) - gz;
+                ap = rol1(ap) + gp;
+            }
+#pragma unroll
+            for (int q = 0; q < SYM_NS; ++q) {
+                two_sum(sx[q], cx[q], tx[q]);
+                two_sum(sy[q], cy[q], ty[q]);
+                two_sum(sz[q], cz[q], tz[q]);
+                two_sum(sp[q], cp[q], tp[q]);
+            }
+            jr[c & 1][w][(lane + 63) & 63] = make_float4(ax, ay, az, ap);
+            __syncthreads();
+            if (tid < 256) {
+                const int comp = tid >> 6, jj = tid & 63;
+                double acc = 0.0;
+#pragma unroll
+                for (int v = 0; v < NWAVE; ++v) acc += (double)((const float*)&jr[c & 1][v][jj])[comp];
+                // the only update of this slot in this launch: a no-return FP64
+                // atomic keeps the wave from waiting on the read-modify-write
+                unsafeAtomicAdd(part + ((long)comp * ny + 1 + (r - r0)) * n + J * SYM_BLOCK + c * 64 + jj, acc);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SYM_NS; ++q) {
+        const long i = (long)g * SYM_BLOCK + (w * SYM_NS + q) * 64 + lane;
+        part[(0L * ny) * n + i] += (double)sx[q] + (double)cx[q];
+        part[(1L * ny) * n + i] += (double)sy[q] + (double)cy[q];
+        part[(2L * ny) * n + i] += (double)sz[q] + (double)cz[q];
+        part[(3L * ny) * n + i] += (double)sp[q] + (double)cp[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void pack_positions(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                      const uint32_t* __restrict__ Z, int n, uint4* __restrict__ P) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) P[i] = make_uint4(X[i], Y[i], Z[i], 0u);
+}
+
+__global__ __launch_bounds__(256) void fill_f64(double* __restrict__ p, long n) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
 // FP64 judge: block s handles sink idx[s]; 256 lanes stride the sources.
 template <int KERNEL>
 __global__ __launch_bounds__(256) void kick_sample_fp64(const uint32_t* __restrict__ X,
@@ -261,6 +500,8 @@ __global__ __launch_bounds__(256) void kick_sample_fp64(const uint32_t* __restri
     }
 }
 
+uint4* g_packed = nullptr;
+size_t g_packed_bytes = 0;
 double* g_sample_out = nullptr;
 size_t g_sample_bytes = 0;
 long* g_sample_idx = nullptr;
@@ -275,6 +516,9 @@ void release_kick_buffers() {
     c.partial_bytes = 0;
     dev_free(c.maxrung);
     c.maxrung = nullptr;
+    dev_free(g_packed);
+    g_packed = nullptr;
+    g_packed_bytes = 0;
     dev_free(g_sample_out);
     g_sample_out = nullptr;
     g_sample_bytes = 0;
@@ -320,7 +564,7 @@ using namespace ct2;
 extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung) {
     CT2_TRY(require_particles());
     CT2_TRY(check_kick_params(p));
-    if (flags & ~(unsigned)(CT2_KICK_NAIVE | CT2_KICK_STORE))
+    if (flags & ~(unsigned)(CT2_KICK_NAIVE | CT2_KICK_STORE | CT2_KICK_ORDERED))
         return set_error(CT2_ERR_INVALID, "unknown kick flags 0x%x", flags);
     Context& c = ctx();
     const int n = (int)c.n;
@@ -339,20 +583,50 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
     kc.hinvu = (float)(1.0 / (h * two32));
     kc.hinv3u = (float)(1.0 / ((h * two32) * (h * two32) * (h * two32)));
 
+    const bool naive = flags & CT2_KICK_NAIVE;
+    const bool plummer = c.opts.kernel == CT2_KERNEL_PLUMMER;
+    int nsplit_eff;
+    if (i0 == 0 && n % SYM_BLOCK == 0 && !naive && !(flags & CT2_KICK_ORDERED)) {
+        // each unordered pair once: a diagonal launch, then launches of R rounds
+        const int gs = n / SYM_BLOCK, rmax = gs / 2;
+        const size_t plane = sizeof(double) * 4 * (size_t)n;
+        const int rcap = (int)std::max<size_t>(1, ((size_t)4 << 30) / plane - 1);
+        const int R = std::max(1, std::min({16, rmax, rcap}));
+        nsplit_eff = R + 1;
+        CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, plane * nsplit_eff, "kick partial sums"));
+        CT2_TRY(ensure_scratch((void**)&g_packed, &g_packed_bytes, sizeof(uint4) * (size_t)n, "packed positions"));
+        pack_positions<<<(n + 255) / 256, 256, 0, c.stream>>>(c.x, c.y, c.z, n, g_packed);
+        CT2_LAUNCH_CHECK();
+        const long total = 4L * nsplit_eff * n;
+        fill_f64<<<(unsigned)((total + 255) / 256), 256, 0, c.stream>>>(c.partial, total);
+        CT2_LAUNCH_CHECK();
+        if (plummer)
+            kick_sym_diag<CT2_KERNEL_PLUMMER><<<gs, SYM_BLOCK, 0, c.stream>>>(g_packed, n, nsplit_eff, kc, c.partial, c.maxrung);
+        else
+            kick_sym_diag<CT2_KERNEL_CT2><<<gs, SYM_BLOCK, 0, c.stream>>>(g_packed, n, nsplit_eff, kc, c.partial, c.maxrung);
+        CT2_LAUNCH_CHECK();
+        for (int r0 = 1; r0 <= rmax; r0 += R) {
+            const int r1 = std::min(r0 + R, rmax + 1);
+            if (plummer)
+                kick_sym_pairs<CT2_KERNEL_PLUMMER><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, n, nsplit_eff, gs, r0, r1, kc, c.partial);
+            else
+                kick_sym_pairs<CT2_KERNEL_CT2><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, n, nsplit_eff, gs, r0, r1, kc, c.partial);
+            CT2_LAUNCH_CHECK();
+        }
+    } else {
     // sink blocks and source chunks: enough workgroups to fill 256 CUs
     const int nb = (ns + KICK_BLOCK - 1) / KICK_BLOCK;
     const int max_split = std::max(1, n / (4 * KICK_TILE));
     const int nsplit = std::min(max_split, std::max(1, (4096 + nb - 1) / nb));
     int jchunk = (n + nsplit - 1) / nsplit;
     jchunk = (jchunk + KICK_TILE - 1) / KICK_TILE * KICK_TILE;
-    const int nsplit_eff = (n + jchunk - 1) / jchunk;
+    nsplit_eff = (n + jchunk - 1) / jchunk;
 
     CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, sizeof(double) * 4 * (size_t)nsplit_eff * ns,
                            "kick partial sums"));
 
     const dim3 grid(nb, nsplit_eff);
-    const bool naive = flags & CT2_KICK_NAIVE;
-    if (c.opts.kernel == CT2_KERNEL_PLUMMER) {
+    if (plummer) {
         if (naive)
             kick_direct<CT2_KERNEL_PLUMMER, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
         else
@@ -364,6 +638,7 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
             kick_direct<CT2_KERNEL_CT2, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
     }
     CT2_LAUNCH_CHECK();
+    }
 
     const EpiConst e = make_epilogue(p, c.opts.hsoft);
     // force in box units: sum(dxu * w3u) * 2^64 * GM; potential: sum(wu) * 2^32 * GM
