@@ -50,6 +50,13 @@ __device__ __forceinline__ float gelu(float z) {
     const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
     return 0.5f * z * (1.0f + t);
 }
+// The same GELU without the library tanh (branch-free, for the fused
+// inference kernel): tanh(u) = 1 - 2 / (1 + e^{2u}); within ~1e-6 of gelu.
+__device__ __forceinline__ float gelu_fast(float z) {
+    const float u = GELU_C * (z + 0.044715f * z * z * z);
+    const float t = 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * u));
+    return 0.5f * z * (1.0f + t);
+}
 __device__ __forceinline__ float gelu_d(float z) {
     const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
     return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * GELU_C * (1.0f + 3.0f * 0.044715f * z * z);
@@ -311,11 +318,125 @@ __global__ void apply_kernel(const float* __restrict__ Yo, long r0, int m, float
     fz[p] += unit * Yo[(size_t)r * KOUT + 2];
 }
 
+// ------------------------------------------------------------------ fused inference
+// The three layers of one tile of particles in one kernel, activations kept
+// in registers.  Products are taken transposed, Z^T = W^T X^T: the weights are
+// the MFMA A operand (rows = output units), the particles the B operand
+// (columns), so that a lane's 16x16x32 output -- units 4 (lane/16) + j of
+// particle lane%16 -- can be fed to the next layer's B operand without moving:
+// of a 32-wide k-step, element i of lane group g is unit 16 (i/4) + 4 g + i%4
+// of the step, and the next layer's weights are stored with their k columns
+// in that order (kperm).  Eight waves per workgroup, 32 particles (two column
+// blocks) per wave; W2 (128 KB) sits in LDS for the kernel's life, W1 and W3
+// come from L2; persistent workgroups walk the tiles.
+constexpr int INF_WAVES = 8, INF_PW = 16, INF_TILE = INF_WAVES * INF_PW;
+
+__host__ __device__ constexpr int kperm(int q) { return 16 * ((q & 7) >> 2) + 4 * (q >> 3) + (q & 3); }
+
+// W2p[n][32 s + q] = W2[32 s + kperm(q)][n];  W3p[n][32 s + q] = W3[32 s + kperm(q)][n], n < 16
+__global__ void pack_infer(const float* __restrict__ w, bf16_t* __restrict__ W2p, bf16_t* __restrict__ W3p) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < HID * HID) {
+        const int n = t / HID, c = t % HID;
+        const int k = (c & ~31) + kperm(c & 31);
+        W2p[t] = f2bf(w[OFF_W2 + (size_t)k * HID + n]);
+    }
+    if (t < 16 * HID) {
+        const int n = t / HID, c = t % HID;
+        const int k = (c & ~31) + kperm(c & 31);
+        W3p[t] = f2bf(w[OFF_W3 + (size_t)k * KOUT + n]);
+    }
+}
+
+// bias + GELU of a 256-unit column block, packed as the next layer's B operand
+// (bias points at the lane's first unit, 4 * (lane / 16))
+__device__ __forceinline__ void bias_gelu_pack(const floatx4 (&acc)[16], const float* __restrict__ bias,
+                                               bf16x8 (&h)[8]) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float4 b = *(const float4*)(bias + 16 * r);
+        h[r >> 1][4 * (r & 1) + 0] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][0] + b.x)));
+        h[r >> 1][4 * (r & 1) + 1] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][1] + b.y)));
+        h[r >> 1][4 * (r & 1) + 2] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][2] + b.z)));
+        h[r >> 1][4 * (r & 1) + 3] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][3] + b.w)));
+    }
+}
+
+__global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const bf16_t* __restrict__ X, int ntiles,
+                                                           const bf16_t* __restrict__ W1t,
+                                                           const bf16_t* __restrict__ W2p,
+                                                           const bf16_t* __restrict__ W3p, const float* __restrict__ w,
+                                                           long r0, int m, float unit, float* __restrict__ fx,
+                                                           float* __restrict__ fy, float* __restrict__ fz) {
+    __shared__ uint4 sW2[HID * HID / 8];  // 128 KB
+    for (int e = threadIdx.x; e < HID * HID / 8; e += INF_WAVES * 64) sW2[e] = ((const uint4*)W2p)[e];
+    __syncthreads();
+    const bf16_t* sw2 = (const bf16_t*)sW2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
+    const float* b1 = w + OFF_B1;
+    const float* b2 = w + OFF_B2;
+    const float* b3 = w + OFF_B3;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        // the weight addresses are rebuilt per tile from an opaque zero: hoisted
+        // out of the loop they would take ~150 registers and spill
+        int zero;
+        asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+        const bf16_t* w1 = W1t + zero + l16 * KIN + 8 * g;
+        const bf16_t* w3 = W3p + zero + l16 * HID + 8 * g;
+        const float* bb1 = b1 + zero + 4 * g;
+        const float* bb2 = b2 + zero + 4 * g;
+        const int p = tile * INF_TILE + wave * INF_PW + l16;  // this lane's particle (column)
+        floatx4 acc[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = floatx4{0.f, 0.f, 0.f, 0.f};
+        // layer 1: K = 128
+#pragma unroll
+        for (int ks = 0; ks < KIN / 32; ++ks) {
+            const bf16x8 x = load8(X + (size_t)p * KIN + 32 * ks + 8 * g);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const bf16x8 a = load8(w1 + 16 * r * KIN + 32 * ks);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x, acc[r], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight: no spills
+        }
+        bf16x8 h[8];
+        bias_gelu_pack(acc, bb1, h);
+        // layer 2: K = 256, weights from LDS
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < HID / 32; ++ks) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const bf16x8 a = load8(sw2 + (16 * r + l16) * HID + 32 * ks + 8 * g);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], acc[r], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        bias_gelu_pack(acc, bb2, h);
+        // layer 3: outputs 0..15 (3 used)
+        floatx4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < HID / 32; ++ks) {
+            const bf16x8 a = load8(w3 + 32 * ks);
+            y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], y, 0, 0, 0);
+        }
+        // lanes of group 0 hold outputs 0..3 of their particle
+        if (g == 0 && p < m) {
+            fx[r0 + p] += unit * (y[0] + b3[0]);
+            fy[r0 + p] += unit * (y[1] + b3[1]);
+            fz[r0 + p] += unit * (y[2] + b3[2]);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ state
 struct Mlp {
     bool ready = false;
     float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     bf16_t *W1t = nullptr, *W2t = nullptr, *W3t = nullptr, *W2n = nullptr, *W3n = nullptr;
+    bf16_t *W2p = nullptr, *W3p = nullptr;  // k-permuted copies for the fused inference kernel
     long adam_t = 0;
     // work buffers for up to cap rows
     int cap = 0;
@@ -349,7 +470,7 @@ void free_work() {
 
 // Work buffers for `rows` rows (rounded up to 128); training buffers only when asked.
 int ensure_work(int rows, bool training) {
-    rows = (rows + 127) / 128 * 128;
+    rows = (rows + 255) / 256 * 256;
     const bool have_train = g_mlp.Z1 != nullptr;
     if (g_mlp.cap >= rows && (!training || have_train)) return CT2_OK;
     const int cap = std::max(rows, g_mlp.cap);
@@ -395,6 +516,8 @@ int require_mlp() {
 
 int repack(hipStream_t st) {
     pack_weights<<<(HID * HID + 255) / 256, 256, 0, st>>>(g_mlp.w, g_mlp.W1t, g_mlp.W2t, g_mlp.W3t, g_mlp.W2n, g_mlp.W3n);
+    CT2_LAUNCH_CHECK();
+    pack_infer<<<(HID * HID + 255) / 256, 256, 0, st>>>(g_mlp.w, g_mlp.W2p, g_mlp.W3p);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
@@ -473,12 +596,13 @@ int mlp_correct_forces() {
     const float unit = (float)force_unit();
     for (long r0 = 0; r0 < n; r0 += chunk) {
         const int m = (int)std::min<long>(chunk, n - r0);
-        const int M = (m + 127) / 128 * 128;
+        const int M = (m + INF_TILE - 1) / INF_TILE * INF_TILE;
         features_kernel<<<(m + 255) / 256, 256, 0, st>>>(c.x, c.y, c.z, nullptr, r0, m, fg, g_mlp.X, nullptr);
         CT2_LAUNCH_CHECK();
         if (M > m) CT2_HIP(hipMemsetAsync(g_mlp.X + (size_t)m * KIN, 0, (size_t)(M - m) * KIN * 2, st));
-        CT2_TRY(forward(st, M, false));
-        apply_kernel<<<(m + 255) / 256, 256, 0, st>>>(g_mlp.Yo, r0, m, unit, c.fx, c.fy, c.fz);
+        const int ntiles = M / INF_TILE;
+        mlp_infer<<<std::min(ntiles, 256), INF_WAVES * 64, 0, st>>>(g_mlp.X, ntiles, g_mlp.W1t, g_mlp.W2p, g_mlp.W3p,
+                                                                    g_mlp.w, r0, m, unit, c.fx, c.fy, c.fz);
         CT2_LAUNCH_CHECK();
     }
     return CT2_OK;
@@ -487,8 +611,8 @@ int mlp_correct_forces() {
 void release_mlp() {
     free_work();
     for (void* p : {(void*)g_mlp.w, (void*)g_mlp.m, (void*)g_mlp.v, (void*)g_mlp.grad, (void*)g_mlp.W1t,
-                    (void*)g_mlp.W2t, (void*)g_mlp.W3t, (void*)g_mlp.W2n, (void*)g_mlp.W3n, (void*)g_mlp.DX,
-                    (void*)g_mlp.DT})
+                    (void*)g_mlp.W2t, (void*)g_mlp.W3t, (void*)g_mlp.W2n, (void*)g_mlp.W3n, (void*)g_mlp.W2p,
+                    (void*)g_mlp.W3p, (void*)g_mlp.DX, (void*)g_mlp.DT})
         dev_free(p);
     g_mlp = Mlp{};
 }
@@ -510,7 +634,9 @@ extern "C" int ct2_mlp_init(uint64_t seed) {
         (rc = dev_alloc((void**)&g_mlp.W2t, HID * HID * 2, "W2t")) ||
         (rc = dev_alloc((void**)&g_mlp.W3t, HID * KOUT * 2, "W3t")) ||
         (rc = dev_alloc((void**)&g_mlp.W2n, HID * HID * 2, "W2n")) ||
-        (rc = dev_alloc((void**)&g_mlp.W3n, HID * KOUT * 2, "W3n"))) {
+        (rc = dev_alloc((void**)&g_mlp.W3n, HID * KOUT * 2, "W3n")) ||
+        (rc = dev_alloc((void**)&g_mlp.W2p, HID * HID * 2, "W2p")) ||
+        (rc = dev_alloc((void**)&g_mlp.W3p, 16 * HID * 2, "W3p"))) {
         release_mlp();
         return rc;
     }

@@ -124,3 +124,42 @@ def test_mlp_training_reduces_pm_error(ct2):
     print(f"held-out median rel err: PM {np.median(e_pm):.3e}  PM+MLP {np.median(e_ml):.3e}; "
           f"p90 {np.percentile(e_pm, 90):.3e} -> {np.percentile(e_ml, 90):.3e}", flush=True)
     assert np.median(e_ml) < 0.8 * np.median(e_pm)
+
+
+def test_fused_inference_matches_gemm_forward(ct2, orc):
+    """ct2_pm_mlp_forces runs the three layers in one kernel (transposed
+    products, activations kept in registers, k-permuted weights).  Its
+    correction must equal F_unit times the GEMM-path forward of the same
+    features and the oracle's bf16 forward, to bf16 rounding; ragged n
+    exercises the padded tail."""
+    x, y, z = ct2.ic_plummer_mix(20011, 10000, 4, 0.02, 0.2, 12)
+    n = x.size
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z)
+    gm = float(np.float32(1.0 / n))
+    ct2.set_force_options(1e-3, gm, 0)
+    ng = 32
+    ct2.pm_init(ng, 1.25)
+    ct2.mlp_init(5)
+    w = ct2.mlp_get_weights()
+    rng = np.random.default_rng(4)
+    kin, hid, kout = 128, 256, 32
+    b1 = kin * hid
+    b2 = b1 + hid + hid * hid
+    b3 = b2 + hid + hid * kout
+    w[b1:b1 + hid] = rng.standard_normal(hid) * 0.3  # non-zero biases: exercise their indexing
+    w[b2:b2 + hid] = rng.standard_normal(hid) * 0.3
+    w[b3:b3 + 3] = rng.standard_normal(3) * 0.1
+    ct2.mlp_set_weights(w)
+    pm = np.stack(ct2.pm_forces()[:3]).astype(np.float64)
+    fused = np.stack(ct2.pm_mlp_forces()[:3]).astype(np.float64)
+    idx = np.concatenate([np.arange(0, n, 7), np.arange(n - 20, n)])
+    X = ct2.mlp_features(idx)
+    funit = gm * ng * ng
+    got = ((fused - pm)[:, idx] / funit).T
+    gemm = ct2.mlp_forward(X).astype(np.float64)
+    want = orc.mlp_forward(w.astype(np.float64), X, bf16=True)[:, :3]
+    scale = np.sqrt(np.mean(want ** 2))
+    for ref in (gemm, want):
+        err = np.abs(got - ref)
+        assert np.median(err) < 2e-3 * scale and np.max(err) < 3e-2 * scale, (np.median(err) / scale, err.max() / scale)
