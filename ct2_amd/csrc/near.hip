@@ -36,7 +36,12 @@
 namespace ct2 {
 namespace {
 
-constexpr int NEAR_BLOCK = 256;  // sinks per work item == sources per LDS tile
+#ifndef NEAR_BLOCK_SIZE
+#define NEAR_BLOCK_SIZE 64
+#endif
+// sinks per work item == sources per LDS tile: one wave, so that a work item's
+// sinks span little of their row and the wave skips distant sources
+constexpr int NEAR_BLOCK = NEAR_BLOCK_SIZE;
 constexpr int NEAR_MAX_NC = 64;  // cells per side at most
 constexpr int SCAN_THREADS = 1024;
 
@@ -52,6 +57,8 @@ struct NearState {
     int* count = nullptr;          // [nc^3]
     int* start = nullptr;          // [nc^3 + 1]
     int* row_chunk = nullptr;      // [nc^2 + 1]
+    int2* item = nullptr;          // [cap / NEAR_BLOCK + nc^2] (row, chunk) per work item
+    size_t item_cap = 0;
 } g_near;
 
 struct NearConst {
@@ -218,82 +225,65 @@ __device__ __forceinline__ void near_pair(float r2, const NearConst& q, float& k
     p = ps - pl;
 }
 
-// Work item b: row r with row_chunk[r] <= b < row_chunk[r+1], chunk b - row_chunk[r].
+// Work items, one per NEAR_BLOCK sinks of a row: item[row_chunk[r] + k] = (r, k).
+__global__ __launch_bounds__(256) void near_items(const int* __restrict__ row_chunk, int nrow, int2* __restrict__ item) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nrow) return;
+    for (int k = row_chunk[r], c = 0; k < row_chunk[r + 1]; ++k, ++c) item[k] = make_int2(r, c);
+}
+
 template <int KERNEL>
 __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict__ P, const int* __restrict__ start,
-                                                         const int* __restrict__ row_chunk, int nc, NearConst q,
+                                                         const int* __restrict__ row_chunk,
+                                                         const int2* __restrict__ item, int nc, NearConst q,
                                                          double fscale, double pscale, double pconst,
                                                          float* __restrict__ fx, float* __restrict__ fy,
                                                          float* __restrict__ fz, float* __restrict__ pot) {
     __shared__ uint4 tile[NEAR_BLOCK];
     __shared__ int rng[18][2];
-    __shared__ int nrng, srow, s0s, s1s;
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
-    const int nrow = nc * nc;
-    if (tid == 0) {
-        int lo = 0, hi = nrow;  // find the last row with row_chunk[row] <= b
-        if (b >= row_chunk[nrow]) {
-            srow = -1;
-        } else {
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (row_chunk[mid] <= b) lo = mid;
-                else hi = mid;
-            }
-            srow = lo;
-            const int r0 = start[lo * nc], r1 = start[(lo + 1) * nc];
-            s0s = r0 + (b - row_chunk[lo]) * NEAR_BLOCK;
-            s1s = min(r1, s0s + NEAR_BLOCK);
-        }
-    }
-    __syncthreads();
-    const int row = srow;
-    if (row < 0) return;  // uniform across the workgroup
-    const int s0 = s0s, s1 = s1s;
+    if (b >= row_chunk[nc * nc]) return;  // uniform across the workgroup
+    const int2 it = item[b];
+    const int row = it.x;
+    const int s0 = start[row * nc] + it.y * NEAR_BLOCK;
+    const int s1 = min(start[(row + 1) * nc], s0 + NEAR_BLOCK);
     const int cx = row / nc, cy = row % nc;
-    if (tid == 0) {
+    if (tid < 9) {  // one neighbouring row per lane: at most two index ranges (z wraps)
         const int kz0 = (int)cell_of(P[s0].z, (uint32_t)nc), kz1 = (int)cell_of(P[s1 - 1].z, (uint32_t)nc);
-        int k = 0;
-        for (int di = -1; di <= 1; ++di)
-            for (int dj = -1; dj <= 1; ++dj) {
-                const int rr = ((cx + di + nc) % nc) * nc + (cy + dj + nc) % nc;
-                const int c0 = rr * nc;
-                if (kz1 - kz0 + 3 >= nc) {
-                    rng[k][0] = start[c0];
-                    rng[k][1] = start[c0 + nc];
-                    ++k;
-                } else {
-                    const int lo = kz0 - 1, hi = kz1 + 1;  // inclusive cell range along z
-                    if (lo < 0) {
-                        rng[k][0] = start[c0 + lo + nc];
-                        rng[k][1] = start[c0 + nc];
-                        ++k;
-                        rng[k][0] = start[c0];
-                        rng[k][1] = start[c0 + hi + 1];
-                        ++k;
-                    } else if (hi >= nc) {
-                        rng[k][0] = start[c0 + lo];
-                        rng[k][1] = start[c0 + nc];
-                        ++k;
-                        rng[k][0] = start[c0];
-                        rng[k][1] = start[c0 + hi - nc + 1];
-                        ++k;
-                    } else {
-                        rng[k][0] = start[c0 + lo];
-                        rng[k][1] = start[c0 + hi + 1];
-                        ++k;
-                    }
-                }
+        const int di = tid / 3 - 1, dj = tid % 3 - 1;
+        const int c0 = (((cx + di + nc) % nc) * nc + (cy + dj + nc) % nc) * nc;
+        int a0, e0, a1 = 0, e1 = 0;
+        if (kz1 - kz0 + 3 >= nc) {  // the run covers the whole row
+            a0 = start[c0];
+            e0 = start[c0 + nc];
+        } else {
+            const int lo = kz0 - 1, hi = kz1 + 1;  // inclusive cell range along z
+            if (lo < 0) {
+                a0 = start[c0 + lo + nc];
+                e0 = start[c0 + nc];
+                a1 = start[c0];
+                e1 = start[c0 + hi + 1];
+            } else if (hi >= nc) {
+                a0 = start[c0 + lo];
+                e0 = start[c0 + nc];
+                a1 = start[c0];
+                e1 = start[c0 + hi - nc + 1];
+            } else {
+                a0 = start[c0 + lo];
+                e0 = start[c0 + hi + 1];
             }
-        nrng = k;
+        }
+        rng[2 * tid][0] = a0;
+        rng[2 * tid][1] = e0;
+        rng[2 * tid + 1][0] = a1;
+        rng[2 * tid + 1][1] = e1;
     }
     __syncthreads();
     const int i = s0 + tid;
     const uint4 me = P[i < s1 ? i : s1 - 1];
     float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f, cx_ = 0.f, cy_ = 0.f, cz_ = 0.f, cp_ = 0.f;
-    const int nr = nrng;
-    for (int r = 0; r < nr; ++r) {
+    for (int r = 0; r < 18; ++r) {
         const int a = rng[r][0], e = rng[r][1];
         for (int j0 = a; j0 < e; j0 += NEAR_BLOCK) {
             const int cnt = min(NEAR_BLOCK, e - j0);
@@ -344,9 +334,12 @@ void free_near_buffers() {
     dev_free(g_near.count);
     dev_free(g_near.start);
     dev_free(g_near.row_chunk);
+    dev_free(g_near.item);
     g_near.cell = g_near.slot = nullptr;
     g_near.P0 = g_near.P = nullptr;
     g_near.count = g_near.start = g_near.row_chunk = nullptr;
+    g_near.item = nullptr;
+    g_near.item_cap = 0;
     g_near.cap = 0;
     g_near.ncell_cap = 0;
 }
@@ -377,6 +370,14 @@ int ensure_buffers(size_t n, int nc) {
         CT2_TRY(dev_alloc((void**)&g_near.start, (c3 + 1) * sizeof(int), "near-field cell starts"));
         CT2_TRY(dev_alloc((void**)&g_near.row_chunk, ((size_t)nc * nc + 1) * sizeof(int), "near-field work items"));
         g_near.ncell_cap = nc;
+    }
+    const size_t items = n / NEAR_BLOCK + 1 + (size_t)nc * nc;
+    if (items > g_near.item_cap) {
+        dev_free(g_near.item);
+        g_near.item = nullptr;
+        g_near.item_cap = 0;
+        CT2_TRY(dev_alloc((void**)&g_near.item, items * sizeof(int2), "near-field work items"));
+        g_near.item_cap = items;
     }
     return CT2_OK;
 }
@@ -427,14 +428,18 @@ int near_forces(double rs) {
     const double gm = c.opts.GM;
     const double fscale = gm * 0x1p64, pscale = gm * 0x1p32;
     const double pconst = gm * (double)n * 4.0 * M_PI * rs * rs;
+    near_items<<<(nrow + 255) / 256, 256, 0, st>>>(g_near.row_chunk, nrow, g_near.item);
+    CT2_LAUNCH_CHECK();
     // work items: at most one partial chunk per row beyond n / NEAR_BLOCK full ones
     const unsigned items = (unsigned)((n + NEAR_BLOCK - 1) / NEAR_BLOCK + nrow);
     if (c.opts.kernel == CT2_KERNEL_PLUMMER)
-        near_force<CT2_KERNEL_PLUMMER><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk, nc, q,
-                                                                     fscale, pscale, pconst, c.fx, c.fy, c.fz, c.pot);
+        near_force<CT2_KERNEL_PLUMMER><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk,
+                                                                     g_near.item, nc, q, fscale, pscale, pconst, c.fx,
+                                                                     c.fy, c.fz, c.pot);
     else
-        near_force<CT2_KERNEL_CT2><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk, nc, q,
-                                                                 fscale, pscale, pconst, c.fx, c.fy, c.fz, c.pot);
+        near_force<CT2_KERNEL_CT2><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk, g_near.item,
+                                                                 nc, q, fscale, pscale, pconst, c.fx, c.fy, c.fz,
+                                                                 c.pot);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
