@@ -49,6 +49,15 @@ class Cosmology(C.Structure):
     _fields_ = [("H", C.c_double), ("omega_m", C.c_double), ("omega_r", C.c_double), ("omega_lam", C.c_double)]
 
 
+class Options(C.Structure):
+    """ct2_options: the options block of options.hpp:14-32."""
+
+    _fields_ = [("bwidth", C.c_float), ("code_to_cm", C.c_double), ("code_to_g", C.c_double),
+                ("code_to_s", C.c_double)] + \
+        [(k, C.c_float) for k in ("eta", "GM", "hsoft", "hubble", "omega_lam", "omega_m", "omega_r", "mass_res")] + \
+        [("Ngrid", C.c_uint64), ("nparts", C.c_uint64), ("ppcell", C.c_int), ("Theta", C.c_float)]
+
+
 class ParticlePtrs(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("vx", C.c_void_p),
                 ("vy", C.c_void_p), ("vz", C.c_void_p), ("rung", C.c_void_p), ("n", C.c_size_t)]
@@ -121,6 +130,9 @@ _SIGS = {
     "ct2_cosmos_time": ([C.POINTER(Cosmology), C.c_double, C.c_double], C.c_double),
     "ct2_cosmos_kick_factor": ([C.POINTER(Cosmology), C.c_double, C.c_double], C.c_double),
     "ct2_cosmos_drift_factor": ([C.POINTER(Cosmology), C.c_double, C.c_double], C.c_double),
+    "ct2_options_defaults": ([C.POINTER(Options)], C.c_int),
+    "ct2_options_derive": ([C.POINTER(Options)], C.c_int),
+    "ct2_options_cosmology": ([C.POINTER(Options), C.POINTER(Cosmology)], C.c_int),
     "ct2_ic_plummer_mix": ([C.c_size_t, C.c_size_t, C.c_int, C.c_double, C.c_double, C.c_uint64,
                             _P, _P, _P], C.c_int),
 }

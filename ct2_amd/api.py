@@ -469,3 +469,28 @@ def cosmos_kick_factor(c, a0: float, a1: float) -> float:
 
 def cosmos_drift_factor(c, a0: float, a1: float) -> float:
     return L.load().ct2_cosmos_drift_factor(C.byref(c), a0, a1)
+
+
+# ---------------------------------------------------------------- options, code units
+Options = L.Options
+
+
+def options(**kw) -> "L.Options":
+    """The options block with the reference's defaults (options.cpp:51-61),
+    overridden by ``kw``, and its derived fields (options.cpp:88-95) filled."""
+    o = L.Options()
+    L.check(L.load().ct2_options_defaults(C.byref(o)))
+    for k, v in kw.items():
+        if k not in {f[0] for f in L.Options._fields_}:
+            raise KeyError(f"unknown option {k!r}")
+        setattr(o, k, v)
+    L.check(L.load().ct2_options_derive(C.byref(o)))
+    return o
+
+
+def options_cosmology(o) -> "L.Cosmology":
+    """The cosmology block of an options block: H = H0 code_to_s hubble in code
+    time units (cosmology.cpp:15)."""
+    c = L.Cosmology()
+    L.check(L.load().ct2_options_cosmology(C.byref(o), C.byref(c)))
+    return c

@@ -111,3 +111,65 @@ extern "C" double ct2_cosmos_drift_factor(const ct2_cosmology* c, double a0, dou
 extern "C" double ct2_cosmos_kick_factor(const ct2_cosmology* c, double a0, double a1) {
     return integrate_lna(c, a0, a1, [](double a) { return 1.0 / a; }, 1e-12);
 }
+
+// ------------------------------------------------------------ options, units
+// options.cpp:51-61 (defaults) and :88-95 (derived fields).  Each derived
+// field is formed in FP64; the dimensionless ones are rounded to FP32 as the
+// reference's float members are, the code units stay FP64 (in FP32 the
+// mass unit overflows at the default mass_res).
+namespace {
+// cgs constants of constants.hpp:14-39
+constexpr double kC = 2.99792458e10;      // speed of light, cm/s
+constexpr double kG = 6.67259e-8;         // gravitational constant
+constexpr double kSigma = 5.67051e-5;     // Stefan-Boltzmann
+constexpr double kH0 = 1e7 / 3.086e24;    // 100 km/s/Mpc in 1/s
+constexpr double kMsun = 1.989e33;        // solar mass, g
+constexpr double kTcmb = 2.73;            // K
+constexpr double kNeff = 3.086;           // effective neutrino species
+constexpr double kPi = 3.14159265358979323846;
+}  // namespace
+
+extern "C" int ct2_options_defaults(ct2_options* o) {
+    if (!o) return ct2::set_error(CT2_ERR_INVALID, "ct2_options_defaults: null options");
+    *o = ct2_options{};
+    o->hsoft = (float)(1.0 / 50.0);
+    o->bwidth = 3.0f;
+    o->omega_m = 0.25f;
+    o->mass_res = 1e9f;
+    o->hubble = 0.7f;
+    o->eta = 0.15f;
+    o->Theta = 1.0f;
+    o->Ngrid = 32;
+    o->ppcell = 256;
+    return CT2_OK;
+}
+
+extern "C" int ct2_options_derive(ct2_options* o) {
+    if (!o) return ct2::set_error(CT2_ERR_INVALID, "ct2_options_derive: null options");
+    if (o->Ngrid == 0 || !(o->mass_res > 0.f) || !(o->hubble > 0.f) || !(o->omega_m > 0.f))
+        return ct2::set_error(CT2_ERR_INVALID, "ct2_options_derive: Ngrid, mass_res, hubble and omega_m must be positive");
+    const uint64_t n = o->Ngrid * o->Ngrid * o->Ngrid;
+    o->GM = (float)(1.0 / (double)n);
+    o->nparts = n;
+    o->code_to_g = kMsun * (double)o->mass_res;
+    // mean density nparts * code_to_g / code_to_cm^3 = omega_m * 3 H^2 / (8 pi G)
+    const double h = kH0 * (double)o->hubble;
+    o->code_to_cm = std::cbrt(o->code_to_g * (8.0 * kPi) * (double)n * kG / (3.0 * (double)o->omega_m * h * h));
+    o->code_to_s = o->code_to_cm / kC;
+    // photons (energy density 4 sigma T^4 / c) plus N_eff neutrino species at
+    // 7/8 (4/11)^(4/3) of the photon density each, over 3 H^2 c^2 / (8 pi G)
+    const double T = kTcmb * (double)o->Theta;
+    const double nu = 1.0 + kNeff * (7.0 / 8.0) * std::pow(4.0 / 11.0, 4.0 / 3.0);
+    o->omega_r = (float)(32.0 * kPi / 3.0 * kG * kSigma * nu * (T * T) * (T * T) / (kC * kC * kC * h * h));
+    o->omega_lam = (float)(1.0 - (double)o->omega_r - (double)o->omega_m);
+    return CT2_OK;
+}
+
+extern "C" int ct2_options_cosmology(const ct2_options* o, ct2_cosmology* c) {
+    if (!o || !c) return ct2::set_error(CT2_ERR_INVALID, "ct2_options_cosmology: null argument");
+    c->H = kH0 * o->code_to_s * (double)o->hubble;
+    c->omega_m = o->omega_m;
+    c->omega_r = o->omega_r;
+    c->omega_lam = o->omega_lam;
+    return CT2_OK;
+}

@@ -320,6 +320,44 @@ double ct2_cosmos_time(const ct2_cosmology* c, double a0, double a1);
 double ct2_cosmos_kick_factor(const ct2_cosmology* c, double a0, double a1);
 double ct2_cosmos_drift_factor(const ct2_cosmology* c, double a0, double a1);
 
+/* ------------------------------------------------------ options, units */
+/* The options block (options.hpp:14-32) with the reference's field types.
+ * ct2_options_defaults: the command-line defaults (options.cpp:51-61).
+ * ct2_options_derive: GM = 1/Ngrid^3, nparts = Ngrid^3, code units in cgs
+ *   (code_to_g = M_sun mass_res, code_to_cm the side of a box holding nparts
+ *   such masses at omega_m times the critical density, code_to_s =
+ *   code_to_cm / c), omega_r from the CMB temperature 2.73 Theta K with
+ *   N_eff = 3.086, and omega_lam = 1 - omega_r - omega_m (options.cpp:88-95,
+ *   constants of constants.hpp:14-39).  CT2_ERR_INVALID if Ngrid, mass_res,
+ *   hubble or omega_m is not positive.
+ * ct2_options_cosmology: the block the cosmology functions read, with
+ *   H = H0 code_to_s hubble in code time units (cosmology.cpp:15,47).
+ * The three code units are FP64 here, FP32 in the reference: M_sun mass_res
+ * exceeds the FP32 range for mass_res above 1.7e5 solar masses, so the
+ * reference's code_to_g, code_to_cm, code_to_s and H are infinite at its own
+ * default mass_res of 1e9.  The other fields keep the FP32 rounding. */
+typedef struct {
+    float bwidth;
+    double code_to_cm;
+    double code_to_g;
+    double code_to_s;
+    float eta;
+    float GM;
+    float hsoft;
+    float hubble;
+    float omega_lam;
+    float omega_m;
+    float omega_r;
+    float mass_res;
+    uint64_t Ngrid;
+    uint64_t nparts;
+    int ppcell;
+    float Theta;
+} ct2_options;
+int ct2_options_defaults(ct2_options* o);
+int ct2_options_derive(ct2_options* o);
+int ct2_options_cosmology(const ct2_options* o, ct2_cosmology* c);
+
 #ifdef __cplusplus
 }
 #endif

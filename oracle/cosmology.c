@@ -89,3 +89,28 @@ double orc_cosmos_time(const orc_cosmo* c, double a0, double a1) {
     }
     return t;
 }
+
+/* options.cpp:88-95 in the reference's order of evaluation, each right-hand
+ * side in FP64 (the float operands promoted); GM and the omegas stored to
+ * FP32 as there, the code units kept in FP64.  Constants of
+ * constants.hpp:14-39 (cgs). */
+void orc_options_derive(uint64_t ngrid, float mass_res, float hubble, float omega_m, float theta, double* out) {
+    const double c = 2.99792458e10, G = 6.67259e-8, sigma = 5.67051e-5, H0 = 1e7 / 3.086e24, M0 = 1.989e33;
+    const double Neff = 3.086;
+    const uint64_t nparts = ngrid * ngrid * ngrid;
+    const float GM = (float)(1.0 / (double)nparts);
+    const double code_to_g = M0 * mass_res;
+    const double code_to_cm =
+        pow(code_to_g * (8.0 * M_PI) * (double)nparts * G / (3.0 * omega_m * sq(H0 * hubble)), 1.0 / 3.0);
+    const double code_to_s = code_to_cm / c;
+    const float omega_r = (float)(32.0 * M_PI / 3.0 * G * sigma * (1 + Neff * (7. / 8.0) * pow(4. / 11., 4. / 3.)) *
+                                  pow(H0, -2) * pow(c, -3) * pow(2.73 * theta, 4) * pow(hubble, -2));
+    const float omega_lam = (float)(1.0 - omega_r - omega_m);
+    out[0] = GM;
+    out[1] = code_to_g;
+    out[2] = code_to_cm;
+    out[3] = code_to_s;
+    out[4] = omega_r;
+    out[5] = omega_lam;
+    out[6] = H0 * code_to_s * hubble;
+}

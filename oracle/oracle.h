@@ -108,6 +108,14 @@ double orc_cosmos_dadt(const orc_cosmo* c, double a);
 double orc_cosmos_conformal_time(const orc_cosmo* c, double a0, double a1);
 double orc_cosmos_time(const orc_cosmo* c, double a0, double a1);
 
+/* ---- derived options and code units (options.cpp:88-95) ----
+ * out: GM, code_to_g, code_to_cm, code_to_s, omega_r, omega_lam, H.  GM and
+ * the omegas are rounded to FP32 as the reference's options block stores
+ * them; the code units are kept in FP64 (in FP32 M_sun mass_res overflows at
+ * the reference's default mass_res); H = H0 code_to_s hubble
+ * (cosmology.cpp:15). */
+void orc_options_derive(uint64_t ngrid, float mass_res, float hubble, float omega_m, float theta, double* out);
+
 #ifdef __cplusplus
 }
 #endif

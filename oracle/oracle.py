@@ -92,6 +92,7 @@ def _declare(L):
     L.orc_cosmos_conformal_time.restype = d
     L.orc_cosmos_time.argtypes = [C.POINTER(Cosmo), d, d]
     L.orc_cosmos_time.restype = d
+    L.orc_options_derive.argtypes = [C.c_uint64, f, f, f, f, _P]
 
 
 def _p(a: np.ndarray):
@@ -335,3 +336,13 @@ def cosmos_conformal_time(c: Cosmo, a0, a1):
 
 def cosmos_time(c: Cosmo, a0, a1):
     return lib().orc_cosmos_time(C.byref(c), a0, a1)
+
+
+def options_derive(ngrid, mass_res, hubble, omega_m, theta):
+    """Derived options (options.cpp:88-95): a dict of GM, omega_r, omega_lam
+    (FP32 values), code_to_g, code_to_cm, code_to_s and the H of
+    cosmology.cpp:15 (FP64)."""
+    out = np.zeros(7, np.float64)
+    lib().orc_options_derive(ngrid, mass_res, hubble, omega_m, theta, _p(out))
+    keys = ("GM", "code_to_g", "code_to_cm", "code_to_s", "omega_r", "omega_lam", "H")
+    return {k: float(v) for k, v in zip(keys, out)}
