@@ -381,7 +381,7 @@ __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __res
 #pragma unroll SYM_UNROLL
             for (int t = 0; t < 64; ++t) {
                 const uint4 src = Sl[t];
-                float gx = 0.f, gy = 0.f, gz = 0.f, gp = 0.f;
+                float gx, gy, gz, gp;  // source-side sum of this step, started by sink 0
 #pragma unroll
                 for (int q = 0; q < SYM_NS; ++q) {
                     float dx, dy, dz, k, p;
@@ -390,10 +390,17 @@ __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __res
                     ty[q] = fmaf(dy, k, ty[q]);
                     tz[q] = fmaf(dz, k, tz[q]);
                     tp[q] += p;
-                    gx = fmaf(dx, k, gx);
-                    gy = fmaf(dy, k, gy);
-                    gz = fmaf(dz, k, gz);
-                    gp += p;
+                    if (q == 0) {
+                        gx = dx * k;
+                        gy = dy * k;
+                        gz = dz * k;
+                        gp = p;
+                    } else {
+                        gx = fmaf(dx, k, gx);
+                        gy = fmaf(dy, k, gy);
+                        gz = fmaf(dz, k, gz);
+                        gp += p;
+                    }
                 }
                 ax = rol1(ax) - gx;
                 ay = rol1(ay) - gy;
