@@ -13,7 +13,8 @@
 // PM + near field tends to the periodic direct sum.
 //
 // Pipeline (all on the library stream, deterministic):
-//   near_bin      cell of each particle, per-cell counts (atomics) and slots
+//   near_bin      cell of each particle, per-cell counts and slots (one atomic
+//                 per run of equal cells in a wave)
 //   near_scan     exclusive scan of the counts -> cell starts
 //   near_rows     work items: chunks of NEAR_BLOCK sinks within one (x, y) row
 //   near_scatter  (x, y, z, index) into its cell's segment (slot order)
@@ -32,6 +33,7 @@
 
 #include "internal.h"
 #include "near_erfc.h"
+#include "wave.h"
 
 namespace ct2 {
 namespace {
@@ -85,10 +87,13 @@ __global__ __launch_bounds__(256) void near_bin(const uint32_t* __restrict__ X, 
                                                 int* __restrict__ count, int* __restrict__ cell,
                                                 int* __restrict__ slot) {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    const int c = (int)((cell_of(X[p], nc) * nc + cell_of(Y[p], nc)) * nc + cell_of(Z[p], nc));
-    cell[p] = c;
-    slot[p] = atomicAdd(count + c, 1);
+    int c = -1;
+    if (p < n) {
+        c = (int)((cell_of(X[p], nc) * nc + cell_of(Y[p], nc)) * nc + cell_of(Z[p], nc));
+        cell[p] = c;
+    }
+    const int k = run_increment(count, c);  // every lane of the wave takes part
+    if (p < n) slot[p] = k;
 }
 
 // Exclusive scan of len values (one workgroup): out[0..len], out[len] = total.

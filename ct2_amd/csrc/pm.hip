@@ -3,8 +3,12 @@
 // Replaces the reference's multipole-mesh far field (particle_compute_multipoles
 // and particle_compute_expansions, particle.cpp:180-271, evaluated by L2P at
 // kick.cpp:29) with cloud-in-cell mass assignment, keeping its pipeline:
-//   mesh_deposit  fixed-point CIC (default) or TSC weights, uint64 atomics:
-//                 deterministic, bit-identical to the CPU oracle (12 B/particle read)
+//   deposit       fixed-point CIC (default) or TSC weights, uint64 atomics:
+//                 deterministic, bit-identical to the CPU oracle.  Grids of 32
+//                 nodes a side or more: particles binned by 16 x 16 x 32-node
+//                 tile, each tile summed in LDS, then one global atomic per
+//                 touched node (dep_*); smaller grids: one global atomic per
+//                 particle and stencil node (mesh_deposit)
 //   to_complex    density grid -> FP32 complex grid
 //   fft3d         forward FP32 transform (fft.hip)
 //   green_mul     rho_hat * G (FP64 Green's table, octant-symmetric), then
@@ -18,6 +22,7 @@
 
 #include "epilogue.h"
 #include "internal.h"
+#include "wave.h"
 
 namespace ct2 {
 namespace {
@@ -34,6 +39,13 @@ struct PmState {
     float2* A = nullptr;              // rho_hat -> Fx + i Fy
     float2* B = nullptr;              // Fz + i pot
     double* G = nullptr;              // Green's table on the octant, (ng/2+1)^3
+    // tiled deposit: particles binned by tile of 16 x 16 x 32 nodes
+    int ntile = 0;
+    int* tcount = nullptr;  // [ntile]
+    int* tstart = nullptr;  // [ntile + 1]
+    size_t cap = 0;         // particles slot / order hold
+    int* slot = nullptr;    // [cap] position within the tile
+    int* order = nullptr;   // [cap] particle indices grouped by tile
 } g_pm;
 int g_assign = 1;  // mass assignment: 1 CIC, 2 TSC (kept across ct2_pm_init)
 
@@ -90,6 +102,135 @@ __global__ __launch_bounds__(PM_BLOCK) void mesh_deposit(const uint32_t* __restr
             for (int dz = 0; dz <= ORDER; ++dz)
                 atomicAdd(W + row + ((k0 + dz) & mask), (unsigned long long)((axy * az[dz]) >> 31));
         }
+}
+
+// Tiled deposit for grids of 32 nodes and more a side.  The grid is cut into
+// tiles of 16 x 16 x 32 nodes; a particle belongs to the tile of its anchor
+// node (CIC: its first node; TSC: its nearest node), so its stencil lies in the
+// tile grown by one node on every side.  Particles are binned by tile
+// (dep_count, tile_scan, dep_scatter); one workgroup per tile then adds its
+// particles' weights into an LDS copy of the grown tile (uint64 LDS atomics)
+// and adds each nonzero LDS node into the grid with one global atomic.  The
+// weights are those of mesh_deposit and integer sums do not depend on order,
+// so the grid is bit-identical to mesh_deposit's.
+constexpr int TILE_BX = 4, TILE_BY = 4, TILE_BZ = 5;  // log2 tile size per axis
+constexpr int TLX = (1 << TILE_BX) + 2, TLY = (1 << TILE_BY) + 2, TLZ = (1 << TILE_BZ) + 2;
+constexpr int TL_NODES = TLX * TLY * TLZ;  // 11,016 nodes: 88,128 B of LDS
+constexpr int DEP_THREADS = 1024;
+constexpr int TILE_MIN_G = TILE_BZ;  // grids of at least 2^5 nodes a side
+
+template <int ORDER>
+__device__ __forceinline__ uint32_t anchor_node(uint32_t x, int s) {
+    return ORDER == 1 ? x >> s : (uint32_t)(x + (1u << (s - 1))) >> s;
+}
+
+template <int ORDER>
+__device__ __forceinline__ int tile_of(uint32_t x, uint32_t y, uint32_t z, int g) {
+    const int s = 32 - g;
+    const int tx = (int)(anchor_node<ORDER>(x, s) >> TILE_BX);
+    const int ty = (int)(anchor_node<ORDER>(y, s) >> TILE_BY);
+    const int tz = (int)(anchor_node<ORDER>(z, s) >> TILE_BZ);
+    return (tx << (2 * g - TILE_BY - TILE_BZ)) + (ty << (g - TILE_BZ)) + tz;
+}
+
+__global__ __launch_bounds__(PM_BLOCK) void fill_i32(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * PM_BLOCK + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(PM_BLOCK) void dep_count(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                      const uint32_t* __restrict__ Z, int n, int g,
+                                                      int* __restrict__ tcount, int* __restrict__ slot) {
+    const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+    const int t = p < n ? tile_of<ORDER>(X[p], Y[p], Z[p], g) : -1;
+    const int k = run_increment(tcount, t);  // every lane of the wave takes part
+    if (p < n) slot[p] = k;
+}
+
+// Exclusive scan of len counts (one workgroup): out[0..len], out[len] = total.
+__global__ __launch_bounds__(DEP_THREADS) void tile_scan(const int* __restrict__ in, int len, int* __restrict__ out) {
+    __shared__ int part[DEP_THREADS];
+    const int t = threadIdx.x;
+    const int per = (len + DEP_THREADS - 1) / DEP_THREADS;
+    const int b = min(len, t * per), e = min(len, b + per);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < DEP_THREADS; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int i = b; i < e; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (t == DEP_THREADS - 1) out[len] = part[t];
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(PM_BLOCK) void dep_scatter(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                        const uint32_t* __restrict__ Z, int n, int g,
+                                                        const int* __restrict__ tstart, const int* __restrict__ slot,
+                                                        int* __restrict__ order) {
+    const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+    if (p >= n) return;
+    order[tstart[tile_of<ORDER>(X[p], Y[p], Z[p], g)] + slot[p]] = p;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(DEP_THREADS) void dep_tile(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                        const uint32_t* __restrict__ Z, const int* __restrict__ order,
+                                                        const int* __restrict__ tstart, int g,
+                                                        unsigned long long* __restrict__ W) {
+    __shared__ unsigned long long acc[TL_NODES];
+    const int tile = blockIdx.x;
+    const int b = tstart[tile], e = tstart[tile + 1];
+    if (b == e) return;  // uniform across the workgroup
+    const int tid = threadIdx.x;
+    const int s = 32 - g;
+    const uint32_t mask = (1u << g) - 1u;
+    const uint32_t ox = (uint32_t)(tile >> (2 * g - TILE_BY - TILE_BZ)) << TILE_BX;
+    const uint32_t oy = (uint32_t)((tile >> (g - TILE_BZ)) & ((1 << (g - TILE_BY)) - 1)) << TILE_BY;
+    const uint32_t oz = (uint32_t)(tile & ((1 << (g - TILE_BZ)) - 1)) << TILE_BZ;
+    for (int i = tid; i < TL_NODES; i += DEP_THREADS) acc[i] = 0ull;
+    __syncthreads();
+    // local index of the stencil's first node: anchor - origin + (CIC: 1, TSC: 0)
+    constexpr uint32_t SHIFT = ORDER == 1 ? 1u : 0u;
+    for (int k = b + tid; k < e; k += DEP_THREADS) {
+        const int p = order[k];
+        const uint32_t x = X[p], y = Y[p], z = Z[p];
+        uint32_t lo;
+        uint64_t ax[ORDER + 1], ay[ORDER + 1], az[ORDER + 1];
+        assign_axis<ORDER>(x, s, lo, ax);
+        assign_axis<ORDER>(y, s, lo, ay);
+        assign_axis<ORDER>(z, s, lo, az);
+        const int lx = (int)(anchor_node<ORDER>(x, s) - ox + SHIFT);
+        const int ly = (int)(anchor_node<ORDER>(y, s) - oy + SHIFT);
+        const int lz = (int)(anchor_node<ORDER>(z, s) - oz + SHIFT);
+#pragma unroll
+        for (int dx = 0; dx <= ORDER; ++dx)
+#pragma unroll
+            for (int dy = 0; dy <= ORDER; ++dy) {
+                const uint64_t axy = ax[dx] * ay[dy];
+                unsigned long long* row = acc + ((lx + dx) * TLY + (ly + dy)) * TLZ + lz;
+#pragma unroll
+                for (int dz = 0; dz <= ORDER; ++dz)
+                    atomicAdd(row + dz, (unsigned long long)((axy * az[dz]) >> 31));
+            }
+    }
+    __syncthreads();
+    for (int i = tid; i < TL_NODES; i += DEP_THREADS) {
+        const unsigned long long v = acc[i];
+        if (!v) continue;
+        const int lx = i / (TLY * TLZ), ly = (i / TLZ) % TLY, lz = i % TLZ;
+        const size_t gx = (ox + lx - 1u) & mask, gy = (oy + ly - 1u) & mask, gz = (oz + lz - 1u) & mask;
+        atomicAdd(W + ((((gx << g) + gy) << g) + gz), v);
+    }
 }
 
 __global__ __launch_bounds__(PM_BLOCK) void to_complex(const unsigned long long* __restrict__ W,
@@ -164,7 +305,52 @@ void free_pm() {
     dev_free(g_pm.A);
     dev_free(g_pm.B);
     dev_free(g_pm.G);
+    dev_free(g_pm.tcount);
+    dev_free(g_pm.tstart);
+    dev_free(g_pm.slot);
+    dev_free(g_pm.order);
     g_pm = PmState{};
+}
+
+// Per-particle buffers of the tiled deposit, grown to the container size.
+int ensure_deposit(size_t n) {
+    if (n <= g_pm.cap) return CT2_OK;
+    dev_free(g_pm.slot);
+    dev_free(g_pm.order);
+    g_pm.slot = g_pm.order = nullptr;
+    g_pm.cap = 0;
+    CT2_TRY(dev_alloc((void**)&g_pm.slot, n * sizeof(int), "PM deposit slots"));
+    CT2_TRY(dev_alloc((void**)&g_pm.order, n * sizeof(int), "PM deposit order"));
+    g_pm.cap = n;
+    return CT2_OK;
+}
+
+// Density of the container's particles into g_pm.W (zeroed here).
+template <int ORDER>
+int deposit(const Context& c) {
+    const int n = (int)c.n, g = g_pm.g;
+    const long n3 = 1L << (3 * g);
+    const unsigned gb = (unsigned)((n3 + PM_BLOCK - 1) / PM_BLOCK), pb = (unsigned)((n + PM_BLOCK - 1) / PM_BLOCK);
+    fill_u64<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, n3);
+    CT2_LAUNCH_CHECK();
+    if (g < TILE_MIN_G) {
+        mesh_deposit<ORDER><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.W);
+        CT2_LAUNCH_CHECK();
+        return CT2_OK;
+    }
+    CT2_TRY(ensure_deposit(c.n));
+    const int nt = g_pm.ntile;
+    fill_i32<<<(nt + PM_BLOCK - 1) / PM_BLOCK, PM_BLOCK, 0, c.stream>>>(g_pm.tcount, nt);
+    CT2_LAUNCH_CHECK();
+    dep_count<ORDER><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.tcount, g_pm.slot);
+    CT2_LAUNCH_CHECK();
+    tile_scan<<<1, DEP_THREADS, 0, c.stream>>>(g_pm.tcount, nt, g_pm.tstart);
+    CT2_LAUNCH_CHECK();
+    dep_scatter<ORDER><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.tstart, g_pm.slot, g_pm.order);
+    CT2_LAUNCH_CHECK();
+    dep_tile<ORDER><<<nt, DEP_THREADS, 0, c.stream>>>(c.x, c.y, c.z, g_pm.order, g_pm.tstart, g, g_pm.W);
+    CT2_LAUNCH_CHECK();
+    return CT2_OK;
 }
 
 // FP64 Green's table on the octant |m_d| <= ng/2 (host, libm):
@@ -222,13 +408,7 @@ int pm_forces() {
     const int n = (int)c.n, g = g_pm.g;
     const long n3 = 1L << (3 * g);
     const unsigned gb = (unsigned)((n3 + PM_BLOCK - 1) / PM_BLOCK), pb = (unsigned)((n + PM_BLOCK - 1) / PM_BLOCK);
-    fill_u64<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, n3);
-    CT2_LAUNCH_CHECK();
-    if (g_assign == 1)
-        mesh_deposit<1><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.W);
-    else
-        mesh_deposit<2><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.W);
-    CT2_LAUNCH_CHECK();
+    CT2_TRY(g_assign == 1 ? deposit<1>(c) : deposit<2>(c));
     to_complex<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, g_pm.A, n3);
     CT2_LAUNCH_CHECK();
     CT2_TRY(fft3d(g_pm.A, g_pm.ng, 1, false));
@@ -278,6 +458,14 @@ extern "C" int ct2_pm_init(int ng, double rs_cells) {
     }
     g_pm.ng = ng;
     g_pm.g = ilog2(ng);
+    if (g_pm.g >= TILE_MIN_G) {
+        g_pm.ntile = 1 << (3 * g_pm.g - TILE_BX - TILE_BY - TILE_BZ);
+        if ((rc = dev_alloc((void**)&g_pm.tcount, (size_t)g_pm.ntile * sizeof(int), "PM tile counts")) ||
+            (rc = dev_alloc((void**)&g_pm.tstart, (size_t)(g_pm.ntile + 1) * sizeof(int), "PM tile starts"))) {
+            free_pm();
+            return rc;
+        }
+    }
     g_pm.rs_cells = rs_cells;
     g_pm.gm_table = 0.0f;
     return CT2_OK;

@@ -18,7 +18,7 @@ def load(ct2, x, y, z, vx=None, vy=None, vz=None, rung=None):
     ct2.particles_upload(x, y, z, vx, vy, vz, rung)
 
 
-@pytest.mark.parametrize("ng", [16, 64])
+@pytest.mark.parametrize("ng", [16, 32, 64, 256])
 def test_cic_deposit_bit_exact(ct2, orc, ng):
     """Integer atomics make the deposit order-independent: the GPU grid equals
     the CPU restatement bit for bit, clusters (heavy contention) included."""
@@ -143,7 +143,7 @@ def tsc(ct2):
     ct2.pm_assignment(1)
 
 
-@pytest.mark.parametrize("ng", [16, 128])
+@pytest.mark.parametrize("ng", [16, 32, 128, 256])
 def test_tsc_deposit_bit_exact(ct2, orc, tsc, ng):
     x, y, z = ct2.ic_plummer_mix(50000, 30000, 3, 0.005, 0.05, 22)
     load(ct2, x, y, z)
@@ -151,6 +151,35 @@ def test_tsc_deposit_bit_exact(ct2, orc, tsc, ng):
     ct2.pm_init(ng, 1.25)
     ct2.pm_forces()
     assert np.array_equal(ct2.pm_density_download(), orc.mesh_deposit(x, y, z, ng.bit_length() - 1, 2))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("ng", [8, 32, 64])
+@pytest.mark.parametrize("n", [1, 63, 1037, 4096])
+def test_deposit_edge_cases_bit_exact(ct2, orc, order, ng, n):
+    """Ragged particle counts (partial waves), lattice order (long runs of
+    one tile), and positions on the wrap, on nodes and on half nodes."""
+    rng = np.random.default_rng(n * 131 + ng)
+    side = int(round(n ** (1 / 3))) + 1
+    q = (np.arange(n) % side).astype(np.uint64)
+    x = ((q * TWO32) // side).astype(np.uint32)
+    y = ((((np.arange(n) // side) % side).astype(np.uint64) * TWO32) // side).astype(np.uint32)
+    z = rng.integers(0, TWO32, n, dtype=np.uint64).astype(np.uint32)
+    s = 32 - (ng.bit_length() - 1)
+    special = np.array([0, 1, TWO32 - 1, 1 << (s - 1), (1 << (s - 1)) - 1, TWO32 - (1 << (s - 1)), 1 << s,
+                        TWO32 - (1 << s)], np.uint64).astype(np.uint32)
+    k = min(n, special.size)
+    x[:k], y[-k:], z[:k] = special[:k], special[:k], special[::-1][:k]
+    load(ct2, x, y, z)
+    ct2.set_force_options(1e-3, 1.0 / n, 0)
+    ct2.pm_assignment(order)
+    try:
+        ct2.pm_init(ng, 1.25)
+        ct2.pm_forces()
+        got = ct2.pm_density_download()
+    finally:
+        ct2.pm_assignment(1)
+    assert np.array_equal(got, orc.mesh_deposit(x, y, z, ng.bit_length() - 1, order))
 
 
 @pytest.mark.parametrize("ng,rs", [(32, 1.25), (64, 2.0)])
