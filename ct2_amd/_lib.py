@@ -90,6 +90,7 @@ _SIGS = {
     "ct2_particle_rung_counts": ([_P, C.c_int, _P], C.c_int),
     "ct2_particle_set_minrung": ([C.c_int], C.c_int),
     "ct2_pm_near_cells": ([], C.c_int),
+    "ct2_near_kernel_table": ([C.c_double, C.c_double, C.c_int, _P, _P], C.c_int),
     "ct2_pm_assignment": ([C.c_int], C.c_int),
     "ct2_kick_pm": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_pm_density_download": ([_P], C.c_int),

@@ -189,6 +189,15 @@ def pm_near_cells() -> int:
     return int(L.load().ct2_pm_near_cells())
 
 
+def near_kernel_table(rs_u: float, rcut_u: float, nt: int = 64):
+    """The cubic pieces of the near-field kernel's filtered part (host
+    function): (k4, p4), each float32 of shape (nt, 4)."""
+    k4, p4 = np.zeros((nt, 4), np.float32), np.zeros((nt, 4), np.float32)
+    L.check(L.load().ct2_near_kernel_table(rs_u, rcut_u, nt, k4.ctypes.data_as(C.c_void_p),
+                                           p4.ctypes.data_as(C.c_void_p)))
+    return k4, p4
+
+
 def pm_forces():
     """PM force and potential of every particle: (fx, fy, fz, pot)."""
     L.check(L.load().ct2_pm_forces())

@@ -29,10 +29,10 @@
 // the union of their neighbourhoods is 9 contiguous (or wrapped) cell ranges
 // of the sorted array.
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "internal.h"
-#include "near_erfc.h"
 #include "wave.h"
 
 namespace ct2 {
@@ -45,6 +45,11 @@ namespace {
 // sinks span little of their row and the wave skips distant sources
 constexpr int NEAR_BLOCK = NEAR_BLOCK_SIZE;
 constexpr int NEAR_MAX_NC = 64;  // cells per side at most
+// The filtered Newtonian kernel is evaluated from NEAR_NT cubic pieces in r^2
+// over [0, rcut^2): within 2e-8 (relative) of the closed form, below FP32
+// rounding (built on the host in FP64 by near_kernel_table).
+constexpr int NEAR_NT = 64;
+static_assert(NEAR_NT <= NEAR_BLOCK, "one table entry per thread is loaded into LDS");
 constexpr int SCAN_THREADS = 1024;
 
 struct NearState {
@@ -61,16 +66,15 @@ struct NearState {
     int* row_chunk = nullptr;      // [nc^2 + 1]
     int2* item = nullptr;          // [cap / NEAR_BLOCK + nc^2] (row, chunk) per work item
     size_t item_cap = 0;
+    float4* table = nullptr;       // [2 * NEAR_NT]: k_l pieces, then p_l pieces
+    double table_rsu = 0.0, table_rcu = 0.0;  // what the table was built for
 } g_near;
 
 struct NearConst {
     float rc2u;      // rcut^2 in 2^-64 units
     float eps2u;     // Plummer eps^2
     float h2u, hinvu, hinv3u;  // ct2 kernel
-    float inv4rs2u;  // 1 / (4 rs^2)
-    float inv2rsu;   // 1 / (2 rs)
-    float kl0;       // (2/sqrt(pi)) / (2 rs)^3
-    float pl0;       // (2/sqrt(pi)) / (2 rs)
+    float tscale;    // NEAR_NT / rcut^2: r^2 -> table coordinate
 };
 
 __device__ __forceinline__ uint32_t cell_of(uint32_t x, uint32_t nc) {
@@ -175,9 +179,12 @@ __device__ __forceinline__ void two_sum(float& s, float& c, float v) {
 }
 
 // Softened kernel minus the filtered Newtonian kernel for one pair, in
-// 2^-32 units: returns the force factor k (F += dx k) and potential p.
+// 2^-32 units: returns the force factor k (F += dx k) and potential p.  The
+// filtered part (k_l, p_l) comes from the cubic pieces in LDS (tk, tp), so
+// no branch depends on the separation beyond the kernel's own.
 template <int KERNEL>
-__device__ __forceinline__ void near_pair(float r2, const NearConst& q, float& k, float& p) {
+__device__ __forceinline__ void near_pair(float r2, const NearConst& q, const float4* tk, const float4* tp, float& k,
+                                          float& p) {
     float ks, ps;
     if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
         const float w = __builtin_amdgcn_rsqf(r2 + q.eps2u);
@@ -193,39 +200,12 @@ __device__ __forceinline__ void near_pair(float r2, const NearConst& q, float& k
             ps = w;
         }
     }
-    const float u2 = r2 * q.inv4rs2u;
-    float kl, pl;
-    if (u2 < 0.25f) {
-        // series of [erf(u) - (2u/sqrt(pi)) e^{-u^2}] / u^3 and erf(u) / u
-        // (2/sqrt(pi) factored out), exact limits at r = 0 (self pair)
-        float sk = fmaf(u2, 1.0f / 5400.0f, -1.0f / 780.0f);
-        sk = fmaf(u2, sk, 1.0f / 132.0f);
-        sk = fmaf(u2, sk, -1.0f / 27.0f);
-        sk = fmaf(u2, sk, 1.0f / 7.0f);
-        sk = fmaf(u2, sk, -2.0f / 5.0f);
-        sk = fmaf(u2, sk, 2.0f / 3.0f);
-        float sp = fmaf(u2, 1.0f / 9360.0f, -1.0f / 1320.0f);
-        sp = fmaf(u2, sp, 1.0f / 216.0f);
-        sp = fmaf(u2, sp, -1.0f / 42.0f);
-        sp = fmaf(u2, sp, 1.0f / 10.0f);
-        sp = fmaf(u2, sp, -1.0f / 3.0f);
-        sp = fmaf(u2, sp, 1.0f);
-        kl = q.kl0 * sk;
-        pl = q.pl0 * sp;
-    } else {
-        // erfc(u) = e^{-u^2} g(u), g a degree-10 fit on u in [0.5, 4.5]
-        // (near_erfc.h); beyond 4.5, erfc < 2e-10 and s is clamped
-        const float rn = __builtin_amdgcn_rsqf(r2);
-        const float u = (r2 * rn) * q.inv2rsu;
-        const float ex = __expf(-u2);
-        const float s = fminf(fmaf(u, 2.0f / (NEAR_ERFC_U1 - NEAR_ERFC_U0), -(NEAR_ERFC_U0 + NEAR_ERFC_U1) / (NEAR_ERFC_U1 - NEAR_ERFC_U0)), 1.0f);
-        float g = NEAR_ERFC_C[NEAR_ERFC_DEG];
-#pragma unroll
-        for (int k = NEAR_ERFC_DEG - 1; k >= 0; --k) g = fmaf(g, s, NEAR_ERFC_C[k]);
-        const float rn3 = (rn * rn) * rn;
-        kl = rn3 * fmaf(-ex, fmaf(1.1283791670955126f, u, g), 1.0f);  // 1 - erfc(u) - (2u/sqrt(pi)) e^{-u^2}
-        pl = rn * fmaf(-ex, g, 1.0f);                                  // 1 - erfc(u)
-    }
+    const float t = r2 * q.tscale;  // in [0, NEAR_NT) inside the cutoff
+    const int i = min((int)t, NEAR_NT - 1);
+    const float f = t - (float)i;
+    const float4 a = tk[i], b = tp[i];
+    const float kl = fmaf(fmaf(fmaf(a.w, f, a.z), f, a.y), f, a.x);
+    const float pl = fmaf(fmaf(fmaf(b.w, f, b.z), f, b.y), f, b.x);
     k = ks - kl;
     p = ps - pl;
 }
@@ -241,10 +221,12 @@ template <int KERNEL>
 __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict__ P, const int* __restrict__ start,
                                                          const int* __restrict__ row_chunk,
                                                          const int2* __restrict__ item, int nc, NearConst q,
-                                                         double fscale, double pscale, double pconst,
-                                                         float* __restrict__ fx, float* __restrict__ fy,
-                                                         float* __restrict__ fz, float* __restrict__ pot) {
+                                                         const float4* __restrict__ table, double fscale,
+                                                         double pscale, double pconst, float* __restrict__ fx,
+                                                         float* __restrict__ fy, float* __restrict__ fz,
+                                                         float* __restrict__ pot) {
     __shared__ uint4 tile[NEAR_BLOCK];
+    __shared__ float4 lk[NEAR_NT], lp[NEAR_NT];  // the kernel's cubic pieces
     __shared__ int rng[18][2];
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
@@ -284,6 +266,10 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
         rng[2 * tid + 1][0] = a1;
         rng[2 * tid + 1][1] = e1;
     }
+    if (tid < NEAR_NT) {
+        lk[tid] = table[tid];
+        lp[tid] = table[NEAR_NT + tid];
+    }
     __syncthreads();
     const int i = s0 + tid;
     const uint4 me = P[i < s1 ? i : s1 - 1];
@@ -303,7 +289,7 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                 if (r2 < q.rc2u) {  // a source out of range of the whole wave is skipped
                     float k, p;
-                    near_pair<KERNEL>(r2, q, k, p);
+                    near_pair<KERNEL>(r2, q, lk, lp, k, p);
                     tx = fmaf(dx, k, tx);
                     ty = fmaf(dy, k, ty);
                     tz = fmaf(dz, k, tz);
@@ -340,6 +326,9 @@ void free_near_buffers() {
     dev_free(g_near.start);
     dev_free(g_near.row_chunk);
     dev_free(g_near.item);
+    dev_free(g_near.table);
+    g_near.table = nullptr;
+    g_near.table_rsu = g_near.table_rcu = 0.0;
     g_near.cell = g_near.slot = nullptr;
     g_near.P0 = g_near.P = nullptr;
     g_near.count = g_near.start = g_near.row_chunk = nullptr;
@@ -389,6 +378,71 @@ int ensure_buffers(size_t n, int nc) {
 
 }  // namespace
 
+// Long-range (filtered) part of the pair kernel at t = u^2, u = r / (2 rs),
+// without the 1/(2 rs)^3 and 1/(2 rs) factors:
+//   K(t) = [erf(u) - (2u/sqrt(pi)) e^{-u^2}] / u^3,   P(t) = erf(u) / u,
+// by their Taylor series in t below u = 0.05, where the closed forms cancel.
+static double lr_K(double t) {
+    const double c = 2.0 / std::sqrt(M_PI);
+    if (t < 0.0025) return c * (2.0 / 3.0 + t * (-2.0 / 5.0 + t * (1.0 / 7.0 + t * (-1.0 / 27.0))));
+    const double u = std::sqrt(t);
+    return (std::erf(u) - c * u * std::exp(-t)) / (t * u);
+}
+static double lr_P(double t) {
+    const double c = 2.0 / std::sqrt(M_PI);
+    if (t < 0.0025) return c * (1.0 + t * (-1.0 / 3.0 + t * (1.0 / 10.0 + t * (-1.0 / 42.0))));
+    const double u = std::sqrt(t);
+    return std::erf(u) / u;
+}
+
+// nt cubic pieces of k_l = K(t)/(2 rs)^3 and p_l = P(t)/(2 rs) over r^2 in
+// [0, rcut^2) (2^-32 position units); piece i is a polynomial in
+// f = r^2 nt / rcut^2 - i in [0, 1), coefficients (c0, c1, c2, c3), computed in
+// FP64 by interpolation at the four Chebyshev nodes of the piece.
+void near_kernel_table(double rsu, double rcu, int nt, float* k4, float* p4) {
+    constexpr int D = 4;
+    double node[D], inv[D][D];
+    for (int j = 0; j < D; ++j) node[j] = 0.5 - 0.5 * std::cos(M_PI * (j + 0.5) / D);
+    // inverse of the Vandermonde matrix V[j][m] = node[j]^m (Gauss-Jordan)
+    double a[D][2 * D];
+    for (int j = 0; j < D; ++j)
+        for (int m = 0; m < 2 * D; ++m) a[j][m] = m < D ? std::pow(node[j], m) : (m - D == j ? 1.0 : 0.0);
+    for (int col = 0; col < D; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < D; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        for (int m = 0; m < 2 * D; ++m) std::swap(a[col][m], a[piv][m]);
+        const double d = a[col][col];
+        for (int m = 0; m < 2 * D; ++m) a[col][m] /= d;
+        for (int r = 0; r < D; ++r)
+            if (r != col) {
+                const double f = a[r][col];
+                for (int m = 0; m < 2 * D; ++m) a[r][m] -= f * a[col][m];
+            }
+    }
+    for (int m = 0; m < D; ++m)
+        for (int j = 0; j < D; ++j) inv[m][j] = a[m][D + j];
+    const double s = 0.5 / rsu;  // 1 / (2 rs)
+    const double tmax = rcu * rcu * s * s;  // u^2 at the cutoff
+    for (int i = 0; i < nt; ++i) {
+        double vk[D], vp[D];
+        for (int j = 0; j < D; ++j) {
+            const double t = (i + node[j]) * tmax / nt;
+            vk[j] = lr_K(t) * s * s * s;
+            vp[j] = lr_P(t) * s;
+        }
+        for (int m = 0; m < D; ++m) {
+            double ck = 0.0, cp = 0.0;
+            for (int j = 0; j < D; ++j) {
+                ck += inv[m][j] * vk[j];
+                cp += inv[m][j] * vp[j];
+            }
+            k4[4 * i + m] = (float)ck;
+            p4[4 * i + m] = (float)cp;
+        }
+    }
+}
+
 bool near_enabled() { return g_near.rcut_rs > 0.0; }
 
 // Adds the near field to the stored PM force (rs in box units).
@@ -418,18 +472,23 @@ int near_forces(double rs) {
     CT2_LAUNCH_CHECK();
 
     const double u = 0x1p32;
-    const double rsu = rs * u, h = c.opts.hsoft, hu = h * u;
+    const double rsu = rs * u, rcu = rcut * u, h = c.opts.hsoft, hu = h * u;
     NearConst q;
-    q.rc2u = (float)(rcut * u * rcut * u);
+    q.rc2u = (float)(rcu * rcu);
     q.eps2u = (float)(hu * hu);
     q.h2u = (float)(hu * hu);
     q.hinvu = (float)(1.0 / hu);
     q.hinv3u = (float)(1.0 / (hu * hu * hu));
-    q.inv4rs2u = (float)(0.25 / (rsu * rsu));
-    q.inv2rsu = (float)(0.5 / rsu);
-    const double c2 = 2.0 / std::sqrt(M_PI);
-    q.kl0 = (float)(c2 * std::pow(0.5 / rsu, 3));
-    q.pl0 = (float)(c2 * 0.5 / rsu);
+    q.tscale = (float)(NEAR_NT / (rcu * rcu));
+    if (!g_near.table) CT2_TRY(dev_alloc((void**)&g_near.table, 2 * NEAR_NT * sizeof(float4), "near-field table"));
+    if (g_near.table_rsu != rsu || g_near.table_rcu != rcu) {
+        std::vector<float> t(8 * NEAR_NT);
+        near_kernel_table(rsu, rcu, NEAR_NT, t.data(), t.data() + 4 * NEAR_NT);
+        CT2_HIP(hipMemcpyAsync(g_near.table, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        CT2_HIP(hipStreamSynchronize(st));  // t is released on return
+        g_near.table_rsu = rsu;
+        g_near.table_rcu = rcu;
+    }
     const double gm = c.opts.GM;
     const double fscale = gm * 0x1p64, pscale = gm * 0x1p32;
     const double pconst = gm * (double)n * 4.0 * M_PI * rs * rs;
@@ -439,12 +498,12 @@ int near_forces(double rs) {
     const unsigned items = (unsigned)((n + NEAR_BLOCK - 1) / NEAR_BLOCK + nrow);
     if (c.opts.kernel == CT2_KERNEL_PLUMMER)
         near_force<CT2_KERNEL_PLUMMER><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk,
-                                                                     g_near.item, nc, q, fscale, pscale, pconst, c.fx,
-                                                                     c.fy, c.fz, c.pot);
+                                                                     g_near.item, nc, q, g_near.table, fscale, pscale,
+                                                                     pconst, c.fx, c.fy, c.fz, c.pot);
     else
         near_force<CT2_KERNEL_CT2><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk, g_near.item,
-                                                                 nc, q, fscale, pscale, pconst, c.fx, c.fy, c.fz,
-                                                                 c.pot);
+                                                                 nc, q, g_near.table, fscale, pscale, pconst, c.fx,
+                                                                 c.fy, c.fz, c.pot);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
@@ -464,3 +523,10 @@ void release_near() {
 }
 
 }  // namespace ct2
+
+extern "C" int ct2_near_kernel_table(double rs_u, double rcut_u, int nt, float* k4, float* p4) {
+    if (!(rs_u > 0.0) || !(rcut_u > 0.0) || nt < 1 || !k4 || !p4)
+        return ct2::set_error(CT2_ERR_INVALID, "ct2_near_kernel_table: bad argument");
+    ct2::near_kernel_table(rs_u, rcut_u, nt, k4, p4);
+    return CT2_OK;
+}

@@ -153,6 +153,17 @@ int ct2_pm_forces(void);
 int ct2_pm_near(double rcut_rs);
 /* Cells per side of the last near-field evaluation (0 before the first). */
 int ct2_pm_near_cells(void);
+/* The near-field kernel's filtered (long-range) part as the GPU evaluates it,
+ * for a split radius rs_u and cutoff rcut_u in 2^-32 position units: nt
+ * cubic pieces in r^2 over [0, rcut_u^2) of
+ *   k_l = [erf(u) - (2u/sqrt(pi)) e^{-u^2}] / (u^3 (2 rs)^3),
+ *   p_l = erf(u) / (u (2 rs)),   u = r / (2 rs),
+ * piece i evaluated at f = r^2 nt / rcut_u^2 - i as c0 + f (c1 + f (c2 + f c3)),
+ * coefficients (c0..c3) of piece i at k4[4i..4i+3] and p4[4i..4i+3].  The
+ * pair kernel subtracts these from the softened Newtonian kernel.  Host only,
+ * no device state.  No counterpart in the reference (its near field is the
+ * plain softened kernel with the mesh's far field left out). */
+int ct2_near_kernel_table(double rs_u, double rcut_u, int nt, float* k4, float* p4);
 /* Mass assignment of the PM force: 1 = cloud in cell (default), 2 =
  * triangular-shaped cloud (27 nodes, quadratic spline; the Green's table
  * deconvolves sinc^6 instead of sinc^4).  TSC lowers the mesh's
