@@ -142,7 +142,46 @@ def cpu_baseline(x, y, z, soft, gm, target_s: float):
     dt = time.perf_counter() - t0
     return {"value": m * n / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
             "sample": f"FP64 all-pairs kick of {m} random sinks against all {n} sources "
-                      f"({m * n:.3e} interactions, {dt:.1f} s, OpenMP threads={threads})"}
+                      f"({m * n:.3e} interactions, {dt:.1f} s, OpenMP threads={threads})",
+            "cpu_case": cpu_case(threads)}
+
+
+def cpu_case(threads: int) -> dict:
+    """The CPU-runnable case (32^3 particles, lattice + Gaussian 0.2 cell,
+    eps = 1/1600): one FP64 all-pairs kick (N^2 = 1.07e9 interactions) on all
+    threads and on one, one drift, and a 32^3 complex FP64 forward + inverse
+    FFT of the CIC density with the radix-2 transform in ct2's bit-reversed
+    order; medians of 5 runs after a warm-up."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(1)
+    side = 32
+    n = side ** 3
+    q = (np.indices((side,) * 3).reshape(3, -1).T + 0.5) / side + rng.normal(0.0, 0.2 / side, (n, 3))
+    x, y, z = (np.round((q[:, d] % 1.0) * 2.0 ** 32).astype(np.uint64).astype(np.uint32) for d in range(3))
+    soft, gm = 1.0 / (50 * side), 1.0 / n
+    v = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(3)]
+
+    def med(fn, reps=5):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    out = {}
+    for t in (threads, 1):
+        O.set_threads(t)
+        s = med(lambda: O.kick(x, y, z, gm, soft, O.KERNEL_PLUMMER), reps=5 if t > 1 else 3)
+        out[f"kick_fp64_{t}_threads"] = {"s": round(s, 4), "interactions_per_s": float(n) * n / s}
+    O.set_threads(threads)
+    s = med(lambda: O.drift(x, y, z, *v, 1e-3, 1.0))
+    out["drift"] = {"s": s, "GB/s": round(36 * n / s / 1e9, 2), "note": "includes the wrapper's copies of x, y, z"}
+    rho = O.cic_deposit(x, y, z, 5).astype(np.float64) * 2.0 ** -32
+    s = med(lambda: O.fft3d(O.fft3d(rho.astype(np.complex128), 1), -1))
+    out["fft3d_32_fp64_round_trip"] = {"s": s, "GB/s": round(2 * 96 * 32 ** 3 / s / 1e9, 3)}
+    return out
 
 
 def main() -> int:

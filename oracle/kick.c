@@ -9,6 +9,10 @@
 
 #include "oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 /* Minimum-image separation of two fixed32 coordinates, in box units.
  * sfmm's distance(x1, x0) as used at kick.cpp:44-46: the wrapped integer
  * difference read as signed.  Exact in double. */
@@ -179,4 +183,21 @@ void orc_drift(size_t n, uint32_t* x, uint32_t* y, uint32_t* z, const float* vx,
         y[i] += orc_fixed32_from_float(vy[i] * lambda);
         z[i] += orc_fixed32_from_float(vz[i] * lambda);
     }
+}
+
+/* Thread count of the OpenMP loops (for timing the oracle on 1 or all cores). */
+void orc_set_threads(int t) {
+#ifdef _OPENMP
+    omp_set_num_threads(t > 0 ? t : 1);
+#else
+    (void)t;
+#endif
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

@@ -51,6 +51,8 @@ int orc_kick_epilogue(size_t n, const float* fx, const float* fy, const float* f
 
 /* ---- drift (drift.cpp:4-17) ---- */
 uint32_t orc_fixed32_from_float(float s);
+void orc_set_threads(int t); /* OpenMP threads of the oracle's loops */
+int orc_max_threads(void);
 void orc_drift(size_t n, uint32_t* x, uint32_t* y, uint32_t* z, const float* vx,
                const float* vy, const float* vz, float dt, float scale);
 

@@ -85,6 +85,8 @@ def _declare(L):
     L.orc_mlp_forward.argtypes = [_P, sz, _P, i, _P]
     L.orc_mlp_grad.argtypes = [_P, sz, _P, _P, i, _P]
     L.orc_mlp_grad.restype = d
+    L.orc_set_threads.argtypes = [i]
+    L.orc_max_threads.restype = i
     L.orc_cosmos_update.argtypes = [C.POINTER(Cosmo), _P, _P, _P, d]
     L.orc_cosmos_dadt.argtypes = [C.POINTER(Cosmo), d]
     L.orc_cosmos_dadt.restype = d
@@ -134,6 +136,15 @@ def kick_epilogue(fx, fy, fz, vx, vy, vz, rung, params: KickParams, hsoft):
 
 def fixed32_from_float(s: float) -> int:
     return int(lib().orc_fixed32_from_float(s))
+
+
+def set_threads(t: int) -> None:
+    """OpenMP threads of the oracle's parallel loops."""
+    lib().orc_set_threads(int(t))
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
 
 
 def drift(x, y, z, vx, vy, vz, dt, scale):
