@@ -50,12 +50,15 @@ __device__ __forceinline__ float gelu(float z) {
     const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
     return 0.5f * z * (1.0f + t);
 }
-// The same GELU without the library tanh (branch-free, for the fused
-// inference kernel): tanh(u) = 1 - 2 / (1 + e^{2u}); within ~1e-6 of gelu.
+// The same GELU for the fused inference kernel, branch-free and without the
+// library tanh or a division: 0.5 z (1 + tanh u) = z / (1 + e^{-2u}), with
+// u = c (z + 0.044715 z^3), from the hardware exp2 and reciprocal (each within
+// about 1 ulp); within ~1e-6 of gelu, far below the bf16 rounding that follows.
 __device__ __forceinline__ float gelu_fast(float z) {
-    const float u = GELU_C * (z + 0.044715f * z * z * z);
-    const float t = 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * u));
-    return 0.5f * z * (1.0f + t);
+    constexpr float K1 = -2.0f * GELU_C * 1.4426950408889634f;  // -2 c log2(e)
+    constexpr float K3 = K1 * 0.044715f;
+    const float e = __builtin_amdgcn_exp2f(z * fmaf(z * z, K3, K1));
+    return z * __builtin_amdgcn_rcpf(1.0f + e);
 }
 __device__ __forceinline__ float gelu_d(float z) {
     const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
@@ -326,10 +329,18 @@ __global__ void apply_kernel(const float* __restrict__ Yo, long r0, int m, float
 // particle lane%16 -- can be fed to the next layer's B operand without moving:
 // of a 32-wide k-step, element i of lane group g is unit 16 (i/4) + 4 g + i%4
 // of the step, and the next layer's weights are stored with their k columns
-// in that order (kperm).  Eight waves per workgroup, 32 particles (two column
-// blocks) per wave; W2 (128 KB) sits in LDS for the kernel's life, W1 and W3
-// come from L2; persistent workgroups walk the tiles.
-constexpr int INF_WAVES = 8, INF_PW = 16, INF_TILE = INF_WAVES * INF_PW;
+// in that order (kperm).  Eight waves per workgroup, INF_CB column blocks of
+// 16 particles per wave; W2 (128 KB) sits in LDS for the kernel's life, W1
+// and W3 come from L2; persistent workgroups walk the tiles.
+#ifndef INF_CB_N
+#define INF_CB_N 2
+#endif
+// INF_CB column blocks of 16 particles per wave: each weight fragment loaded
+// (W1 from L2, W2 from LDS) feeds INF_CB MFMAs
+#ifndef INF_WAVES_N
+#define INF_WAVES_N 8
+#endif
+constexpr int INF_WAVES = INF_WAVES_N, INF_CB = INF_CB_N, INF_PW = 16 * INF_CB, INF_TILE = INF_WAVES * INF_PW;
 
 __host__ __device__ constexpr int kperm(int q) { return 16 * ((q & 7) >> 2) + 4 * (q >> 3) + (q & 3); }
 
@@ -348,17 +359,31 @@ __global__ void pack_infer(const float* __restrict__ w, bf16_t* __restrict__ W2p
     }
 }
 
+// Two floats to two bf16 (round to nearest even) in one word, low half first:
+// one v_cvt_pk_bf16_f32 on gfx950.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
 // bias + GELU of a 256-unit column block, packed as the next layer's B operand
-// (bias points at the lane's first unit, 4 * (lane / 16))
+// (bias points at the lane's first unit, 4 * (lane / 16)): units 4 of step r
+// go to words 2 (r & 1), 2 (r & 1) + 1 of h[r / 2]
 __device__ __forceinline__ void bias_gelu_pack(const floatx4 (&acc)[16], const float* __restrict__ bias,
-                                               bf16x8 (&h)[8]) {
+                                               uint4 (&h)[8]) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const float4 b = *(const float4*)(bias + 16 * r);
-        h[r >> 1][4 * (r & 1) + 0] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][0] + b.x)));
-        h[r >> 1][4 * (r & 1) + 1] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][1] + b.y)));
-        h[r >> 1][4 * (r & 1) + 2] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][2] + b.z)));
-        h[r >> 1][4 * (r & 1) + 3] = __builtin_bit_cast(__bf16, f2bf(gelu_fast(acc[r][3] + b.w)));
+        const uint32_t lo = pack_bf16x2(gelu_fast(acc[r][0] + b.x), gelu_fast(acc[r][1] + b.y));
+        const uint32_t hi = pack_bf16x2(gelu_fast(acc[r][2] + b.z), gelu_fast(acc[r][3] + b.w));
+        if (r & 1) {
+            h[r >> 1].z = lo;
+            h[r >> 1].w = hi;
+        } else {
+            h[r >> 1].x = lo;
+            h[r >> 1].y = hi;
+        }
     }
 }
 
@@ -385,48 +410,70 @@ __global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const bf16_t* __rest
         const bf16_t* w3 = W3p + zero + l16 * HID + 8 * g;
         const float* bb1 = b1 + zero + 4 * g;
         const float* bb2 = b2 + zero + 4 * g;
-        const int p = tile * INF_TILE + wave * INF_PW + l16;  // this lane's particle (column)
-        floatx4 acc[16];
+        // this lane's particles (columns): one in each of the wave's INF_CB blocks
+        const int p0 = tile * INF_TILE + wave * INF_PW + l16;
+        floatx4 acc[INF_CB][16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = floatx4{0.f, 0.f, 0.f, 0.f};
-        // layer 1: K = 128
+        for (int c = 0; c < INF_CB; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+        // layer 1: K = 128; each weight fragment serves every column block
 #pragma unroll
         for (int ks = 0; ks < KIN / 32; ++ks) {
-            const bf16x8 x = load8(X + (size_t)p * KIN + 32 * ks + 8 * g);
+            bf16x8 x[INF_CB];
+#pragma unroll
+            for (int c = 0; c < INF_CB; ++c) x[c] = load8(X + (size_t)(p0 + 16 * c) * KIN + 32 * ks + 8 * g);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const bf16x8 a = load8(w1 + 16 * r * KIN + 32 * ks);
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x, acc[r], 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < INF_CB; ++c)
+                    acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c], acc[c][r], 0, 0, 0);
             }
             __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight: no spills
         }
-        bf16x8 h[8];
-        bias_gelu_pack(acc, bb1, h);
+        uint4 h[INF_CB][8];
+#pragma unroll
+        for (int c = 0; c < INF_CB; ++c) bias_gelu_pack(acc[c], bb1, h[c]);
         // layer 2: K = 256, weights from LDS
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < INF_CB; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < HID / 32; ++ks) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const bf16x8 a = load8(sw2 + (16 * r + l16) * HID + 32 * ks + 8 * g);
-                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], acc[r], 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < INF_CB; ++c)
+                    acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][ks]),
+                                                                        acc[c][r], 0, 0, 0);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-        bias_gelu_pack(acc, bb2, h);
+#pragma unroll
+        for (int c = 0; c < INF_CB; ++c) bias_gelu_pack(acc[c], bb2, h[c]);
         // layer 3: outputs 0..15 (3 used)
-        floatx4 y = {0.f, 0.f, 0.f, 0.f};
+        floatx4 y[INF_CB];
+#pragma unroll
+        for (int c = 0; c < INF_CB; ++c) y[c] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < HID / 32; ++ks) {
             const bf16x8 a = load8(w3 + 32 * ks);
-            y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], y, 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < INF_CB; ++c)
+                y[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][ks]), y[c], 0, 0, 0);
         }
-        // lanes of group 0 hold outputs 0..3 of their particle
-        if (g == 0 && p < m) {
-            fx[r0 + p] += unit * (y[0] + b3[0]);
-            fy[r0 + p] += unit * (y[1] + b3[1]);
-            fz[r0 + p] += unit * (y[2] + b3[2]);
+        // lanes of group 0 hold outputs 0..3 of their particles
+#pragma unroll
+        for (int c = 0; c < INF_CB; ++c) {
+            const int p = p0 + 16 * c;
+            if (g == 0 && p < m) {
+                fx[r0 + p] += unit * (y[c][0] + b3[0]);
+                fy[r0 + p] += unit * (y[c][1] + b3[1]);
+                fz[r0 + p] += unit * (y[c][2] + b3[2]);
+            }
         }
     }
 }
@@ -470,7 +517,8 @@ void free_work() {
 
 // Work buffers for `rows` rows (rounded up to 128); training buffers only when asked.
 int ensure_work(int rows, bool training) {
-    rows = (rows + 255) / 256 * 256;
+    constexpr int ROUND = INF_TILE > 256 ? INF_TILE : 256;  // whole inference tiles and GEMM tiles
+    rows = (rows + ROUND - 1) / ROUND * ROUND;
     const bool have_train = g_mlp.Z1 != nullptr;
     if (g_mlp.cap >= rows && (!training || have_train)) return CT2_OK;
     const int cap = std::max(rows, g_mlp.cap);
