@@ -39,6 +39,14 @@ __host__ __device__ inline bf16_t f2bf(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (bf16_t)(u >> 16);
 }
+// Two floats to two bf16 (round to nearest even) in one word, low half first:
+// one v_cvt_pk_bf16_f32 on gfx950.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
 __host__ __device__ inline float bf2f(bf16_t h) {
     const uint32_t u = (uint32_t)h << 16;
     float f;
@@ -238,6 +246,18 @@ __global__ __launch_bounds__(256) void features_kernel(const uint32_t* __restric
         for (int k = 0; k < KIN; ++k) outf[(size_t)r * KIN + k] = f[k];
 }
 
+// The per-node part of the features for the fused inference kernel: node c's
+// four features (density / mean, PM force / F_unit) rounded to bf16 once, as
+// 8 bytes, so that a particle's 27-node gather reads 8 B per node from one
+// array.  The values are features_kernel's, bit for bit.
+__global__ __launch_bounds__(256) void node_grid(FeatGrid fg, long n3, uint2* __restrict__ node) {
+    const long cc = (long)blockIdx.x * 256 + threadIdx.x;
+    if (cc >= n3) return;
+    const float2 av = fg.A[cc], bv = fg.B[cc];
+    const float d = (float)((double)fg.W[cc] * 0x1p-32) * fg.dscale;
+    node[cc] = make_uint2(pack_bf16x2(d, av.x * fg.fscale), pack_bf16x2(av.y * fg.fscale, bv.x * fg.fscale));
+}
+
 // ------------------------------------------------------------------ training data
 // target = (F_judge - F_pm) / F_unit; column 3 = the sample's loss weight
 // 1 / (|F_judge| / F_unit)^2, so that the loss measures relative force error
@@ -359,14 +379,6 @@ __global__ void pack_infer(const float* __restrict__ w, bf16_t* __restrict__ W2p
     }
 }
 
-// Two floats to two bf16 (round to nearest even) in one word, low half first:
-// one v_cvt_pk_bf16_f32 on gfx950.
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
-}
-
 // bias + GELU of a 256-unit column block, packed as the next layer's B operand
 // (bias points at the lane's first unit, 4 * (lane / 16)): units 4 of step r
 // go to words 2 (r & 1), 2 (r & 1) + 1 of h[r / 2]
@@ -387,7 +399,17 @@ __device__ __forceinline__ void bias_gelu_pack(const floatx4 (&acc)[16], const f
     }
 }
 
-__global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const bf16_t* __restrict__ X, int ntiles,
+// Grid index of stencil node q (0..26: (a, b, c) = (q / 9, q / 3 % 3, q % 3) - 1)
+// around the nearest node (ic, jc, kc): the order of gather_features.
+__device__ __forceinline__ size_t stencil_node(int q, uint32_t ic, uint32_t jc, uint32_t kc, int g, uint32_t mask) {
+    const uint32_t a = (uint32_t)(q / 9) - 1u, b = (uint32_t)(q / 3 % 3) - 1u, c = (uint32_t)(q % 3) - 1u;
+    return ((size_t)((ic + a) & mask) << (2 * g)) + ((size_t)((jc + b) & mask) << g) + ((kc + c) & mask);
+}
+
+__global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const uint32_t* __restrict__ PX,
+                                                           const uint32_t* __restrict__ PY,
+                                                           const uint32_t* __restrict__ PZ,
+                                                           const uint2* __restrict__ node, int gbits, int ntiles,
                                                            const bf16_t* __restrict__ W1t,
                                                            const bf16_t* __restrict__ W2p,
                                                            const bf16_t* __restrict__ W3p, const float* __restrict__ w,
@@ -412,17 +434,48 @@ __global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const bf16_t* __rest
         const float* bb2 = b2 + zero + 4 * g;
         // this lane's particles (columns): one in each of the wave's INF_CB blocks
         const int p0 = tile * INF_TILE + wave * INF_PW + l16;
+        // their nearest nodes and in-cell offsets (features 108..110); the
+        // padded tail of the last tile repeats particle m - 1
+        const int s = 32 - gbits;
+        const uint32_t mask = (1u << gbits) - 1u, half = 1u << (s - 1);
+        const float inv = 1.0f / (float)(1u << s);
+        uint32_t ic[INF_CB], jc[INF_CB], kc[INF_CB], oxy[INF_CB], oz0[INF_CB];
+#pragma unroll
+        for (int c = 0; c < INF_CB; ++c) {
+            const long q = r0 + min(p0 + 16 * c, m - 1);
+            const uint32_t x = PX[q], y = PY[q], z = PZ[q];
+            ic[c] = ((x + half) >> s) & mask;
+            jc[c] = ((y + half) >> s) & mask;
+            kc[c] = ((z + half) >> s) & mask;
+            oxy[c] = pack_bf16x2((float)(int32_t)(x - (ic[c] << s)) * inv, (float)(int32_t)(y - (jc[c] << s)) * inv);
+            oz0[c] = pack_bf16x2((float)(int32_t)(z - (kc[c] << s)) * inv, 0.0f);
+        }
         floatx4 acc[INF_CB][16];
 #pragma unroll
         for (int c = 0; c < INF_CB; ++c)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
-        // layer 1: K = 128; each weight fragment serves every column block
+        // layer 1: K = 128; each weight fragment serves every column block.  The
+        // operand of lane group g at k-step ks is features 32 ks + 8 g .. + 7:
+        // stencil nodes 8 ks + 2 g and + 1 (4 features each) from the node grid;
+        // at ks = 3, group 1 holds node 26 and the offsets, groups 2, 3 zeros.
 #pragma unroll
         for (int ks = 0; ks < KIN / 32; ++ks) {
             bf16x8 x[INF_CB];
 #pragma unroll
-            for (int c = 0; c < INF_CB; ++c) x[c] = load8(X + (size_t)(p0 + 16 * c) * KIN + 32 * ks + 8 * g);
+            for (int c = 0; c < INF_CB; ++c) {
+                const int q = 8 * ks + 2 * g;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (ks < 3 || g == 0) {
+                    const uint2 n0 = node[stencil_node(q, ic[c], jc[c], kc[c], gbits, mask)];
+                    const uint2 n1 = node[stencil_node(q + 1, ic[c], jc[c], kc[c], gbits, mask)];
+                    v = make_uint4(n0.x, n0.y, n1.x, n1.y);
+                } else if (g == 1) {
+                    const uint2 n0 = node[stencil_node(26, ic[c], jc[c], kc[c], gbits, mask)];
+                    v = make_uint4(n0.x, n0.y, oxy[c], oz0[c]);
+                }
+                x[c] = __builtin_bit_cast(bf16x8, v);
+            }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const bf16x8 a = load8(w1 + 16 * r * KIN + 32 * ks);
@@ -484,6 +537,8 @@ struct Mlp {
     float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     bf16_t *W1t = nullptr, *W2t = nullptr, *W3t = nullptr, *W2n = nullptr, *W3n = nullptr;
     bf16_t *W2p = nullptr, *W3p = nullptr;  // k-permuted copies for the fused inference kernel
+    uint2* node = nullptr;                   // bf16 per-node features of the fused inference kernel
+    long node_cap = 0;
     long adam_t = 0;
     // work buffers for up to cap rows
     int cap = 0;
@@ -639,18 +694,25 @@ int mlp_correct_forces() {
     hipStream_t st = c.stream;
     const long n = (long)c.n;
     const int chunk = (int)std::min<long>(n, 1L << 22);
-    CT2_TRY(ensure_work(chunk, false));
     const FeatGrid fg = feat_grid();
     const float unit = (float)force_unit();
+    const long n3 = 1L << (3 * fg.g);
+    if (n3 > g_mlp.node_cap) {
+        dev_free(g_mlp.node);
+        g_mlp.node = nullptr;
+        g_mlp.node_cap = 0;
+        CT2_TRY(dev_alloc((void**)&g_mlp.node, (size_t)n3 * sizeof(uint2), "mlp node grid"));
+        g_mlp.node_cap = n3;
+    }
+    node_grid<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(fg, n3, g_mlp.node);
+    CT2_LAUNCH_CHECK();
     for (long r0 = 0; r0 < n; r0 += chunk) {
         const int m = (int)std::min<long>(chunk, n - r0);
         const int M = (m + INF_TILE - 1) / INF_TILE * INF_TILE;
-        features_kernel<<<(m + 255) / 256, 256, 0, st>>>(c.x, c.y, c.z, nullptr, r0, m, fg, g_mlp.X, nullptr);
-        CT2_LAUNCH_CHECK();
-        if (M > m) CT2_HIP(hipMemsetAsync(g_mlp.X + (size_t)m * KIN, 0, (size_t)(M - m) * KIN * 2, st));
         const int ntiles = M / INF_TILE;
-        mlp_infer<<<std::min(ntiles, 256), INF_WAVES * 64, 0, st>>>(g_mlp.X, ntiles, g_mlp.W1t, g_mlp.W2p, g_mlp.W3p,
-                                                                    g_mlp.w, r0, m, unit, c.fx, c.fy, c.fz);
+        mlp_infer<<<std::min(ntiles, 256), INF_WAVES * 64, 0, st>>>(c.x, c.y, c.z, g_mlp.node, fg.g, ntiles, g_mlp.W1t,
+                                                                    g_mlp.W2p, g_mlp.W3p, g_mlp.w, r0, m, unit, c.fx,
+                                                                    c.fy, c.fz);
         CT2_LAUNCH_CHECK();
     }
     return CT2_OK;
@@ -660,7 +722,7 @@ void release_mlp() {
     free_work();
     for (void* p : {(void*)g_mlp.w, (void*)g_mlp.m, (void*)g_mlp.v, (void*)g_mlp.grad, (void*)g_mlp.W1t,
                     (void*)g_mlp.W2t, (void*)g_mlp.W3t, (void*)g_mlp.W2n, (void*)g_mlp.W3n, (void*)g_mlp.W2p,
-                    (void*)g_mlp.W3p, (void*)g_mlp.DX, (void*)g_mlp.DT})
+                    (void*)g_mlp.W3p, (void*)g_mlp.DX, (void*)g_mlp.DT, (void*)g_mlp.node})
         dev_free(p);
     g_mlp = Mlp{};
 }
