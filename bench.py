@@ -100,27 +100,43 @@ def near_leg(api, ev: HipEvents, side: int, kick_ms: float, idx, J, reps: int, n
     return out
 
 
-def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: int) -> dict:
+def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: int, method: str) -> dict:
     """The stepping configuration: side^3 particles from a Gaussian random
-    field, 512^3 grid, KDK leapfrog a = 0.02 -> 0.03 with FP64 factors, force
-    = TSC PM + near field, the periodic FP64 direct sum judging `judge`
-    particles each step.  The direct kick at this N is not run (N^2 / rate at
-    the measured rate would take minutes per step); its time is derived."""
+    field, 512^3 grid, KDK leapfrog a = 0.02 -> 0.03 with FP64 factors, the
+    periodic FP64 direct sum judging `judge` particles each step.  method
+    "pm_mlp": CIC PM (split 1.25 cells) + the learned correction trained on
+    this configuration's distribution (the force the configuration names);
+    "pm_near": TSC PM (split 2.5 cells) + the near field to 5.5 split radii.
+    The direct kick at this N is not run (N^2 / rate at the measured rate
+    would take minutes per step); its time is derived."""
     from ct2_amd import stepping
-    rec = stepping.run(side, 2 * side, steps=steps, judge=judge, method="pm", timer=ev.time,
-                       log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=2.5, near=5.5, order=2)
+    from ct2_amd.train_mlp import load_weights
+    kw = dict(steps=steps, judge=judge, timer=ev.time, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if method == "pm_mlp":
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ct2_amd", "weights", "mlp_grf_ng2x.npz")
+        trained = load_weights(path)
+        if trained is None:
+            return {"method": method, "skipped": f"no trained weights at {path}"}
+        rec = stepping.run(side, 2 * side, method="pm_mlp", weights=trained[0], rs_cells=1.25, near=0.0, order=1,
+                           **kw)
+    else:
+        rec = stepping.run(side, 2 * side, method="pm", rs_cells=2.5, near=5.5, order=2, **kw)
     n = side ** 3
     ms = [r["force_kick_ms"] for r in rec["records"]][1:]  # the first call includes one-time set-up
     med = [r["median_rel_err"] for r in rec["records"]]
     direct_ms = n * float(n) / rate * 1e3
-    return {"n": n, "pm_grid": 2 * side, "assignment": "tsc", "rs_cells": 2.5, "rcut_rs": 5.5, "steps": steps,
-            "a": [rec["a_start"], rec["a_end"]], "force_kick_ms_median": round(float(np.median(ms)), 2),
-            "drift_ms_median": round(float(np.median([r["drift_ms"] for r in rec["records"][:-1]])), 3),
-            "median_rel_err_per_step": [float(f"{v:.3e}") for v in med],
-            "p90_rel_err_max": float(max(r["p90_rel_err"] for r in rec["records"])),
-            "judge_sinks_per_step": judge, "direct_kick_ms_derived": round(direct_ms, 1),
-            "speedup_vs_direct": round(direct_ms / float(np.median(ms)), 1),
-            "target_median_1e-3_and_10x_met": bool(max(med) <= 1e-3 and direct_ms / float(np.median(ms)) >= 10)}
+    out = {"method": method, "n": n, "pm_grid": 2 * side, "assignment": rec["assignment"],
+           "rs_cells": rec["rs_cells"], "rcut_rs": rec["near_rcut_rs"], "steps": steps,
+           "a": [rec["a_start"], rec["a_end"]], "force_kick_ms_median": round(float(np.median(ms)), 2),
+           "drift_ms_median": round(float(np.median([r["drift_ms"] for r in rec["records"][:-1]])), 3),
+           "median_rel_err_per_step": [float(f"{v:.3e}") for v in med],
+           "p90_rel_err_max": float(max(r["p90_rel_err"] for r in rec["records"])),
+           "judge_sinks_per_step": judge, "direct_kick_ms_derived": round(direct_ms, 1),
+           "speedup_vs_direct": round(direct_ms / float(np.median(ms)), 1),
+           "target_median_1e-3_and_10x_met": bool(max(med) <= 1e-3 and direct_ms / float(np.median(ms)) >= 10)}
+    if method == "pm_mlp":
+        out["pm_alone_median_rel_err_per_step"] = [float(f"{r['no_mlp_median_rel_err']:.3e}") for r in rec["records"]]
+    return out
 
 
 def cpu_baseline(x, y, z, soft, gm, target_s: float):
@@ -298,7 +314,8 @@ def main() -> int:
     steps_res = None
     if args.stepping_side:
         print("[bench] stepping configuration ...", file=sys.stderr, flush=True)
-        steps_res = stepping_leg(api, ev, inter / (kick_avg * 1e-3), args.stepping_side, args.stepping_steps, 16384)
+        steps_res = [stepping_leg(api, ev, inter / (kick_avg * 1e-3), args.stepping_side, args.stepping_steps, 16384,
+                                  method) for method in ("pm_near", "pm_mlp")]
     pmres["direct_fp32_vs_fp64_min_image"] = {"median_rel_err": float(np.median(e_direct)),
                                               "p90": float(np.percentile(e_direct, 90)),
                                               "p99": float(np.percentile(e_direct, 99))}
