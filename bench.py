@@ -139,6 +139,33 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
     return out
 
 
+def fft_leg(api, ev: HipEvents, reps: int) -> dict:
+    """FP32 3-D FFT: 256^3 round trip (96 n^3 B) and 512^3 forward (48 n^3 B)
+    on device-resident random data, median of `reps` HIP-event timings."""
+    fft = {}
+    for nf, mode in ((256, "round_trip"), (512, "forward")):
+        buf = api.DeviceBuffer(nf ** 3 * 8)
+        rng = np.random.default_rng(nf)
+        buf.upload((rng.standard_normal(2 * nf ** 3, dtype=np.float32)).view(np.complex64))
+        if mode == "round_trip":
+            def op():
+                api.fft3d_device(buf, nf, 1)
+                api.fft3d_device(buf, nf, -1)
+            alg = 96 * nf ** 3
+        else:
+            def op():
+                api.fft3d_device(buf, nf, 1)
+            alg = 48 * nf ** 3
+        op()
+        ms = float(np.median(time_calls(ev, op, reps)))
+        fft[f"{nf}^3_{mode}"] = {"ms": round(ms, 4), "GB/s": round(alg / ms / 1e6, 1),
+                                 "frac_hbm": round(alg / ms / 1e6 / HBM_PEAK_GBS, 4),
+                                 "compulsory_GB/s": round(16 * nf ** 3 * (2 if mode == "round_trip" else 1) / ms / 1e6, 1)}
+        buf.close()
+        print(f"[bench] fft {nf}^3 {mode}: {ms:.3f} ms", file=sys.stderr, flush=True)
+    return fft
+
+
 def cpu_baseline(x, y, z, soft, gm, target_s: float):
     """FP64 oracle kick timed on the host cores on a bounded sample of sinks
     (every source), scaled to interactions/s."""
@@ -215,6 +242,9 @@ def main() -> int:
     ap.add_argument("--stepping-steps", type=int, default=10)
     ap.add_argument("--kick-only", action="store_true",
                     help="time the kick alone (for counter runs under rocprofv3 --pmc)")
+    ap.add_argument("--only", choices=["fft", "near"], default=None,
+                    help="time one leg alone (for counter runs under rocprofv3 --pmc): the FP32 FFTs, or the "
+                         "PM + near-field force (TSC, grid 2 side, split 1.5 cells, cutoff 5.5 split radii)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -244,6 +274,24 @@ def main() -> int:
     api.particles_upload(x, y, z, np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32),
                          np.zeros(n, np.int8))
     api.set_force_options(soft, gm, api.KERNEL_PLUMMER)
+    if args.only == "fft":
+        line = {"leg": "fft", "fft_fp32": fft_leg(api, ev, args.fft_reps)}
+        api.finalize()
+        print(json.dumps(line), flush=True)
+        return 0
+    if args.only == "near":
+        from ct2_amd import _lib
+        lib = _lib.load()
+        api.pm_assignment(2)
+        api.pm_init(2 * side, 1.5)
+        api.pm_near(5.5)
+        _lib.check(lib.ct2_pm_forces())
+        ms = [float(v) for v in time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), args.reps)]
+        line = {"leg": "pm_near", "n": n, "pm_grid": 2 * side, "rs_cells": 1.5, "rcut_rs": 5.5,
+                "cells_per_side": api.pm_near_cells(), "ms": ms}
+        api.finalize()
+        print(json.dumps(line), flush=True)
+        return 0
     # top-level kick of kick_step (main.cpp:50-68): ascending, descending, top
     params = api.kick_params(ascending=True, descending=True, top=True, first=False, rung=0, t0=1e-3,
                              eta=0.15, max_dt=1e-3, scale=1.0)
@@ -273,28 +321,7 @@ def main() -> int:
     drift_ms = time_calls(ev, lambda: api.drift(1e-9, 1.0), 20)
     drift_avg = float(np.median(drift_ms))
 
-    # FFT: 256^3 round trip (96 n^3 B) and 512^3 forward (48 n^3 B), FP32
-    fft = {}
-    for nf, mode in ((256, "round_trip"), (512, "forward")):
-        buf = api.DeviceBuffer(nf ** 3 * 8)
-        rng = np.random.default_rng(nf)
-        buf.upload((rng.standard_normal(2 * nf ** 3, dtype=np.float32)).view(np.complex64))
-        if mode == "round_trip":
-            def op():
-                api.fft3d_device(buf, nf, 1)
-                api.fft3d_device(buf, nf, -1)
-            alg = 96 * nf ** 3
-        else:
-            def op():
-                api.fft3d_device(buf, nf, 1)
-            alg = 48 * nf ** 3
-        op()
-        ms = float(np.median(time_calls(ev, op, args.fft_reps)))
-        fft[f"{nf}^3_{mode}"] = {"ms": round(ms, 4), "GB/s": round(alg / ms / 1e6, 1),
-                                 "frac_hbm": round(alg / ms / 1e6 / HBM_PEAK_GBS, 4),
-                                 "compulsory_GB/s": round(16 * nf ** 3 * (2 if mode == "round_trip" else 1) / ms / 1e6, 1)}
-        buf.close()
-        print(f"[bench] fft {nf}^3 {mode}: {ms:.3f} ms", file=sys.stderr, flush=True)
+    fft = fft_leg(api, ev, args.fft_reps)
 
     # force accuracy of the direct kick against the FP64 direct sum (same
     # minimum-image force) on sampled sinks
