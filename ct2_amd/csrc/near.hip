@@ -49,7 +49,6 @@ constexpr int NEAR_MAX_NC = 64;  // cells per side at most
 // over [0, rcut^2): within 2e-8 (relative) of the closed form, below FP32
 // rounding (built on the host in FP64 by near_kernel_table).
 constexpr int NEAR_NT = 64;
-static_assert(NEAR_NT <= NEAR_BLOCK, "one table entry per thread is loaded into LDS");
 constexpr int SCAN_THREADS = 1024;
 
 struct NearState {
@@ -66,7 +65,7 @@ struct NearState {
     int* row_chunk = nullptr;      // [nc^2 + 1]
     int2* item = nullptr;          // [cap / NEAR_BLOCK + nc^2] (row, chunk) per work item
     size_t item_cap = 0;
-    float4* table = nullptr;       // [2 * NEAR_NT]: k_l pieces, then p_l pieces
+    float4* table = nullptr;       // [2 (NEAR_NT + 1)]: k_l pieces, then p_l pieces
     double table_rsu = 0.0, table_rcu = 0.0;  // what the table was built for
 } g_near;
 
@@ -200,9 +199,11 @@ __device__ __forceinline__ void near_pair(float r2, const NearConst& q, const fl
             ps = w;
         }
     }
-    const float t = r2 * q.tscale;  // in [0, NEAR_NT) inside the cutoff
-    const int i = min((int)t, NEAR_NT - 1);
-    const float f = t - (float)i;
+    // t in [0, NEAR_NT] inside the cutoff (NEAR_NT itself only by rounding: the
+    // table's extra piece NEAR_NT is the constant end value of the last one)
+    const float t = r2 * q.tscale;
+    const int i = (int)t;
+    const float f = __builtin_amdgcn_fractf(t);
     const float4 a = tk[i], b = tp[i];
     const float kl = fmaf(fmaf(fmaf(a.w, f, a.z), f, a.y), f, a.x);
     const float pl = fmaf(fmaf(fmaf(b.w, f, b.z), f, b.y), f, b.x);
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
                                                          float* __restrict__ fy, float* __restrict__ fz,
                                                          float* __restrict__ pot) {
     __shared__ uint4 tile[NEAR_BLOCK];
-    __shared__ float4 lk[NEAR_NT], lp[NEAR_NT];  // the kernel's cubic pieces
+    __shared__ float4 lk[NEAR_NT + 1], lp[NEAR_NT + 1];  // the kernel's cubic pieces
     __shared__ int rng[18][2];
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
@@ -266,9 +267,9 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
         rng[2 * tid + 1][0] = a1;
         rng[2 * tid + 1][1] = e1;
     }
-    if (tid < NEAR_NT) {
-        lk[tid] = table[tid];
-        lp[tid] = table[NEAR_NT + tid];
+    for (int e = tid; e <= NEAR_NT; e += NEAR_BLOCK) {
+        lk[e] = table[e];
+        lp[e] = table[NEAR_NT + 1 + e];
     }
     __syncthreads();
     const int i = s0 + tid;
@@ -297,8 +298,16 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
                 }
             };
             if (cnt == NEAR_BLOCK) {
-#pragma unroll 4
-                for (int j = 0; j < NEAR_BLOCK; ++j) pair(tile[j]);
+                // four sources read together: the in-range branch inside pair()
+                // would otherwise keep each read (and its wait) next to its use
+#pragma unroll 2
+                for (int j = 0; j < NEAR_BLOCK; j += 4) {
+                    const uint4 s0 = tile[j], s1 = tile[j + 1], s2 = tile[j + 2], s3 = tile[j + 3];
+                    pair(s0);
+                    pair(s1);
+                    pair(s2);
+                    pair(s3);
+                }
             } else {
                 for (int j = 0; j < cnt; ++j) pair(tile[j]);
             }
@@ -480,10 +489,17 @@ int near_forces(double rs) {
     q.hinvu = (float)(1.0 / hu);
     q.hinv3u = (float)(1.0 / (hu * hu * hu));
     q.tscale = (float)(NEAR_NT / (rcu * rcu));
-    if (!g_near.table) CT2_TRY(dev_alloc((void**)&g_near.table, 2 * NEAR_NT * sizeof(float4), "near-field table"));
+    if (!g_near.table)
+        CT2_TRY(dev_alloc((void**)&g_near.table, 2 * (NEAR_NT + 1) * sizeof(float4), "near-field table"));
     if (g_near.table_rsu != rsu || g_near.table_rcu != rcu) {
-        std::vector<float> t(8 * NEAR_NT);
-        near_kernel_table(rsu, rcu, NEAR_NT, t.data(), t.data() + 4 * NEAR_NT);
+        std::vector<float> t(8 * (NEAR_NT + 1), 0.0f);
+        float* tk = t.data();
+        float* tp = t.data() + 4 * (NEAR_NT + 1);
+        near_kernel_table(rsu, rcu, NEAR_NT, tk, tp);
+        for (float* c : {tk, tp}) {  // piece NEAR_NT: the last piece's value at f = 1
+            const float* last = c + 4 * (NEAR_NT - 1);
+            c[4 * NEAR_NT] = ((last[0] + last[1]) + last[2]) + last[3];
+        }
         CT2_HIP(hipMemcpyAsync(g_near.table, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, st));
         CT2_HIP(hipStreamSynchronize(st));  // t is released on return
         g_near.table_rsu = rsu;
