@@ -18,16 +18,17 @@
 //   near_scan     exclusive scan of the counts -> cell starts
 //   near_rows     work items: chunks of NEAR_BLOCK sinks within one (x, y) row
 //   near_scatter  (x, y, z, index) into its cell's segment (slot order)
-//   near_order    each cell's segment sorted by particle index (rank by
-//                 counting), so the summation order never depends on atomics
+//   near_order    each cell's segment sorted by (z slice, particle index)
+//                 (rank by counting), so the summation order never depends on
+//                 atomics, and the start of each of the cell's NEAR_SUB slices
 //   near_force    one workgroup per work item: sinks in registers, the 9
 //                 neighbouring rows' cell ranges streamed through LDS tiles,
 //                 FP32 pair terms with tile-level TwoSum; adds into the
 //                 stored force of each particle.
 // Cells have side >= rcut, so every partner of a sink lies in the 3x3x3 cells
-// around it; a work item's sinks span a run of cells along z in one row, and
-// the union of their neighbourhoods is 9 contiguous (or wrapped) cell ranges
-// of the sorted array.
+// around it; a work item's sinks span a run of slices along z in one row, and
+// the union of their neighbourhoods is 9 contiguous (or wrapped) slice ranges
+// of the sorted array, each widened by the slices rcut spans.
 #include <cmath>
 #include <utility>
 #include <vector>
@@ -45,6 +46,13 @@ namespace {
 // sinks span little of their row and the wave skips distant sources
 constexpr int NEAR_BLOCK = NEAR_BLOCK_SIZE;
 constexpr int NEAR_MAX_NC = 64;  // cells per side at most
+// Each cell is cut into NEAR_SUB slices along z; a cell's segment is ordered
+// by (slice, particle index), so a row's particles are in slice order and a
+// work item's neighbour ranges start and end at slice boundaries.
+#ifndef NEAR_SUB_N
+#define NEAR_SUB_N 16
+#endif
+constexpr int NEAR_SUB = NEAR_SUB_N;
 // The filtered Newtonian kernel is evaluated from NEAR_NT cubic pieces in r^2
 // over [0, rcut^2): within 2e-8 (relative) of the closed form, below FP32
 // rounding (built on the host in FP64 by near_kernel_table).
@@ -63,6 +71,7 @@ struct NearState {
     int* count = nullptr;          // [nc^3]
     int* start = nullptr;          // [nc^3 + 1]
     int* row_chunk = nullptr;      // [nc^2 + 1]
+    int* subst = nullptr;          // [nc^3 NEAR_SUB + 1] sorted index of each slice's first particle
     int2* item = nullptr;          // [cap / NEAR_BLOCK + nc^2] (row, chunk) per work item
     size_t item_cap = 0;
     float4* table = nullptr;       // [2 (NEAR_NT + 1)]: k_l pieces, then p_l pieces
@@ -140,31 +149,54 @@ __global__ __launch_bounds__(256) void near_scatter(const uint32_t* __restrict__
     P0[start[cell[p]] + slot[p]] = make_uint4(X[p], Y[p], Z[p], (uint32_t)p);
 }
 
-// One workgroup per cell: element e goes to rank = #{f in cell : index_f < index_e}.
+// z slice of a particle along its row: u = floor(z nzs / 2^32), nzs = nc NEAR_SUB
+// slices per row; u / NEAR_SUB is the cell along z.
+__device__ __forceinline__ uint32_t slice_of(uint32_t z, uint32_t nzs) {
+    return (uint32_t)(((uint64_t)z * nzs) >> 32);
+}
+
+// One workgroup per cell: element e goes to rank = #{f in cell : key_f < key_e},
+// key = (slice, particle index); then the cell's slice starts.
 __global__ __launch_bounds__(256) void near_order(const uint4* __restrict__ P0, const int* __restrict__ start,
-                                                  uint4* __restrict__ P) {
-    __shared__ uint32_t key[1024];
+                                                  uint32_t nzs, int ncell, uint4* __restrict__ P,
+                                                  int* __restrict__ subst) {
+    __shared__ uint64_t key[1024];
+    __shared__ int cnt[NEAR_SUB];
     const int c = blockIdx.x;
     const int base = start[c], m = start[c + 1] - base;
-    if (m == 0) return;
-    if (m == 1) {
-        if (threadIdx.x == 0) P[base] = P0[base];
-        return;
-    }
+    if (threadIdx.x < NEAR_SUB) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    auto key_of = [&](const uint4 v) -> uint64_t {
+        return ((uint64_t)(slice_of(v.z, nzs) % NEAR_SUB) << 32) | v.w;
+    };
     for (int e0 = 0; e0 < m; e0 += 256) {
         const int e = e0 + threadIdx.x;
         uint4 mine = make_uint4(0u, 0u, 0u, 0u);
-        if (e < m) mine = P0[base + e];
+        uint64_t mk = 0;
+        if (e < m) {
+            mine = P0[base + e];
+            mk = key_of(mine);
+            atomicAdd(&cnt[mk >> 32], 1);
+        }
         int rank = 0;
         for (int f0 = 0; f0 < m; f0 += 1024) {
-            const int cnt = min(1024, m - f0);
+            const int cntf = min(1024, m - f0);
             __syncthreads();
-            for (int f = threadIdx.x; f < cnt; f += 256) key[f] = P0[base + f0 + f].w;
+            for (int f = threadIdx.x; f < cntf; f += 256) key[f] = key_of(P0[base + f0 + f]);
             __syncthreads();
             if (e < m)
-                for (int f = 0; f < cnt; ++f) rank += key[f] < mine.w;
+                for (int f = 0; f < cntf; ++f) rank += key[f] < mk;
         }
         if (e < m) P[base + rank] = mine;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = base;
+        for (int q = 0; q < NEAR_SUB; ++q) {
+            subst[c * NEAR_SUB + q] = run;
+            run += cnt[q];
+        }
+        if (c == ncell - 1) subst[ncell * NEAR_SUB] = run;
     }
 }
 
@@ -220,8 +252,9 @@ __global__ __launch_bounds__(256) void near_items(const int* __restrict__ row_ch
 
 template <int KERNEL>
 __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict__ P, const int* __restrict__ start,
+                                                         const int* __restrict__ subst,
                                                          const int* __restrict__ row_chunk,
-                                                         const int2* __restrict__ item, int nc, NearConst q,
+                                                         const int2* __restrict__ item, int nc, int R, NearConst q,
                                                          const float4* __restrict__ table, double fscale,
                                                          double pscale, double pconst, float* __restrict__ fx,
                                                          float* __restrict__ fy, float* __restrict__ fz,
@@ -238,29 +271,29 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
     const int s1 = min(start[(row + 1) * nc], s0 + NEAR_BLOCK);
     const int cx = row / nc, cy = row % nc;
     if (tid < 9) {  // one neighbouring row per lane: at most two index ranges (z wraps)
-        const int kz0 = (int)cell_of(P[s0].z, (uint32_t)nc), kz1 = (int)cell_of(P[s1 - 1].z, (uint32_t)nc);
+        // the sinks' slices (a row is in slice order) widened by R slices, the
+        // most a partner within rcut can differ
+        const int Z = nc * NEAR_SUB;
+        const int lo = (int)slice_of(P[s0].z, (uint32_t)Z) - R, hi = (int)slice_of(P[s1 - 1].z, (uint32_t)Z) + R;
         const int di = tid / 3 - 1, dj = tid % 3 - 1;
-        const int c0 = (((cx + di + nc) % nc) * nc + (cy + dj + nc) % nc) * nc;
+        const int* sr = subst + (((cx + di + nc) % nc) * nc + (cy + dj + nc) % nc) * Z;  // slice starts of the row
         int a0, e0, a1 = 0, e1 = 0;
-        if (kz1 - kz0 + 3 >= nc) {  // the run covers the whole row
-            a0 = start[c0];
-            e0 = start[c0 + nc];
+        if (hi - lo + 1 >= Z) {  // the whole row
+            a0 = sr[0];
+            e0 = sr[Z];
+        } else if (lo < 0) {
+            a0 = sr[lo + Z];
+            e0 = sr[Z];
+            a1 = sr[0];
+            e1 = sr[hi + 1];
+        } else if (hi >= Z) {
+            a0 = sr[lo];
+            e0 = sr[Z];
+            a1 = sr[0];
+            e1 = sr[hi - Z + 1];
         } else {
-            const int lo = kz0 - 1, hi = kz1 + 1;  // inclusive cell range along z
-            if (lo < 0) {
-                a0 = start[c0 + lo + nc];
-                e0 = start[c0 + nc];
-                a1 = start[c0];
-                e1 = start[c0 + hi + 1];
-            } else if (hi >= nc) {
-                a0 = start[c0 + lo];
-                e0 = start[c0 + nc];
-                a1 = start[c0];
-                e1 = start[c0 + hi - nc + 1];
-            } else {
-                a0 = start[c0 + lo];
-                e0 = start[c0 + hi + 1];
-            }
+            a0 = sr[lo];
+            e0 = sr[hi + 1];
         }
         rng[2 * tid][0] = a0;
         rng[2 * tid][1] = e0;
@@ -334,6 +367,8 @@ void free_near_buffers() {
     dev_free(g_near.count);
     dev_free(g_near.start);
     dev_free(g_near.row_chunk);
+    dev_free(g_near.subst);
+    g_near.subst = nullptr;
     dev_free(g_near.item);
     dev_free(g_near.table);
     g_near.table = nullptr;
@@ -366,12 +401,14 @@ int ensure_buffers(size_t n, int nc) {
         dev_free(g_near.count);
         dev_free(g_near.start);
         dev_free(g_near.row_chunk);
-        g_near.count = g_near.start = g_near.row_chunk = nullptr;
+        dev_free(g_near.subst);
+        g_near.count = g_near.start = g_near.row_chunk = g_near.subst = nullptr;
         g_near.ncell_cap = 0;
         const size_t c3 = (size_t)nc * nc * nc;
         CT2_TRY(dev_alloc((void**)&g_near.count, c3 * sizeof(int), "near-field counts"));
         CT2_TRY(dev_alloc((void**)&g_near.start, (c3 + 1) * sizeof(int), "near-field cell starts"));
         CT2_TRY(dev_alloc((void**)&g_near.row_chunk, ((size_t)nc * nc + 1) * sizeof(int), "near-field work items"));
+        CT2_TRY(dev_alloc((void**)&g_near.subst, (c3 * NEAR_SUB + 1) * sizeof(int), "near-field slice starts"));
         g_near.ncell_cap = nc;
     }
     const size_t items = n / NEAR_BLOCK + 1 + (size_t)nc * nc;
@@ -477,7 +514,7 @@ int near_forces(double rs) {
     CT2_LAUNCH_CHECK();
     near_scatter<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, g_near.cell, g_near.slot, g_near.start, g_near.P0);
     CT2_LAUNCH_CHECK();
-    near_order<<<c3, 256, 0, st>>>(g_near.P0, g_near.start, g_near.P);
+    near_order<<<c3, 256, 0, st>>>(g_near.P0, g_near.start, (uint32_t)(nc * NEAR_SUB), c3, g_near.P, g_near.subst);
     CT2_LAUNCH_CHECK();
 
     const double u = 0x1p32;
@@ -510,16 +547,20 @@ int near_forces(double rs) {
     const double pconst = gm * (double)n * 4.0 * M_PI * rs * rs;
     near_items<<<(nrow + 255) / 256, 256, 0, st>>>(g_near.row_chunk, nrow, g_near.item);
     CT2_LAUNCH_CHECK();
+    // slices a partner within rcut can be away along z (rcut <= 1 / nc: R <= NEAR_SUB)
+    const int R = std::min(NEAR_SUB, (int)std::ceil(rcut * nc * NEAR_SUB));
     // work items: at most one partial chunk per row beyond n / NEAR_BLOCK full ones
     const unsigned items = (unsigned)((n + NEAR_BLOCK - 1) / NEAR_BLOCK + nrow);
     if (c.opts.kernel == CT2_KERNEL_PLUMMER)
-        near_force<CT2_KERNEL_PLUMMER><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk,
-                                                                     g_near.item, nc, q, g_near.table, fscale, pscale,
-                                                                     pconst, c.fx, c.fy, c.fz, c.pot);
+        near_force<CT2_KERNEL_PLUMMER><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.subst,
+                                                                     g_near.row_chunk, g_near.item, nc, R, q,
+                                                                     g_near.table, fscale, pscale, pconst, c.fx,
+                                                                     c.fy, c.fz, c.pot);
     else
-        near_force<CT2_KERNEL_CT2><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.row_chunk, g_near.item,
-                                                                 nc, q, g_near.table, fscale, pscale, pconst, c.fx,
-                                                                 c.fy, c.fz, c.pot);
+        near_force<CT2_KERNEL_CT2><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.subst,
+                                                                 g_near.row_chunk, g_near.item, nc, R, q,
+                                                                 g_near.table, fscale, pscale, pconst, c.fx, c.fy,
+                                                                 c.fz, c.pot);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
