@@ -153,10 +153,12 @@ struct Lanes {
 
 // Shared kernel body.  Element i of batch line c lives at gmem(i) in HBM
 // and at lds(i) in LDS.
-template <typename T, int N, bool ROWS, int DIR, int S, int K = 1>
+// U64: the first stage reads the real field W * 2^-32 from the uint64 array
+// wsrc (same indexing) instead of complex data (the PM density transform).
+template <typename T, int N, bool ROWS, int DIR, int S, int K = 1, bool U64 = false>
 __device__ __forceinline__ void run_stages(typename V2<T>::t* __restrict__ data, typename V2<T>::t* sm,
                                            const typename V2<T>::t* tw, long base, long stride_i, int p,
-                                           int c) {
+                                           int c, const unsigned long long* __restrict__ wsrc = nullptr) {
     using T2 = typename V2<T>::t;
     using CF = Cfg<T, N, K>;
     constexpr int NS = nstages_of(N);
@@ -179,7 +181,9 @@ __device__ __forceinline__ void run_stages(typename V2<T>::t* __restrict__ data,
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = b + r * L;
-                if constexpr (FIRST)
+                if constexpr (FIRST && U64)
+                    v[u * R + r] = T2{(T)((double)wsrc[base + (long)i * stride_i] * 0x1p-32), (T)0};
+                else if constexpr (FIRST)
                     v[u * R + r] = data[base + (long)i * stride_i];
                 else
                     v[u * R + r] = sm[lds(i)];
@@ -209,7 +213,7 @@ __device__ __forceinline__ void run_stages(typename V2<T>::t* __restrict__ data,
                 for (int r = 0; r < R; ++r) sm[lds(o + r * Ns)] = v[u * R + r];
             }
             __syncthreads();
-            run_stages<T, N, ROWS, DIR, S + 1, K>(data, sm, tw, base, stride_i, p, c);
+            run_stages<T, N, ROWS, DIR, S + 1, K, U64>(data, sm, tw, base, stride_i, p, c, wsrc);
         }
     }
 }
@@ -224,10 +228,14 @@ __device__ __forceinline__ void load_twiddles(typename V2<T>::t* tw, const typen
     }
 }
 
-// z pass: lines of N contiguous elements; workgroup = ROWS lines.
-template <typename T, int N, int DIR>
+// z pass: lines of N contiguous elements; workgroup = ROWS lines.  With U64
+// the lines are read from the uint64 field wsrc (real, scaled by 2^-32) and
+// the transform is written to data.
+template <typename T, int N, int DIR, bool U64 = false>
 __global__ __launch_bounds__((Lanes<true, T, N>::value)) void fft_rows(typename V2<T>::t* __restrict__ data,
-                                                                     const typename V2<T>::t* __restrict__ twg) {
+                                                                     const typename V2<T>::t* __restrict__ twg,
+                                                                     const unsigned long long* __restrict__ wsrc =
+                                                                         nullptr) {
     using T2 = typename V2<T>::t;
     using CF = Cfg<T, N>;
     __shared__ T2 sm[CF::ROWS * CF::ROWLEN];
@@ -237,7 +245,7 @@ __global__ __launch_bounds__((Lanes<true, T, N>::value)) void fft_rows(typename 
     const int c = threadIdx.x / CF::P;
     const long base = ((long)blockIdx.x * CF::ROWS + c) * N;
     if constexpr (nstages_of(N) == 1) __syncthreads();
-    run_stages<T, N, true, DIR, 0>(data, sm, tw, base, 1, p, c);
+    run_stages<T, N, true, DIR, 0, 1, U64>(data, sm, tw, base, 1, p, c, wsrc);
 }
 
 // y / x pass: COLS adjacent columns (unit stride) of lines with stride
@@ -293,13 +301,16 @@ int twiddle_table(int N, const typename V2<T>::t** out) {
 }
 
 template <typename T, int N, int DIR>
-int fft3d_launch(typename V2<T>::t* d, hipStream_t st) {
+int fft3d_launch(typename V2<T>::t* d, hipStream_t st, const unsigned long long* wsrc = nullptr) {
     using CF = Cfg<T, N>;
     const typename V2<T>::t* tw = nullptr;
     CT2_TRY(twiddle_table<T>(N, &tw));
     const long n = N, n2 = n * n;
     // z
-    fft_rows<T, N, DIR><<<(unsigned)(n2 / CF::ROWS), Lanes<true, T, N>::value, 0, st>>>(d, tw);
+    if (wsrc)
+        fft_rows<T, N, DIR, true><<<(unsigned)(n2 / CF::ROWS), Lanes<true, T, N>::value, 0, st>>>(d, tw, wsrc);
+    else
+        fft_rows<T, N, DIR><<<(unsigned)(n2 / CF::ROWS), Lanes<true, T, N>::value, 0, st>>>(d, tw);
     CT2_LAUNCH_CHECK();
     const int inner = N / Cfg<T, N, col_k<T, N>()>::COLS;
     // y: columns z, slabs x
@@ -331,6 +342,26 @@ int fft3d_dispatch(void* data, int n, hipStream_t st) {
 }
 
 }  // namespace
+
+// Forward FP32 transform of the real field W * 2^-32 (uint64 fixed point, n^3)
+// into the complex grid out: the density transform of the PM force.
+int fft3d_density(const unsigned long long* W, void* out, int n) {
+    CT2_TRY(require_init());
+    if (!W || !out) return set_error(CT2_ERR_INVALID, "null FFT buffer");
+    hipStream_t st = ctx().stream;
+    float2* d = (float2*)out;
+    switch (n) {
+        case 8: return fft3d_launch<float, 8, 1>(d, st, W);
+        case 16: return fft3d_launch<float, 16, 1>(d, st, W);
+        case 32: return fft3d_launch<float, 32, 1>(d, st, W);
+        case 64: return fft3d_launch<float, 64, 1>(d, st, W);
+        case 128: return fft3d_launch<float, 128, 1>(d, st, W);
+        case 256: return fft3d_launch<float, 256, 1>(d, st, W);
+        case 512: return fft3d_launch<float, 512, 1>(d, st, W);
+        case 1024: return fft3d_launch<float, 1024, 1>(d, st, W);
+        default: return set_error(CT2_ERR_INVALID, "unsupported FFT size %d", n);
+    }
+}
 
 int fft3d(void* data, int n, int dir, bool f64) {
     CT2_TRY(require_init());

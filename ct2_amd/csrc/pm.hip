@@ -9,8 +9,8 @@
 //                 tile, each tile summed in LDS, then one global atomic per
 //                 touched node (dep_*); smaller grids: one global atomic per
 //                 particle and stencil node (mesh_deposit)
-//   to_complex    density grid -> FP32 complex grid
-//   fft3d         forward FP32 transform (fft.hip)
+//   fft3d_density forward FP32 transform of the density (fft.hip; its first
+//                 pass reads the uint64 grid directly)
 //   green_mul     rho_hat * G (FP64 Green's table, octant-symmetric), then
 //                 the spectral gradient; two real fields share one complex
 //                 grid: A = Fx + i Fy, B = Fz + i pot
@@ -233,12 +233,6 @@ __global__ __launch_bounds__(DEP_THREADS) void dep_tile(const uint32_t* __restri
     }
 }
 
-__global__ __launch_bounds__(PM_BLOCK) void to_complex(const unsigned long long* __restrict__ W,
-                                                       float2* __restrict__ A, long n3) {
-    const long c = (long)blockIdx.x * PM_BLOCK + threadIdx.x;
-    if (c < n3) A[c] = make_float2((float)((double)W[c] * 0x1p-32), 0.0f);
-}
-
 // One thread per k-cell.  phi_hat = G rho_hat (FP64); F_hat = -i k phi_hat
 // with the Nyquist component of k zeroed; pot_hat = -phi_hat.
 __global__ __launch_bounds__(PM_BLOCK) void green_mul(float2* __restrict__ A, float2* __restrict__ B,
@@ -409,9 +403,7 @@ int pm_forces() {
     const long n3 = 1L << (3 * g);
     const unsigned gb = (unsigned)((n3 + PM_BLOCK - 1) / PM_BLOCK), pb = (unsigned)((n + PM_BLOCK - 1) / PM_BLOCK);
     CT2_TRY(g_assign == 1 ? deposit<1>(c) : deposit<2>(c));
-    to_complex<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.W, g_pm.A, n3);
-    CT2_LAUNCH_CHECK();
-    CT2_TRY(fft3d(g_pm.A, g_pm.ng, 1, false));
+    CT2_TRY(fft3d_density(g_pm.W, g_pm.A, g_pm.ng));  // density (real) -> rho_hat
     green_mul<<<gb, PM_BLOCK, 0, c.stream>>>(g_pm.A, g_pm.B, g_pm.G, g);
     CT2_LAUNCH_CHECK();
     CT2_TRY(fft3d(g_pm.A, g_pm.ng, -1, false));
