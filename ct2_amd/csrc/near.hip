@@ -156,47 +156,75 @@ __device__ __forceinline__ uint32_t slice_of(uint32_t z, uint32_t nzs) {
 }
 
 // One workgroup per cell: element e goes to rank = #{f in cell : key_f < key_e},
-// key = (slice, particle index); then the cell's slice starts.
+// key = (slice, particle index); then the cell's slice starts.  Cells of up to
+// ORDER_LDS particles are bucketed by slice in LDS first, so an element is
+// ranked only against its own slice; larger cells compare whole keys.
+constexpr int ORDER_LDS = 4096;
 __global__ __launch_bounds__(256) void near_order(const uint4* __restrict__ P0, const int* __restrict__ start,
                                                   uint32_t nzs, int ncell, uint4* __restrict__ P,
                                                   int* __restrict__ subst) {
-    __shared__ uint64_t key[1024];
-    __shared__ int cnt[NEAR_SUB];
+    __shared__ uint32_t bucket[ORDER_LDS];  // particle indices grouped by slice (large cells: keys' low words)
+    __shared__ uint32_t khi[1024];          // large cells: keys' slices
+    __shared__ int cnt[NEAR_SUB], off[NEAR_SUB], fill[NEAR_SUB];
     const int c = blockIdx.x;
     const int base = start[c], m = start[c + 1] - base;
-    if (threadIdx.x < NEAR_SUB) cnt[threadIdx.x] = 0;
+    const int tid = threadIdx.x;
+    if (tid < NEAR_SUB) {
+        cnt[tid] = 0;
+        fill[tid] = 0;
+    }
     __syncthreads();
-    auto key_of = [&](const uint4 v) -> uint64_t {
-        return ((uint64_t)(slice_of(v.z, nzs) % NEAR_SUB) << 32) | v.w;
-    };
+    for (int e = tid; e < m; e += 256) atomicAdd(&cnt[slice_of(P0[base + e].z, nzs) % NEAR_SUB], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int q = 0; q < NEAR_SUB; ++q) {
+            off[q] = run;
+            subst[c * NEAR_SUB + q] = base + run;
+            run += cnt[q];
+        }
+        if (c == ncell - 1) subst[ncell * NEAR_SUB] = base + run;
+    }
+    __syncthreads();
+    if (m <= ORDER_LDS) {
+        for (int e = tid; e < m; e += 256) {
+            const uint4 v = P0[base + e];
+            const uint32_t q = slice_of(v.z, nzs) % NEAR_SUB;
+            bucket[off[q] + atomicAdd(&fill[q], 1)] = v.w;
+        }
+        __syncthreads();
+        for (int e = tid; e < m; e += 256) {
+            const uint4 v = P0[base + e];
+            const uint32_t q = slice_of(v.z, nzs) % NEAR_SUB;
+            const int b = off[q], nb = cnt[q];
+            int rank = 0;
+            for (int f = 0; f < nb; ++f) rank += bucket[b + f] < v.w;
+            P[base + b + rank] = v;
+        }
+        return;
+    }
     for (int e0 = 0; e0 < m; e0 += 256) {
-        const int e = e0 + threadIdx.x;
+        const int e = e0 + tid;
         uint4 mine = make_uint4(0u, 0u, 0u, 0u);
-        uint64_t mk = 0;
+        uint32_t mq = 0;
         if (e < m) {
             mine = P0[base + e];
-            mk = key_of(mine);
-            atomicAdd(&cnt[mk >> 32], 1);
+            mq = slice_of(mine.z, nzs) % NEAR_SUB;
         }
         int rank = 0;
         for (int f0 = 0; f0 < m; f0 += 1024) {
             const int cntf = min(1024, m - f0);
             __syncthreads();
-            for (int f = threadIdx.x; f < cntf; f += 256) key[f] = key_of(P0[base + f0 + f]);
+            for (int f = tid; f < cntf; f += 256) {
+                const uint4 v = P0[base + f0 + f];
+                khi[f] = slice_of(v.z, nzs) % NEAR_SUB;
+                bucket[f] = v.w;
+            }
             __syncthreads();
             if (e < m)
-                for (int f = 0; f < cntf; ++f) rank += key[f] < mk;
+                for (int f = 0; f < cntf; ++f) rank += khi[f] < mq || (khi[f] == mq && bucket[f] < mine.w);
         }
         if (e < m) P[base + rank] = mine;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = base;
-        for (int q = 0; q < NEAR_SUB; ++q) {
-            subst[c * NEAR_SUB + q] = run;
-            run += cnt[q];
-        }
-        if (c == ncell - 1) subst[ncell * NEAR_SUB] = run;
     }
 }
 
