@@ -15,7 +15,10 @@
 //                 the spectral gradient; two real fields share one complex
 //                 grid: A = Fx + i Fy, B = Fz + i pot
 //   fft3d x2      inverse transforms (two instead of four)
-//   mesh_interp   force and potential at the particles (same weights)
+//   interp        force and potential at the particles (same weights): in
+//                 deposit-tile order from LDS-staged half tiles (interp_tile)
+//                 when tiles hold enough particles, else per particle
+//                 (mesh_interp)
 // Grid layout: (i*ng + j)*ng + k with i along x; node c at position c/ng.
 #include <cmath>
 #include <vector>
@@ -294,6 +297,85 @@ __global__ __launch_bounds__(PM_BLOCK) void mesh_interp(const uint32_t* __restri
     pot[p] = acc3;
 }
 
+// Interpolation in tile order, for grids with enough particles per tile: one
+// workgroup per deposit tile (16 x 16 x 32 nodes) stages the force and
+// potential of one 16-node half of the tile along z, grown by one node on
+// every side (18^3 nodes of (Fx, Fy, Fz, pot), 93 KB), in LDS and
+// interpolates the particles whose anchor node lies in that half; then the
+// other half.  The same weights and summation order as mesh_interp, so the
+// results are identical; each node is read from L2 once per tile half instead
+// of once per particle and stencil node.
+constexpr int IH = 1 << (TILE_BZ - 1);  // z nodes per half tile
+constexpr int ILX = (1 << TILE_BX) + 2, ILY = (1 << TILE_BY) + 2, ILZ = IH + 2;
+template <int ORDER>
+__global__ __launch_bounds__(DEP_THREADS) void interp_tile(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                           const uint32_t* __restrict__ Z, const int* __restrict__ order,
+                                                           const int* __restrict__ tstart, int g,
+                                                           const float2* __restrict__ A, const float2* __restrict__ B,
+                                                           float* __restrict__ fx, float* __restrict__ fy,
+                                                           float* __restrict__ fz, float* __restrict__ pot) {
+    __shared__ float4 nd[ILX * ILY * ILZ];
+    const int tile = blockIdx.x;
+    const int b = tstart[tile], e = tstart[tile + 1];
+    if (b == e) return;  // uniform across the workgroup
+    const int tid = threadIdx.x;
+    const int s = 32 - g;
+    const uint32_t mask = (1u << g) - 1u;
+    const uint32_t ox = (uint32_t)(tile >> (2 * g - TILE_BY - TILE_BZ)) << TILE_BX;
+    const uint32_t oy = (uint32_t)((tile >> (g - TILE_BZ)) & ((1 << (g - TILE_BY)) - 1)) << TILE_BY;
+    const uint32_t oz = (uint32_t)(tile & ((1 << (g - TILE_BZ)) - 1)) << TILE_BZ;
+    constexpr uint32_t SHIFT = ORDER == 1 ? 1u : 0u;
+    const float q = 0x1p-21f;
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t hz = oz + (uint32_t)(h * IH);
+        __syncthreads();  // the previous half's reads are done
+        for (int i = tid; i < ILX * ILY * ILZ; i += DEP_THREADS) {
+            const int lx = i / (ILY * ILZ), ly = (i / ILZ) % ILY, lz = i % ILZ;
+            const size_t gx = (ox + lx - 1u) & mask, gy = (oy + ly - 1u) & mask, gz = (hz + lz - 1u) & mask;
+            const size_t cc = (((gx << g) + gy) << g) + gz;
+            const float2 a = A[cc], bb = B[cc];
+            nd[i] = make_float4(a.x, a.y, bb.x, bb.y);
+        }
+        __syncthreads();
+        for (int k = b + tid; k < e; k += DEP_THREADS) {
+            const int p = order[k];
+            const uint32_t z = Z[p];
+            const uint32_t lzs = anchor_node<ORDER>(z, s) - hz;  // anchor z within this half: [0, IH)
+            if (lzs >= (uint32_t)IH) continue;
+            const uint32_t x = X[p], y = Y[p];
+            uint32_t lo;
+            uint64_t ax[ORDER + 1], ay[ORDER + 1], az[ORDER + 1];
+            assign_axis<ORDER>(x, s, lo, ax);
+            assign_axis<ORDER>(y, s, lo, ay);
+            assign_axis<ORDER>(z, s, lo, az);
+            const int lx = (int)(anchor_node<ORDER>(x, s) - ox + SHIFT);
+            const int ly = (int)(anchor_node<ORDER>(y, s) - oy + SHIFT);
+            const int lz = (int)(lzs + SHIFT);
+            float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+#pragma unroll
+            for (int dx = 0; dx <= ORDER; ++dx)
+#pragma unroll
+                for (int dy = 0; dy <= ORDER; ++dy) {
+                    const float wxy = ((float)ax[dx] * q) * ((float)ay[dy] * q);
+                    const float4* row = nd + ((lx + dx) * ILY + (ly + dy)) * ILZ + lz;
+#pragma unroll
+                    for (int dz = 0; dz <= ORDER; ++dz) {
+                        const float w = wxy * ((float)az[dz] * q);
+                        const float4 v = row[dz];
+                        acc0 = fmaf(w, v.x, acc0);
+                        acc1 = fmaf(w, v.y, acc1);
+                        acc2 = fmaf(w, v.z, acc2);
+                        acc3 = fmaf(w, v.w, acc3);
+                    }
+                }
+            fx[p] = acc0;
+            fy[p] = acc1;
+            fz[p] = acc2;
+            pot[p] = acc3;
+        }
+    }
+}
+
 void free_pm() {
     dev_free(g_pm.W);
     dev_free(g_pm.A);
@@ -408,7 +490,16 @@ int pm_forces() {
     CT2_LAUNCH_CHECK();
     CT2_TRY(fft3d(g_pm.A, g_pm.ng, -1, false));
     CT2_TRY(fft3d(g_pm.B, g_pm.ng, -1, false));
-    if (g_assign == 1)
+    // tiles hold n / ntile particles on average: stage a tile's nodes once when
+    // that beats one L2 read per particle and stencil node
+    const bool tiled = g >= TILE_MIN_G && (long)n * (g_assign == 1 ? 8 : 27) > 2L * 2 * ILX * ILY * ILZ * g_pm.ntile;
+    if (tiled && g_assign == 1)
+        interp_tile<1><<<g_pm.ntile, DEP_THREADS, 0, c.stream>>>(c.x, c.y, c.z, g_pm.order, g_pm.tstart, g, g_pm.A,
+                                                                 g_pm.B, c.fx, c.fy, c.fz, c.pot);
+    else if (tiled)
+        interp_tile<2><<<g_pm.ntile, DEP_THREADS, 0, c.stream>>>(c.x, c.y, c.z, g_pm.order, g_pm.tstart, g, g_pm.A,
+                                                                 g_pm.B, c.fx, c.fy, c.fz, c.pot);
+    else if (g_assign == 1)
         mesh_interp<1><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.A, g_pm.B, c.fx, c.fy, c.fz, c.pot);
     else
         mesh_interp<2><<<pb, PM_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, g, g_pm.A, g_pm.B, c.fx, c.fy, c.fz, c.pot);
