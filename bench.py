@@ -166,6 +166,28 @@ def fft_leg(api, ev: HipEvents, reps: int) -> dict:
     return fft
 
 
+def subset_leg(api, ev: HipEvents, n: int, params, reps: int) -> list:
+    """Kicks of a rung subset, as kick_step makes them (main.cpp:50-68):
+    particles on rung >= 1 are partitioned to the end (particle_sort_by_rung)
+    and kicked against every source by the ordered kernel.  Leaves the
+    container permuted, so it runs after the legs that judge fixed indices."""
+    out = []
+    for frac in (8, 64):
+        rung = np.where(np.arange(n) % frac == 0, 1, 0).astype(np.int8)
+        api.particles_upload(rung=rung)
+        api.particle_sort_by_rung(1)
+        first, last = api.particle_current_range()
+        sub = api.kick_params(ascending=True, descending=True, top=False, first=False, rung=1, t0=1e-3,
+                              eta=0.15, max_dt=1e-3, scale=1.0)
+        api.kick(sub)
+        ms = float(np.median(time_calls(ev, lambda: api.kick(sub), reps)))
+        m = last - first
+        out.append({"active": m, "sources": n, "ms": round(ms, 3), "interactions_per_s": m * float(n) / (ms * 1e-3),
+                    "kernel": "kick_direct (ordered pairs)"})
+        api.particle_sort_by_rung(0)
+    return out
+
+
 def cpu_baseline(x, y, z, soft, gm, target_s: float):
     """FP64 oracle kick timed on the host cores on a bounded sample of sinks
     (every source), scaled to interactions/s."""
@@ -338,6 +360,8 @@ def main() -> int:
     print("[bench] PM + near field ...", file=sys.stderr, flush=True)
     nearres = [near_leg(api, ev, side, kick_avg, idx, J, args.reps, ng, rs, rc, order)
                for ng, rs, rc, order in ((2 * side, 1.5, 5.5, 2), (2 * side, 2.0, 5.5, 2), (4 * side, 4.0, 5.5, 1))]
+    print("[bench] rung-subset kicks ...", file=sys.stderr, flush=True)
+    subset = subset_leg(api, ev, n, params, args.reps)
     steps_res = None
     if args.stepping_side:
         print("[bench] stepping configuration ...", file=sys.stderr, flush=True)
@@ -373,6 +397,7 @@ def main() -> int:
                   "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},
         "pm_mlp_vs_direct": pmres,
         "pm_near_vs_direct": nearres,
+        "rung_subset_kicks": subset,
         "stepping": steps_res,
     }
     if not args.no_cpu:
