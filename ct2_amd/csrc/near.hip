@@ -15,16 +15,18 @@
 // Pipeline (all on the library stream, deterministic):
 //   near_bin      cell of each particle, per-cell counts and slots (one atomic
 //                 per run of equal cells in a wave)
-//   near_scan     exclusive scan of the counts -> cell starts
-//   near_rows     work items: chunks of NEAR_BLOCK sinks within one (x, y) row
+//   near_scan     exclusive scan of the counts -> cell starts; again over the
+//                 rows -> work-item prefix (chunks of NEAR_BLOCK sinks per row)
 //   near_scatter  (x, y, z, index) into its cell's segment (slot order)
 //   near_order    each cell's segment sorted by (z slice, particle index)
 //                 (rank by counting), so the summation order never depends on
 //                 atomics, and the start of each of the cell's NEAR_SUB slices
-//   near_force    one workgroup per work item: sinks in registers, the 9
-//                 neighbouring rows' cell ranges streamed through LDS tiles,
-//                 FP32 pair terms with tile-level TwoSum; adds into the
-//                 stored force of each particle.
+//   near_items    the (row, chunk) of every work item
+//   near_force    one wave per work item: 64 sinks in registers, the 9
+//                 neighbouring rows' slice ranges streamed through LDS tiles,
+//                 the filtered kernel from cubic pieces in LDS, FP32 pair
+//                 terms with tile-level TwoSum; adds into the stored force of
+//                 each particle.
 // Cells have side >= rcut, so every partner of a sink lies in the 3x3x3 cells
 // around it; a work item's sinks span a run of slices along z in one row, and
 // the union of their neighbourhoods is 9 contiguous (or wrapped) slice ranges
