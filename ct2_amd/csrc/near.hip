@@ -47,7 +47,14 @@ namespace {
 // sinks per work item == sources per LDS tile: one wave, so that a work item's
 // sinks span little of their row and the wave skips distant sources
 constexpr int NEAR_BLOCK = NEAR_BLOCK_SIZE;
-constexpr int NEAR_MAX_NC = 64;  // cells per side at most
+constexpr int NEAR_MAX_NC = 64;  // cells per side at most (along z; x and y: NEAR_MAX_NC * F)
+// Dense cells (more than NEAR_SPLIT_DENSITY particles on average) are split
+// F = 2 ways in x and y, so a work item's 64 sinks span a squarer box (a
+// quarter of the cross-section, four times the z extent); partners then lie in
+// the 5 x 5 neighbouring columns.  Below that density the 3 x 3 stencil of
+// whole cells needs fewer candidate sources.
+constexpr int NEAR_SPLIT_DENSITY = 200;
+constexpr int NEAR_MAX_ROWS = 25;  // (2F + 1)^2
 // Each cell is cut into NEAR_SUB slices along z; a cell's segment is ordered
 // by (slice, particle index), so a row's particles are in slice order and a
 // work item's neighbour ranges start and end at slice boundaries.
@@ -69,7 +76,7 @@ struct NearState {
     int* slot = nullptr;           // [cap]
     uint4* P0 = nullptr;           // [cap] scattered (x, y, z, index)
     uint4* P = nullptr;            // [cap] sorted by (cell, index)
-    int ncell_cap = 0;
+    size_t ncell_cap = 0, nrow_cap = 0;  // cells and (x, y) columns the index buffers hold
     int* count = nullptr;          // [nc^3]
     int* start = nullptr;          // [nc^3 + 1]
     int* row_chunk = nullptr;      // [nc^2 + 1]
@@ -96,14 +103,15 @@ __global__ __launch_bounds__(256) void fill_int(int* __restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// cells: nxy x nxy columns in (x, y), nc along z; index (cx nxy + cy) nc + kz
 __global__ __launch_bounds__(256) void near_bin(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
-                                                const uint32_t* __restrict__ Z, int n, uint32_t nc,
+                                                const uint32_t* __restrict__ Z, int n, uint32_t nxy, uint32_t nc,
                                                 int* __restrict__ count, int* __restrict__ cell,
                                                 int* __restrict__ slot) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     int c = -1;
     if (p < n) {
-        c = (int)((cell_of(X[p], nc) * nc + cell_of(Y[p], nc)) * nc + cell_of(Z[p], nc));
+        c = (int)((cell_of(X[p], nxy) * nxy + cell_of(Y[p], nxy)) * nc + cell_of(Z[p], nc));
         cell[p] = c;
     }
     const int k = run_increment(count, c);  // every lane of the wave takes part
@@ -284,29 +292,31 @@ template <int KERNEL>
 __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict__ P, const int* __restrict__ start,
                                                          const int* __restrict__ subst,
                                                          const int* __restrict__ row_chunk,
-                                                         const int2* __restrict__ item, int nc, int R, NearConst q,
+                                                         const int2* __restrict__ item, int nxy, int nc, int F,
+                                                         int R, NearConst q,
                                                          const float4* __restrict__ table, double fscale,
                                                          double pscale, double pconst, float* __restrict__ fx,
                                                          float* __restrict__ fy, float* __restrict__ fz,
                                                          float* __restrict__ pot) {
     __shared__ uint4 tile[NEAR_BLOCK];
     __shared__ float4 lk[NEAR_NT + 1], lp[NEAR_NT + 1];  // the kernel's cubic pieces
-    __shared__ int rng[18][2];
+    __shared__ int rng[2 * NEAR_MAX_ROWS][2];
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
-    if (b >= row_chunk[nc * nc]) return;  // uniform across the workgroup
+    if (b >= row_chunk[nxy * nxy]) return;  // uniform across the workgroup
     const int2 it = item[b];
     const int row = it.x;
     const int s0 = start[row * nc] + it.y * NEAR_BLOCK;
     const int s1 = min(start[(row + 1) * nc], s0 + NEAR_BLOCK);
-    const int cx = row / nc, cy = row % nc;
-    if (tid < 9) {  // one neighbouring row per lane: at most two index ranges (z wraps)
+    const int cx = row / nxy, cy = row % nxy;
+    const int side = 2 * F + 1, nr = side * side;  // neighbouring rows: (cx, cy) +- F
+    if (tid < nr) {  // one neighbouring row per lane: at most two index ranges (z wraps)
         // the sinks' slices (a row is in slice order) widened by R slices, the
         // most a partner within rcut can differ
         const int Z = nc * NEAR_SUB;
         const int lo = (int)slice_of(P[s0].z, (uint32_t)Z) - R, hi = (int)slice_of(P[s1 - 1].z, (uint32_t)Z) + R;
-        const int di = tid / 3 - 1, dj = tid % 3 - 1;
-        const int* sr = subst + (((cx + di + nc) % nc) * nc + (cy + dj + nc) % nc) * Z;  // slice starts of the row
+        const int di = tid / side - F, dj = tid % side - F;
+        const int* sr = subst + (((cx + di + nxy) % nxy) * nxy + (cy + dj + nxy) % nxy) * Z;  // slice starts of the row
         int a0, e0, a1 = 0, e1 = 0;
         if (hi - lo + 1 >= Z) {  // the whole row
             a0 = sr[0];
@@ -338,7 +348,7 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
     const int i = s0 + tid;
     const uint4 me = P[i < s1 ? i : s1 - 1];
     float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f, cx_ = 0.f, cy_ = 0.f, cz_ = 0.f, cp_ = 0.f;
-    for (int r = 0; r < 18; ++r) {
+    for (int r = 0; r < 2 * nr; ++r) {
         const int a = rng[r][0], e = rng[r][1];
         for (int j0 = a; j0 < e; j0 += NEAR_BLOCK) {
             const int cnt = min(NEAR_BLOCK, e - j0);
@@ -409,10 +419,10 @@ void free_near_buffers() {
     g_near.item = nullptr;
     g_near.item_cap = 0;
     g_near.cap = 0;
-    g_near.ncell_cap = 0;
+    g_near.ncell_cap = g_near.nrow_cap = 0;
 }
 
-int ensure_buffers(size_t n, int nc) {
+int ensure_buffers(size_t n, size_t ncell, size_t nrow) {
     if (n > g_near.cap) {
         dev_free(g_near.cell);
         dev_free(g_near.slot);
@@ -427,21 +437,21 @@ int ensure_buffers(size_t n, int nc) {
         CT2_TRY(dev_alloc((void**)&g_near.P, n * sizeof(uint4), "near-field sorted particles"));
         g_near.cap = n;
     }
-    if (nc > g_near.ncell_cap) {
+    if (ncell > g_near.ncell_cap || nrow > g_near.nrow_cap) {
         dev_free(g_near.count);
         dev_free(g_near.start);
         dev_free(g_near.row_chunk);
         dev_free(g_near.subst);
         g_near.count = g_near.start = g_near.row_chunk = g_near.subst = nullptr;
-        g_near.ncell_cap = 0;
-        const size_t c3 = (size_t)nc * nc * nc;
-        CT2_TRY(dev_alloc((void**)&g_near.count, c3 * sizeof(int), "near-field counts"));
-        CT2_TRY(dev_alloc((void**)&g_near.start, (c3 + 1) * sizeof(int), "near-field cell starts"));
-        CT2_TRY(dev_alloc((void**)&g_near.row_chunk, ((size_t)nc * nc + 1) * sizeof(int), "near-field work items"));
-        CT2_TRY(dev_alloc((void**)&g_near.subst, (c3 * NEAR_SUB + 1) * sizeof(int), "near-field slice starts"));
-        g_near.ncell_cap = nc;
+        g_near.ncell_cap = g_near.nrow_cap = 0;
+        CT2_TRY(dev_alloc((void**)&g_near.count, ncell * sizeof(int), "near-field counts"));
+        CT2_TRY(dev_alloc((void**)&g_near.start, (ncell + 1) * sizeof(int), "near-field cell starts"));
+        CT2_TRY(dev_alloc((void**)&g_near.row_chunk, (nrow + 1) * sizeof(int), "near-field work items"));
+        CT2_TRY(dev_alloc((void**)&g_near.subst, (ncell * NEAR_SUB + 1) * sizeof(int), "near-field slice starts"));
+        g_near.ncell_cap = ncell;
+        g_near.nrow_cap = nrow;
     }
-    const size_t items = n / NEAR_BLOCK + 1 + (size_t)nc * nc;
+    const size_t items = n / NEAR_BLOCK + 1 + nrow;
     if (items > g_near.item_cap) {
         dev_free(g_near.item);
         g_near.item = nullptr;
@@ -529,14 +539,17 @@ int near_forces(double rs) {
     nc = std::min(nc, NEAR_MAX_NC);
     if (nc < 3) return set_error(CT2_ERR_INVALID, "near-field cutoff %g exceeds a third of the box", rcut);
     const int n = (int)c.n;
-    CT2_TRY(ensure_buffers(c.n, nc));
+    const int F = (double)n / ((double)nc * nc * nc) > NEAR_SPLIT_DENSITY ? 2 : 1;
+    const int nxy = F * nc;
+    const int c3 = nxy * nxy * nc, nrow = nxy * nxy;
+    CT2_TRY(ensure_buffers(c.n, (size_t)c3, (size_t)nrow));
     g_near.nc = nc;
-    const int c3 = nc * nc * nc, nrow = nc * nc;
     const unsigned pb = (unsigned)((n + 255) / 256);
     hipStream_t st = c.stream;
     fill_int<<<(c3 + 255) / 256, 256, 0, st>>>(g_near.count, c3);
     CT2_LAUNCH_CHECK();
-    near_bin<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, (uint32_t)nc, g_near.count, g_near.cell, g_near.slot);
+    near_bin<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, (uint32_t)nxy, (uint32_t)nc, g_near.count, g_near.cell,
+                                 g_near.slot);
     CT2_LAUNCH_CHECK();
     near_scan<<<1, SCAN_THREADS, 0, st>>>(g_near.count, c3, 0, nc, g_near.start);
     CT2_LAUNCH_CHECK();
@@ -583,12 +596,12 @@ int near_forces(double rs) {
     const unsigned items = (unsigned)((n + NEAR_BLOCK - 1) / NEAR_BLOCK + nrow);
     if (c.opts.kernel == CT2_KERNEL_PLUMMER)
         near_force<CT2_KERNEL_PLUMMER><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.subst,
-                                                                     g_near.row_chunk, g_near.item, nc, R, q,
-                                                                     g_near.table, fscale, pscale, pconst, c.fx,
+                                                                     g_near.row_chunk, g_near.item, nxy, nc, F, R,
+                                                                     q, g_near.table, fscale, pscale, pconst, c.fx,
                                                                      c.fy, c.fz, c.pot);
     else
         near_force<CT2_KERNEL_CT2><<<items, NEAR_BLOCK, 0, st>>>(g_near.P, g_near.start, g_near.subst,
-                                                                 g_near.row_chunk, g_near.item, nc, R, q,
+                                                                 g_near.row_chunk, g_near.item, nxy, nc, F, R, q,
                                                                  g_near.table, fscale, pscale, pconst, c.fx, c.fy,
                                                                  c.fz, c.pot);
     CT2_LAUNCH_CHECK();
