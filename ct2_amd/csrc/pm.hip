@@ -491,8 +491,13 @@ int pm_forces() {
     CT2_TRY(fft3d(g_pm.A, g_pm.ng, -1, false));
     CT2_TRY(fft3d(g_pm.B, g_pm.ng, -1, false));
     // tiles hold n / ntile particles on average: stage a tile's nodes once when
-    // that beats one L2 read per particle and stencil node
-    const bool tiled = g >= TILE_MIN_G && (long)n * (g_assign == 1 ? 8 : 27) > 2L * 2 * ILX * ILY * ILZ * g_pm.ntile;
+    // that beats one L2 read per particle and stencil node.  Measured: faster
+    // on 256^3 grids (0.19 -> 0.16 ms, TSC, 2^21 particles), slower on 512^3
+    // (1.0 -> 1.4 ms, 2^24 particles), where the per-particle gathers of
+    // particles in lattice order already hit L2 and the staged workgroup (one
+    // per CU) cannot overlap its loads with its arithmetic.
+    const bool tiled = g >= TILE_MIN_G && g <= 8 &&
+                       (long)n * (g_assign == 1 ? 8 : 27) > 2L * 2 * ILX * ILY * ILZ * g_pm.ntile;
     if (tiled && g_assign == 1)
         interp_tile<1><<<g_pm.ntile, DEP_THREADS, 0, c.stream>>>(c.x, c.y, c.z, g_pm.order, g_pm.tstart, g, g_pm.A,
                                                                  g_pm.B, c.fx, c.fy, c.fz, c.pot);
