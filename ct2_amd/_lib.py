@@ -25,6 +25,7 @@ KERNEL_CT2 = 1
 KICK_NAIVE = 0x1
 KICK_STORE = 0x2
 KICK_ORDERED = 0x4
+KICK_SYMMETRIC = 0x8
 
 
 class Ct2Error(RuntimeError):

@@ -20,6 +20,7 @@ KERNEL_CT2 = L.KERNEL_CT2
 KICK_NAIVE = L.KICK_NAIVE
 KICK_STORE = L.KICK_STORE
 KICK_ORDERED = L.KICK_ORDERED
+KICK_SYMMETRIC = L.KICK_SYMMETRIC
 
 
 def _p(a: np.ndarray | None):

@@ -71,27 +71,25 @@ __device__ __forceinline__ void interact(const uint4 src, uint32_t xi, uint32_t 
     }
 }
 
-// grid: (ceil(nsink/KICK_BLOCK), nsplit), sinks i0 .. n-1 (the active rung
-// range); block y sums sources [y*jchunk, min(n, (y+1)*jchunk)) of all n.
+// grid: (ceil(nsink/KICK_BLOCK), nsplit), sinks i0 .. i1-1; block y sums
+// sources [y*jchunk, min(nsrc, (y+1)*jchunk)) of sources 0 .. nsrc-1.
 // Output: partial[(a*nsplit + y)*nsink + i] for a in {x,y,z,pot}, in units
 // of 2^-64 (force) and 2^-32 (potential); see the scaling in kick_finalize.
 template <int KERNEL, bool NAIVE>
 __global__ __launch_bounds__(KICK_BLOCK) void kick_direct(const uint32_t* __restrict__ X,
                                                           const uint32_t* __restrict__ Y,
-                                                          const uint32_t* __restrict__ Z, int n, int i0,
+                                                          const uint32_t* __restrict__ Z, int i0, int i1, int nsrc,
                                                           int jchunk, KickConst kc,
-                                                          double* __restrict__ partial,
-                                                          int* __restrict__ maxrung) {
+                                                          double* __restrict__ partial) {
     __shared__ uint4 tile[2][KICK_TILE];
     const int tid = threadIdx.x;
-    const int ns = n - i0;  // sinks i0 .. n-1
+    const int ns = i1 - i0;  // sinks i0 .. i1-1
     const int i = blockIdx.x * KICK_BLOCK + tid;
-    const int ic = i < ns ? i0 + i : n - 1;
+    const int ic = i < ns ? i0 + i : i1 - 1;
     const uint32_t xi = X[ic], yi = Y[ic], zi = Z[ic];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) *maxrung = 0;
 
     const int j0 = blockIdx.y * jchunk;
-    const int j1 = min(n, j0 + jchunk);
+    const int j1 = min(nsrc, j0 + jchunk);
     float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f, cp = 0.f;
 
@@ -140,7 +138,15 @@ __global__ __launch_bounds__(KICK_BLOCK) void kick_direct(const uint32_t* __rest
     }
 }
 
-__global__ __launch_bounds__(256) void kick_finalize(const double* __restrict__ partial, int nsplit,
+// Partial sums of the sinks [first, first + count): part[(c*nsplit + y)*count + i - first].
+struct PartialSums {
+    const double* part;
+    int nsplit, first, count;
+};
+
+// Sums, for every sink i in [i0, n), the planes of each of the (up to three)
+// partial-sum sets that cover i, in the set order given.
+__global__ __launch_bounds__(256) void kick_finalize(PartialSums s0, PartialSums s1, PartialSums s2,
                                                      int n, int i0, double fscale, double pscale, EpiConst e,
                                                      float* __restrict__ vx, float* __restrict__ vy,
                                                      float* __restrict__ vz, int8_t* __restrict__ rung,
@@ -152,12 +158,16 @@ __global__ __launch_bounds__(256) void kick_finalize(const double* __restrict__ 
     const int i = i0 + t;
     int rmax = 0;
     if (t < ns) {
-        double a[4];
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (const PartialSums& ps : {s0, s1, s2}) {
+            const int k = i - ps.first;
+            if (!ps.part || k < 0 || k >= ps.count) continue;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            double s = 0.0;
-            for (int y = 0; y < nsplit; ++y) s += partial[((long)c * nsplit + y) * ns + t];
-            a[c] = s;
+            for (int c = 0; c < 4; ++c) {
+                double s = 0.0;
+                for (int y = 0; y < ps.nsplit; ++y) s += ps.part[((long)c * ps.nsplit + y) * ps.count + k];
+                a[c] += s;
+            }
         }
         const float f0 = (float)(a[0] * fscale), f1 = (float)(a[1] * fscale), f2 = (float)(a[2] * fscale);
         fx[i] = f0;
@@ -233,6 +243,11 @@ constexpr int SYM_BLOCK = 64 * SYM_WAVES;
 #define SYM_UNROLL 1  // measured fastest: more unrolling costs occupancy
 #endif
 constexpr int SYM_THREADS = SYM_BLOCK / SYM_NS;
+// smallest core the symmetric kernels take by default: 512 superblocks (the
+// 1/8- and 1/64-active rung subsets of 2^21 particles, 256 and 32 superblocks,
+// ran 4% and 20% slower than the ordered kernel alone; half active, 1,024
+// superblocks, 11% faster)
+constexpr int SYM_MIN_CORE = 512 * SYM_BLOCK;
 static_assert(SYM_THREADS >= 256, "the chunk reduction uses 256 threads (4 components x 64 sources)");
 
 __device__ __forceinline__ float rol1(float v) {  // lane l <- lane l+1 (mod 64)
@@ -296,13 +311,12 @@ __device__ __forceinline__ void sym_tile(const uint4* Sl, int lane, uint32_t xi,
 
 template <int KERNEL>
 __global__ __launch_bounds__(SYM_BLOCK) void kick_sym_diag(const uint4* __restrict__ P, int n, int ny, KickConst kc,
-                                                           double* __restrict__ part, int* __restrict__ maxrung) {
+                                                           double* __restrict__ part) {
     __shared__ uint4 S[SYM_WAVES][128];
     __shared__ float4 jr[SYM_WAVES][64];
     __shared__ double jacc[4][SYM_BLOCK];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long i = (long)blockIdx.x * SYM_BLOCK + tid;
-    if (blockIdx.x == 0 && tid == 0) *maxrung = 0;
     const uint4 me = P[i];
     S[w][lane] = me;
     S[w][lane + 64] = me;
@@ -509,6 +523,74 @@ __global__ __launch_bounds__(256) void kick_sample_fp64(const uint32_t* __restri
 
 uint4* g_packed = nullptr;
 size_t g_packed_bytes = 0;
+double* g_part_a = nullptr;  // ordered pass beside the symmetric core: core sinks x other sources
+size_t g_part_a_bytes = 0;
+double* g_part_b = nullptr;  // ordered pass of the sinks before the core x all sources
+size_t g_part_b_bytes = 0;
+
+// Ordered pass: sinks [i0, i1) against sources [0, nsrc), source chunks
+// sized so that the grid has enough workgroups to fill 256 CUs; partial sums
+// into *buf (grown as needed).  Returns the set's descriptor.
+int ordered_pass(Context& c, int i0, int i1, int nsrc, bool naive, bool plummer, const KickConst& kc, double** buf,
+                 size_t* bytes, PartialSums* out) {
+    const int ns = i1 - i0;
+    const int nb = (ns + KICK_BLOCK - 1) / KICK_BLOCK;
+    const int max_split = std::max(1, nsrc / (4 * KICK_TILE));
+    const int nsplit = std::min(max_split, std::max(1, (4096 + nb - 1) / nb));
+    int jchunk = (nsrc + nsplit - 1) / nsplit;
+    jchunk = (jchunk + KICK_TILE - 1) / KICK_TILE * KICK_TILE;
+    const int nsplit_eff = (nsrc + jchunk - 1) / jchunk;
+    CT2_TRY(ensure_scratch((void**)buf, bytes, sizeof(double) * 4 * (size_t)nsplit_eff * ns, "kick partial sums"));
+    const dim3 grid(nb, nsplit_eff);
+    double* part = *buf;
+    if (plummer) {
+        if (naive)
+            kick_direct<CT2_KERNEL_PLUMMER, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, i0, i1, nsrc, jchunk, kc, part);
+        else
+            kick_direct<CT2_KERNEL_PLUMMER, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, i0, i1, nsrc, jchunk, kc, part);
+    } else {
+        if (naive)
+            kick_direct<CT2_KERNEL_CT2, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, i0, i1, nsrc, jchunk, kc, part);
+        else
+            kick_direct<CT2_KERNEL_CT2, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, i0, i1, nsrc, jchunk, kc, part);
+    }
+    CT2_LAUNCH_CHECK();
+    *out = PartialSums{part, nsplit_eff, i0, ns};
+    return CT2_OK;
+}
+
+// Symmetric pass over the core [c0, c0 + m) (m a multiple of SYM_BLOCK):
+// every unordered pair of the core once, partial sums of the core's sinks
+// into c.partial.
+int symmetric_pass(Context& c, int c0, int m, bool plummer, const KickConst& kc, PartialSums* out) {
+    const int gs = m / SYM_BLOCK, rmax = gs / 2;
+    const size_t plane = sizeof(double) * 4 * (size_t)m;
+    const int rcap = (int)std::max<size_t>(1, ((size_t)4 << 30) / plane - 1);
+    const int R = std::max(1, std::min({16, rmax, rcap}));
+    const int ny = R + 1;
+    CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, plane * ny, "kick partial sums"));
+    CT2_TRY(ensure_scratch((void**)&g_packed, &g_packed_bytes, sizeof(uint4) * (size_t)m, "packed positions"));
+    pack_positions<<<(m + 255) / 256, 256, 0, c.stream>>>(c.x + c0, c.y + c0, c.z + c0, m, g_packed);
+    CT2_LAUNCH_CHECK();
+    const long total = 4L * ny * m;
+    fill_f64<<<(unsigned)((total + 255) / 256), 256, 0, c.stream>>>(c.partial, total);
+    CT2_LAUNCH_CHECK();
+    if (plummer)
+        kick_sym_diag<CT2_KERNEL_PLUMMER><<<gs, SYM_BLOCK, 0, c.stream>>>(g_packed, m, ny, kc, c.partial);
+    else
+        kick_sym_diag<CT2_KERNEL_CT2><<<gs, SYM_BLOCK, 0, c.stream>>>(g_packed, m, ny, kc, c.partial);
+    CT2_LAUNCH_CHECK();
+    for (int r0 = 1; r0 <= rmax; r0 += R) {
+        const int r1 = std::min(r0 + R, rmax + 1);
+        if (plummer)
+            kick_sym_pairs<CT2_KERNEL_PLUMMER><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1, kc, c.partial);
+        else
+            kick_sym_pairs<CT2_KERNEL_CT2><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1, kc, c.partial);
+        CT2_LAUNCH_CHECK();
+    }
+    *out = PartialSums{c.partial, ny, c0, m};
+    return CT2_OK;
+}
 double* g_sample_out = nullptr;
 size_t g_sample_bytes = 0;
 long* g_sample_idx = nullptr;
@@ -526,6 +608,12 @@ void release_kick_buffers() {
     dev_free(g_packed);
     g_packed = nullptr;
     g_packed_bytes = 0;
+    dev_free(g_part_a);
+    g_part_a = nullptr;
+    g_part_a_bytes = 0;
+    dev_free(g_part_b);
+    g_part_b = nullptr;
+    g_part_b_bytes = 0;
     dev_free(g_sample_out);
     g_sample_out = nullptr;
     g_sample_bytes = 0;
@@ -571,8 +659,10 @@ using namespace ct2;
 extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung) {
     CT2_TRY(require_particles());
     CT2_TRY(check_kick_params(p));
-    if (flags & ~(unsigned)(CT2_KICK_NAIVE | CT2_KICK_STORE | CT2_KICK_ORDERED))
+    if (flags & ~(unsigned)(CT2_KICK_NAIVE | CT2_KICK_STORE | CT2_KICK_ORDERED | CT2_KICK_SYMMETRIC))
         return set_error(CT2_ERR_INVALID, "unknown kick flags 0x%x", flags);
+    if ((flags & CT2_KICK_ORDERED) && (flags & CT2_KICK_SYMMETRIC))
+        return set_error(CT2_ERR_INVALID, "CT2_KICK_ORDERED and CT2_KICK_SYMMETRIC exclude each other");
     Context& c = ctx();
     const int n = (int)c.n;
     const int i0 = (int)c.rung_begin;  // sinks: the active rung range [i0, n)
@@ -592,67 +682,32 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
 
     const bool naive = flags & CT2_KICK_NAIVE;
     const bool plummer = c.opts.kernel == CT2_KERNEL_PLUMMER;
-    int nsplit_eff;
-    if (i0 == 0 && n % SYM_BLOCK == 0 && !naive && !(flags & CT2_KICK_ORDERED)) {
-        // each unordered pair once: a diagonal launch, then launches of R rounds
-        const int gs = n / SYM_BLOCK, rmax = gs / 2;
-        const size_t plane = sizeof(double) * 4 * (size_t)n;
-        const int rcap = (int)std::max<size_t>(1, ((size_t)4 << 30) / plane - 1);
-        const int R = std::max(1, std::min({16, rmax, rcap}));
-        nsplit_eff = R + 1;
-        CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, plane * nsplit_eff, "kick partial sums"));
-        CT2_TRY(ensure_scratch((void**)&g_packed, &g_packed_bytes, sizeof(uint4) * (size_t)n, "packed positions"));
-        pack_positions<<<(n + 255) / 256, 256, 0, c.stream>>>(c.x, c.y, c.z, n, g_packed);
-        CT2_LAUNCH_CHECK();
-        const long total = 4L * nsplit_eff * n;
-        fill_f64<<<(unsigned)((total + 255) / 256), 256, 0, c.stream>>>(c.partial, total);
-        CT2_LAUNCH_CHECK();
-        if (plummer)
-            kick_sym_diag<CT2_KERNEL_PLUMMER><<<gs, SYM_BLOCK, 0, c.stream>>>(g_packed, n, nsplit_eff, kc, c.partial, c.maxrung);
-        else
-            kick_sym_diag<CT2_KERNEL_CT2><<<gs, SYM_BLOCK, 0, c.stream>>>(g_packed, n, nsplit_eff, kc, c.partial, c.maxrung);
-        CT2_LAUNCH_CHECK();
-        for (int r0 = 1; r0 <= rmax; r0 += R) {
-            const int r1 = std::min(r0 + R, rmax + 1);
-            if (plummer)
-                kick_sym_pairs<CT2_KERNEL_PLUMMER><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, n, nsplit_eff, gs, r0, r1, kc, c.partial);
-            else
-                kick_sym_pairs<CT2_KERNEL_CT2><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, n, nsplit_eff, gs, r0, r1, kc, c.partial);
-            CT2_LAUNCH_CHECK();
-        }
-    } else {
-    // sink blocks and source chunks: enough workgroups to fill 256 CUs
-    const int nb = (ns + KICK_BLOCK - 1) / KICK_BLOCK;
-    const int max_split = std::max(1, n / (4 * KICK_TILE));
-    const int nsplit = std::min(max_split, std::max(1, (4096 + nb - 1) / nb));
-    int jchunk = (n + nsplit - 1) / nsplit;
-    jchunk = (jchunk + KICK_TILE - 1) / KICK_TILE * KICK_TILE;
-    nsplit_eff = (n + jchunk - 1) / jchunk;
-
-    CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, sizeof(double) * 4 * (size_t)nsplit_eff * ns,
-                           "kick partial sums"));
-
-    const dim3 grid(nb, nsplit_eff);
-    if (plummer) {
-        if (naive)
-            kick_direct<CT2_KERNEL_PLUMMER, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
-        else
-            kick_direct<CT2_KERNEL_PLUMMER, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
-    } else {
-        if (naive)
-            kick_direct<CT2_KERNEL_CT2, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
-        else
-            kick_direct<CT2_KERNEL_CT2, false><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, n, i0, jchunk, kc, c.partial, c.maxrung);
-    }
+    zero_int<<<1, 1, 0, c.stream>>>(c.maxrung);
     CT2_LAUNCH_CHECK();
+    // Each unordered pair once where possible: the symmetric kernels take the
+    // last m = floor(ns / 1024) * 1024 active sinks (the core [c0, n)); the
+    // ordered kernel adds the core's pairs with the particles before it and
+    // the few (< 1024) active sinks before the core against everything.  A
+    // core of fewer than SYM_MIN_CORE sinks has too few superblocks (one
+    // workgroup each) to fill the GPU, and the ordered kernel alone is faster.
+    PartialSums sets[3] = {{nullptr, 0, 0, 0}, {nullptr, 0, 0, 0}, {nullptr, 0, 0, 0}};
+    int m = (naive || (flags & CT2_KICK_ORDERED)) ? 0 : ns / SYM_BLOCK * SYM_BLOCK;
+    if (m < SYM_MIN_CORE && !(flags & CT2_KICK_SYMMETRIC)) m = 0;
+    if (m > 0) {
+        const int c0 = n - m;
+        CT2_TRY(symmetric_pass(c, c0, m, plummer, kc, &sets[0]));
+        if (c0 > 0) CT2_TRY(ordered_pass(c, c0, n, c0, false, plummer, kc, &g_part_a, &g_part_a_bytes, &sets[1]));
+        if (c0 > i0) CT2_TRY(ordered_pass(c, i0, c0, n, false, plummer, kc, &g_part_b, &g_part_b_bytes, &sets[2]));
+    } else {
+        CT2_TRY(ordered_pass(c, i0, n, n, naive, plummer, kc, &c.partial, &c.partial_bytes, &sets[0]));
     }
 
     const EpiConst e = make_epilogue(p, c.opts.hsoft);
     // force in box units: sum(dxu * w3u) * 2^64 * GM; potential: sum(wu) * 2^32 * GM
     const double fscale = (double)c.opts.GM * two32 * two32;
     const double pscale = (double)c.opts.GM * two32;
-    kick_finalize<<<(ns + 255) / 256, 256, 0, c.stream>>>(c.partial, nsplit_eff, n, i0, fscale, pscale, e, c.vx, c.vy,
-                                                          c.vz, c.rung, c.fx, c.fy, c.fz, c.pot, c.maxrung);
+    kick_finalize<<<(ns + 255) / 256, 256, 0, c.stream>>>(sets[0], sets[1], sets[2], n, i0, fscale, pscale, e, c.vx,
+                                                          c.vy, c.vz, c.rung, c.fx, c.fy, c.fz, c.pot, c.maxrung);
     CT2_LAUNCH_CHECK();
     c.forces_valid = true;
     if (max_rung) {

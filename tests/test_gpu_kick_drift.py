@@ -169,13 +169,17 @@ def test_kick_ragged_sizes(ct2, orc, n):
     assert np.allclose(pot, ap, rtol=1e-6)
 
 
-@pytest.mark.parametrize("n", [1024, 2048, 3072, 4096, 65536])
+@pytest.mark.parametrize("n", [1024, 1025, 2048, 3001, 3072, 4096, 65536, 65543])
 @pytest.mark.parametrize("kernel", [0, 1])
 def test_symmetric_and_ordered_kicks_vs_fp64(ct2, orc, n, kernel):
-    """n a multiple of 1024 with every particle a sink: each pair is evaluated
-    once and applied to both particles (one superblock; two; odd and even
-    superblock counts; 64 superblocks = two launches of 16 rounds).  Both
-    paths against the FP64 oracle, on a clustered set."""
+    """Every particle a sink: the symmetric kernels evaluate each pair of the
+    last floor(n / 1024) * 1024 particles once and apply it to both (one
+    superblock; two; odd and even superblock counts; 64 superblocks = two
+    launches of 16 rounds); for ragged n the ordered kernel adds the pairs of
+    that core with the first n % 1024 particles and those particles' own
+    sums.  CT2_KICK_SYMMETRIC forces that split at these sizes (by default a
+    core below 2^19 particles is left to the ordered kernel); the forced,
+    ordered and default paths against the FP64 oracle, on a clustered set."""
     x, y, z = ct2.ic_plummer_mix(n, n // 2, 4, 0.02, 0.2, n + kernel)
     load(ct2, x, y, z)
     soft, gm = f32(2e-3), f32(1.0 / n)
@@ -183,13 +187,13 @@ def test_symmetric_and_ordered_kicks_vs_fp64(ct2, orc, n, kernel):
     sinks = np.arange(n) if n <= 4096 else np.random.default_rng(1).choice(n, 1024, replace=False)
     ax, ay, az, ap = orc.kick(x, y, z, gm, soft, kernel, sinks=sinks)
     A = np.stack([ax, ay, az])
-    for flag in (0, ct2.KICK_ORDERED):
+    for flag in (ct2.KICK_SYMMETRIC, ct2.KICK_ORDERED, 0):
         ct2.kick(None, flag | ct2.KICK_STORE)
         fx, fy, fz, pot = ct2.forces_download()
         e = rel_err(np.stack([fx[sinks], fy[sinks], fz[sinks]]), A)
         assert np.median(e) < 2e-6 and np.percentile(e, 99) < 5e-5, (flag, np.median(e), e.max())
         assert np.max(np.abs(pot[sinks] / ap - 1)) < 1e-6
-        if flag == 0:  # Newton's third law: the pair forces cancel in the total
+        if flag == ct2.KICK_SYMMETRIC:  # Newton's third law: the pair forces cancel in the total
             F = np.stack([fx, fy, fz]).astype(np.float64)
             assert np.abs(F.sum(1)).max() < 1e-6 * np.abs(F).sum(1).max()
 
@@ -203,3 +207,5 @@ def test_kick_rejects_bad_params(ct2):
         ct2.kick(ct2.kick_params(descending=1, rung=0, t0=0.0))
     with pytest.raises(Ct2Error):
         ct2.kick_sample_fp64([10])
+    with pytest.raises(Ct2Error):
+        ct2.kick(None, ct2.KICK_ORDERED | ct2.KICK_SYMMETRIC)
