@@ -24,6 +24,7 @@ namespace ct2 {
 namespace {
 
 constexpr int KIN = 128, HID = 256, KOUT = 32, NOUT = 3, NFEAT = 111;
+constexpr int SPLITK_MAX = 64;  // K slices of the weight-gradient products at most
 constexpr size_t OFF_W1 = 0, OFF_B1 = OFF_W1 + KIN * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
                  OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * KOUT, NPARAM = OFF_B3 + KOUT;
 constexpr float GELU_C = 0.7978845608028654f;
@@ -98,13 +99,18 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tm = (blockIdx.x * 4 + wave) * 32, tn = blockIdx.y * 32;
     if (tm >= M) return;
+    // split K: slice blockIdx.z of gridDim.z sums k in [kb, ke) (EPI_STORE only;
+    // the slice's partial product goes to plane z of ep.f32)
+    const int kc = ((K / 32 + gridDim.z - 1) / gridDim.z) * 32;
+    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+    if constexpr (EPI == EPI_STORE) ep.f32 += (size_t)blockIdx.z * M * ep.ldc;
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
     floatx4 acc[2][2] = {};
     const bf16_t* a0 = A + (size_t)(tm + r16) * lda + kq;
     const bf16_t* a1 = a0 + (size_t)16 * lda;
     const bf16_t* b0 = Bt + (size_t)(tn + r16) * ldb + kq;
     const bf16_t* b1 = b0 + (size_t)16 * ldb;
-    for (int k = 0; k < K; k += 32) {
+    for (int k = kb; k < ke; k += 32) {
         const bf16x8 fa0 = load8(a0 + k), fa1 = load8(a1 + k), fb0 = load8(b0 + k), fb1 = load8(b1 + k);
         acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
@@ -146,8 +152,37 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
 template <int EPI>
 int gemm(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K, const EpiArgs& ep) {
     if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
-    gemm_bf16<EPI><<<dim3(M / 128, N / 32), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, ep);
+    gemm_bf16<EPI><<<dim3(M / 128, N / 32, 1), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, ep);
     CT2_LAUNCH_CHECK();
+    return CT2_OK;
+}
+
+// out[i] = sum over z of part[z][i], z = 0 .. S-1 in order
+__global__ __launch_bounds__(256) void splitk_reduce(const float* __restrict__ part, int S, int len,
+                                                     float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.0f;
+    for (int z = 0; z < S; ++z) s += part[(size_t)z * len + i];
+    out[i] = s;
+}
+
+// C = A B with a long K (the weight gradients, K = batch rows): K is cut into S
+// slices so that the grid fills the GPU; each slice's product goes to its own
+// plane of `scratch` (S * M * N floats) and splitk_reduce adds the planes in
+// slice order, so the result does not depend on scheduling.
+int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K,
+                float* scratch, int smax, float* out) {
+    if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
+    const int tiles = (M / 128) * (N / 32);
+    const int S = std::max(1, std::min({smax, K / 256, (1024 + tiles - 1) / tiles}));
+    EpiArgs ep{S > 1 ? scratch : out, N, nullptr, nullptr, 0, nullptr, nullptr};
+    gemm_bf16<EPI_STORE><<<dim3(M / 128, N / 32, S), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, ep);
+    CT2_LAUNCH_CHECK();
+    if (S > 1) {
+        splitk_reduce<<<(M * N + 255) / 256, 256, 0, st>>>(scratch, S, M * N, out);
+        CT2_LAUNCH_CHECK();
+    }
     return CT2_OK;
 }
 
@@ -321,13 +356,34 @@ __global__ void loss_kernel(const float* __restrict__ Yo, const float* __restric
     if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// db[n] = sum_r D[r][n] (fixed order)
-__global__ void colsum_kernel(const float* __restrict__ D, int M, int N, float* __restrict__ db) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
+// db[n] = sum_r D[r][n] in two fixed-order stages (N <= 256): block b sums
+// rows [256 b, 256 b + 256) of each column (colsum_part), then one block adds
+// the blocks' sums in order (colsum_final)
+constexpr int COLSUM_ROWS = 256;
+__global__ __launch_bounds__(256) void colsum_part(const float* __restrict__ D, int M, int N,
+                                                   float* __restrict__ part) {
+    const int n = threadIdx.x, b = blockIdx.x;
+    if (n >= N) return;
+    const int r0 = b * COLSUM_ROWS, r1 = min(M, r0 + COLSUM_ROWS);
+    float s = 0.0f;
+    for (int r = r0; r < r1; ++r) s += D[(size_t)r * N + n];
+    part[(size_t)b * N + n] = s;
+}
+__global__ __launch_bounds__(256) void colsum_final(const float* __restrict__ part, int nb, int N,
+                                                    float* __restrict__ db) {
+    const int n = threadIdx.x;
     if (n >= N) return;
     float s = 0.0f;
-    for (int r = 0; r < M; ++r) s += D[(size_t)r * N + n];
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * N + n];
     db[n] = s;
+}
+int colsum(hipStream_t st, const float* D, int M, int N, float* part, float* db) {
+    const int nb = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
+    colsum_part<<<nb, 256, 0, st>>>(D, M, N, part);
+    CT2_LAUNCH_CHECK();
+    colsum_final<<<1, 256, 0, st>>>(part, nb, N, db);
+    CT2_LAUNCH_CHECK();
+    return CT2_OK;
 }
 
 // F[p] += F_unit * Y[r][0..2] for p = r0 + r
@@ -546,6 +602,8 @@ struct Mlp {
     bf16_t *dY = nullptr, *dYT = nullptr, *dZ2 = nullptr, *dZ2T = nullptr, *dZ1T = nullptr;
     float *Z1 = nullptr, *Z2 = nullptr, *Yo = nullptr, *T = nullptr, *dYf = nullptr, *dZ2f = nullptr, *dZ1f = nullptr;
     float* lpart = nullptr;
+    float* gsplit = nullptr;  // split-K planes of the weight-gradient products
+    float* cspart = nullptr;  // per-block column sums of the bias gradients
     float* Xf = nullptr;  // fp32 features for host download
     long* idx = nullptr;
     double* judge = nullptr;
@@ -559,12 +617,14 @@ void free_work() {
     for (void* p : {(void*)g_mlp.X, (void*)g_mlp.XT, (void*)g_mlp.H1, (void*)g_mlp.H1T, (void*)g_mlp.H2, (void*)g_mlp.H2T,
                     (void*)g_mlp.dY, (void*)g_mlp.dYT, (void*)g_mlp.dZ2, (void*)g_mlp.dZ2T, (void*)g_mlp.dZ1T,
                     (void*)g_mlp.Z1, (void*)g_mlp.Z2, (void*)g_mlp.Yo, (void*)g_mlp.T, (void*)g_mlp.dYf,
-                    (void*)g_mlp.dZ2f, (void*)g_mlp.dZ1f, (void*)g_mlp.lpart, (void*)g_mlp.Xf, (void*)g_mlp.idx,
+                    (void*)g_mlp.dZ2f, (void*)g_mlp.dZ1f, (void*)g_mlp.lpart, (void*)g_mlp.gsplit,
+                    (void*)g_mlp.cspart, (void*)g_mlp.Xf, (void*)g_mlp.idx,
                     (void*)g_mlp.judge})
         dev_free(p);
     g_mlp.X = g_mlp.XT = g_mlp.H1 = g_mlp.H1T = g_mlp.H2 = g_mlp.H2T = nullptr;
     g_mlp.dY = g_mlp.dYT = g_mlp.dZ2 = g_mlp.dZ2T = g_mlp.dZ1T = nullptr;
     g_mlp.Z1 = g_mlp.Z2 = g_mlp.Yo = g_mlp.T = g_mlp.dYf = g_mlp.dZ2f = g_mlp.dZ1f = g_mlp.lpart = g_mlp.Xf = nullptr;
+    g_mlp.gsplit = g_mlp.cspart = nullptr;
     g_mlp.idx = nullptr;
     g_mlp.judge = nullptr;
     g_mlp.cap = 0;
@@ -603,6 +663,8 @@ int ensure_work(int rows, bool training) {
         ALLOC(g_mlp.dZ2f, R * HID * 4, "mlp dZ2f");
         ALLOC(g_mlp.dZ1f, R * HID * 4, "mlp dZ1f");
         ALLOC(g_mlp.lpart, (R / 8 + 1) * 4, "mlp loss");
+        ALLOC(g_mlp.gsplit, (size_t)SPLITK_MAX * HID * HID * 4, "mlp split-K planes");
+        ALLOC(g_mlp.cspart, (R / COLSUM_ROWS + 1) * HID * 4, "mlp column sums");
         ALLOC(g_mlp.Xf, R * KIN * 4, "mlp X fp32");
         ALLOC(g_mlp.judge, R * 4 * 8, "mlp judge");
     }
@@ -643,26 +705,20 @@ int backward(hipStream_t st, int M) {
     loss_kernel<<<(M + 7) / 8, 256, 0, st>>>(g_mlp.Yo, g_mlp.T, M, g_mlp.dYf, g_mlp.dY, g_mlp.dYT, g_mlp.lpart);
     CT2_LAUNCH_CHECK();
     // dW3 = H2^T dY, db3
-    EpiArgs s3{g + OFF_W3, KOUT, nullptr, nullptr, 0, nullptr, nullptr};
-    CT2_TRY(gemm<EPI_STORE>(st, g_mlp.H2T, M, g_mlp.dYT, M, HID, KOUT, M, s3));
-    colsum_kernel<<<1, 256, 0, st>>>(g_mlp.dYf, M, KOUT, g + OFF_B3);
-    CT2_LAUNCH_CHECK();
+    CT2_TRY(gemm_splitk(st, g_mlp.H2T, M, g_mlp.dYT, M, HID, KOUT, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W3));
+    CT2_TRY(colsum(st, g_mlp.dYf, M, KOUT, g_mlp.cspart, g + OFF_B3));
     // dZ2 = (dY W3^T) * gelu'(Z2)
     EpiArgs d2{g_mlp.dZ2f, HID, g_mlp.dZ2, g_mlp.dZ2T, M, nullptr, g_mlp.Z2};
     CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dY, KOUT, g_mlp.W3n, KOUT, M, HID, KOUT, d2));
     // dW2 = H1^T dZ2, db2
-    EpiArgs s2{g + OFF_W2, HID, nullptr, nullptr, 0, nullptr, nullptr};
-    CT2_TRY(gemm<EPI_STORE>(st, g_mlp.H1T, M, g_mlp.dZ2T, M, HID, HID, M, s2));
-    colsum_kernel<<<1, 256, 0, st>>>(g_mlp.dZ2f, M, HID, g + OFF_B2);
-    CT2_LAUNCH_CHECK();
+    CT2_TRY(gemm_splitk(st, g_mlp.H1T, M, g_mlp.dZ2T, M, HID, HID, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W2));
+    CT2_TRY(colsum(st, g_mlp.dZ2f, M, HID, g_mlp.cspart, g + OFF_B2));
     // dZ1 = (dZ2 W2^T) * gelu'(Z1)
     EpiArgs d1{g_mlp.dZ1f, HID, nullptr, g_mlp.dZ1T, M, nullptr, g_mlp.Z1};
     CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dZ2, HID, g_mlp.W2n, HID, M, HID, HID, d1));
     // dW1 = X^T dZ1, db1
-    EpiArgs s1{g + OFF_W1, HID, nullptr, nullptr, 0, nullptr, nullptr};
-    CT2_TRY(gemm<EPI_STORE>(st, g_mlp.XT, M, g_mlp.dZ1T, M, KIN, HID, M, s1));
-    colsum_kernel<<<1, 256, 0, st>>>(g_mlp.dZ1f, M, HID, g + OFF_B1);
-    CT2_LAUNCH_CHECK();
+    CT2_TRY(gemm_splitk(st, g_mlp.XT, M, g_mlp.dZ1T, M, KIN, HID, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W1));
+    CT2_TRY(colsum(st, g_mlp.dZ1f, M, HID, g_mlp.cspart, g + OFF_B1));
     return CT2_OK;
 }
 
