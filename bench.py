@@ -166,6 +166,24 @@ def fft_leg(api, ev: HipEvents, reps: int) -> dict:
     return fft
 
 
+def train_leg(api, ev: HipEvents, n: int) -> list:
+    """Training throughput of the learned correction: Adam steps of the
+    library's own forward and backward kernels on 32,768 judged samples of the
+    current particles (PM grid of the PM + MLP leg), 568,320 flop per sample
+    (forward + backward)."""
+    api.mlp_init(7)
+    api.mlp_data_reset(32768)
+    api.mlp_data_append(np.random.default_rng(11).choice(n, 32768, replace=False))
+    out = []
+    for batch in (16384, 65536):
+        api.mlp_train(2, batch)
+        steps = 10
+        ms = float(np.median(time_calls(ev, lambda: api.mlp_train(steps, batch), 3))) / steps
+        out.append({"batch": batch, "ms_per_step": round(ms, 4), "TFLOP/s": round(568320 * batch / ms / 1e9, 1),
+                    "frac_bf16_mfma": round(568320 * batch / ms / 1e9 / 2500.0, 4)})
+    return out
+
+
 def subset_leg(api, ev: HipEvents, n: int, params, reps: int) -> list:
     """Kicks of a rung subset, as kick_step makes them (main.cpp:50-68):
     particles on rung >= 1 are partitioned to the end (particle_sort_by_rung)
@@ -360,6 +378,8 @@ def main() -> int:
     print("[bench] PM + near field ...", file=sys.stderr, flush=True)
     nearres = [near_leg(api, ev, side, kick_avg, idx, J, args.reps, ng, rs, rc, order)
                for ng, rs, rc, order in ((2 * side, 1.5, 5.5, 2), (2 * side, 2.0, 5.5, 2), (4 * side, 4.0, 5.5, 1))]
+    print("[bench] learned-correction training ...", file=sys.stderr, flush=True)
+    train = train_leg(api, ev, n)
     print("[bench] rung-subset kicks ...", file=sys.stderr, flush=True)
     subset = subset_leg(api, ev, n, params, args.reps)
     steps_res = None
@@ -398,6 +418,7 @@ def main() -> int:
         "pm_mlp_vs_direct": pmres,
         "pm_near_vs_direct": nearres,
         "rung_subset_kicks": subset,
+        "mlp_training": train,
         "stepping": steps_res,
     }
     if not args.no_cpu:

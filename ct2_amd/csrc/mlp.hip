@@ -357,31 +357,39 @@ __global__ void loss_kernel(const float* __restrict__ Yo, const float* __restric
 }
 
 // db[n] = sum_r D[r][n] in two fixed-order stages (N <= 256): block b sums
-// rows [256 b, 256 b + 256) of each column (colsum_part), then one block adds
-// the blocks' sums in order (colsum_final)
-constexpr int COLSUM_ROWS = 256;
+// rows [64 b, 64 b + 64) of each column with four interleaved accumulators
+// (colsum_part); then block n adds column n's block sums, 256 threads and an
+// LDS tree in a fixed pattern (colsum_final).  The result does not depend on
+// scheduling.
+constexpr int COLSUM_ROWS = 64;
 __global__ __launch_bounds__(256) void colsum_part(const float* __restrict__ D, int M, int N,
                                                    float* __restrict__ part) {
     const int n = threadIdx.x, b = blockIdx.x;
     if (n >= N) return;
     const int r0 = b * COLSUM_ROWS, r1 = min(M, r0 + COLSUM_ROWS);
-    float s = 0.0f;
-    for (int r = r0; r < r1; ++r) s += D[(size_t)r * N + n];
-    part[(size_t)b * N + n] = s;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int r = r0; r < r1; ++r) s[(r - r0) & 3] += D[(size_t)r * N + n];
+    part[(size_t)b * N + n] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 __global__ __launch_bounds__(256) void colsum_final(const float* __restrict__ part, int nb, int N,
                                                     float* __restrict__ db) {
-    const int n = threadIdx.x;
-    if (n >= N) return;
+    __shared__ float red[256];
+    const int n = blockIdx.x, t = threadIdx.x;
     float s = 0.0f;
-    for (int b = 0; b < nb; ++b) s += part[(size_t)b * N + n];
-    db[n] = s;
+    for (int b = t; b < nb; b += 256) s += part[(size_t)b * N + n];
+    red[t] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) db[n] = red[0];
 }
 int colsum(hipStream_t st, const float* D, int M, int N, float* part, float* db) {
     const int nb = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
     colsum_part<<<nb, 256, 0, st>>>(D, M, N, part);
     CT2_LAUNCH_CHECK();
-    colsum_final<<<1, 256, 0, st>>>(part, nb, N, db);
+    colsum_final<<<N, 256, 0, st>>>(part, nb, N, db);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
