@@ -96,6 +96,36 @@ def test_kicks_act_on_the_current_range(ct2, orc):
     assert not np.array_equal(p2.vx[first:], p1.vx[first:])
 
 
+@pytest.mark.parametrize("flag_name", ["KICK_SYMMETRIC", "KICK_ORDERED"])
+def test_subset_kick_with_symmetric_core(ct2, orc, flag_name):
+    """Active range [first, n) of 3,000 + 2,500 particles: with
+    CT2_KICK_SYMMETRIC its last 2,048 sinks form the symmetric core, the
+    ordered kernel adds the core's pairs with everything before it and the
+    other active sinks' sums; all against the FP64 oracle, and the inactive
+    particles untouched."""
+    n = 5500
+    rng = np.random.default_rng(9)
+    x, y, z = ct2.ic_plummer_mix(n, 2000, 3, 0.02, 0.2, 17)
+    rung = np.where(rng.random(n) < 0.45, 0, 1).astype(np.int8)
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z, None, None, None, rung)
+    gm, soft = float(np.float32(1.0 / n)), float(np.float32(2e-3))
+    ct2.set_force_options(soft, gm, ct2.KERNEL_PLUMMER)
+    ct2.particle_sort_by_rung(1)
+    first, _ = ct2.particle_current_range()
+    assert n - first >= 2048 and (n - first) % 1024 != 0
+    p0 = ct2.particles_download()
+    ct2.kick(None, getattr(ct2, flag_name) | ct2.KICK_STORE)
+    fx, fy, fz, pot = ct2.forces_download()
+    ax, ay, az, ap = orc.kick(p0.x, p0.y, p0.z, gm, soft, orc.KERNEL_PLUMMER, sinks=np.arange(first, n))
+    W = np.stack([ax, ay, az])
+    e = np.linalg.norm(np.stack([fx, fy, fz])[:, first:] - W, axis=0) / np.linalg.norm(W, axis=0)
+    assert np.median(e) < 2e-6 and np.percentile(e, 99) < 5e-5, (np.median(e), e.max())
+    assert np.max(np.abs(pot[first:] / ap - 1)) < 1e-6
+    p1 = ct2.particles_download()
+    assert np.array_equal(p1.vx[:first], p0.vx[:first])
+
+
 class Trace:
     def __init__(self, lines):
         self.lines, self.i = lines, 0
