@@ -289,8 +289,8 @@ def main() -> int:
                     help="first run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for how "
-              "the kick and the FFT would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the kick would shard by sinks "
+              "after one all_gather of positions, the FFT by slabs with one all_to_all)", file=sys.stderr)
         return 2
 
     if args.copy_probe:
