@@ -98,6 +98,8 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
                                                  int ldb, int M, int N, int K, EpiArgs ep) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tm = (blockIdx.x * 4 + wave) * 32, tn = blockIdx.y * 32;
+    // a wave's output tile transposed, for the epilogues that also write C^T
+    __shared__ __align__(16) bf16_t tt[(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) ? 4 : 1][32][32];
     if (tm >= M) return;
     // split K: slice blockIdx.z of gridDim.z sums k in [kb, ke) (EPI_STORE only;
     // the slice's partial product goes to plane z of ep.f32)
@@ -137,16 +139,30 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
                     if (ep.f32) ep.f32[o] = z;
                     const bf16_t hv = f2bf(gelu(z));
                     ep.h[o] = hv;
-                    if (ep.ht) ep.ht[(size_t)col * ep.ldt + row] = hv;
+                    tt[wave][col - tn][row - tm] = hv;
                 } else {  // EPI_DGELU
                     const float d = v * gelu_d(ep.zin[o]);
                     ep.f32[o] = d;
                     const bf16_t hv = f2bf(d);
                     if (ep.h) ep.h[o] = hv;
-                    if (ep.ht) ep.ht[(size_t)col * ep.ldt + row] = hv;
+                    tt[wave][col - tn][row - tm] = hv;
                 }
             }
         }
+    if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
+        // the transposed copy: the wave's 32 x 32 tile through LDS, written as
+        // 32 rows of 64 contiguous bytes (16 bytes per lane and store)
+        if (ep.ht) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int e = q * 64 + lane, c = e >> 2, part = e & 3;
+                *(uint4*)(ep.ht + (size_t)(tn + c) * ep.ldt + tm + 8 * part) = *(const uint4*)&tt[wave][c][8 * part];
+            }
+        }
+    }
 }
 
 template <int EPI>
@@ -319,15 +335,31 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 // Batch of M rows drawn uniformly (with replacement) from the data set:
 // X[M][128], XT[128][M], T[M][4].
-__global__ void batch_kernel(const bf16_t* __restrict__ DX, const float* __restrict__ DT, long size, uint64_t key, int M,
-                             bf16_t* __restrict__ X, bf16_t* __restrict__ XT, float* __restrict__ T) {
-    const int r = blockIdx.x;
-    const long p = (long)(mix64(key + (uint64_t)r) % (uint64_t)size);
-    const int k = threadIdx.x;  // 128 lanes
-    const bf16_t v = DX[(size_t)p * KIN + k];
-    X[(size_t)r * KIN + k] = v;
-    XT[(size_t)k * M + r] = v;
-    if (k < 4) T[4 * (size_t)r + k] = DT[4 * (size_t)p + k];
+// Block b draws rows [64 b, 64 b + 64) (M % 64 == 0); XT is written through an
+// LDS transpose, 64 contiguous rows (128 bytes) per feature.
+constexpr int BATCH_ROWS = 64, BATCH_THREADS = 4 * KIN;
+__global__ __launch_bounds__(BATCH_THREADS) void batch_kernel(const bf16_t* __restrict__ DX,
+                                                              const float* __restrict__ DT, long size, uint64_t key,
+                                                              int M, bf16_t* __restrict__ X, bf16_t* __restrict__ XT,
+                                                              float* __restrict__ T) {
+    __shared__ bf16_t tile[KIN][BATCH_ROWS + 2];
+    const int k = threadIdx.x % KIN;  // one feature per lane, four rows at a time
+    const int rb = blockIdx.x * BATCH_ROWS;
+    for (int q = threadIdx.x / KIN; q < BATCH_ROWS; q += BATCH_THREADS / KIN) {
+        const int r = rb + q;
+        const long p = (long)(mix64(key + (uint64_t)r) % (uint64_t)size);
+        const bf16_t v = DX[(size_t)p * KIN + k];
+        X[(size_t)r * KIN + k] = v;
+        tile[k][q] = v;
+        if (k < 4) T[4 * (size_t)r + k] = DT[4 * (size_t)p + k];
+    }
+    __syncthreads();
+    // feature f of the block's rows: 64 contiguous elements of XT row f
+    for (int e = threadIdx.x; e < KIN * BATCH_ROWS / 2; e += BATCH_THREADS) {
+        const int f = e / (BATCH_ROWS / 2), h = e % (BATCH_ROWS / 2);
+        const uint32_t w = (uint32_t)tile[f][2 * h] | ((uint32_t)tile[f][2 * h + 1] << 16);
+        *(uint32_t*)(XT + (size_t)f * M + rb + 2 * h) = w;
+    }
 }
 
 // dY = 2 w (Y - T) / M on the first 3 outputs (w = T[r][3]), 0 elsewhere;
@@ -995,7 +1027,8 @@ extern "C" int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, floa
     std::vector<float> lp((batch + 7) / 8);
     for (int s = 0; s < steps; ++s) {
         const uint64_t key = (seed * 0xD1B54A32D192ED03ull) ^ ((uint64_t)s * 0x632BE59BD9B4E019ull);
-        batch_kernel<<<batch, KIN, 0, st>>>(g_mlp.DX, g_mlp.DT, g_mlp.dsize, key, batch, g_mlp.X, g_mlp.XT, g_mlp.T);
+        batch_kernel<<<batch / BATCH_ROWS, BATCH_THREADS, 0, st>>>(g_mlp.DX, g_mlp.DT, g_mlp.dsize, key, batch,
+                                                                   g_mlp.X, g_mlp.XT, g_mlp.T);
         CT2_LAUNCH_CHECK();
         CT2_TRY(forward(st, batch, true));
         CT2_TRY(backward(st, batch));
