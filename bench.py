@@ -187,8 +187,10 @@ def train_leg(api, ev: HipEvents, n: int) -> list:
 def subset_leg(api, ev: HipEvents, n: int, params, reps: int) -> list:
     """Kicks of a rung subset, as kick_step makes them (main.cpp:50-68):
     particles on rung >= 1 are partitioned to the end (particle_sort_by_rung)
-    and kicked against every source by the ordered kernel.  Leaves the
-    container permuted, so it runs after the legs that judge fixed indices."""
+    and kicked against every source: pairs within the active set once
+    (symmetric core) when it holds >= 65,536 particles, the rest by the
+    ordered kernel.  Leaves the container permuted, so it runs after the legs
+    that judge fixed indices."""
     out = []
     for frac in (8, 64):
         rung = np.where(np.arange(n) % frac == 0, 1, 0).astype(np.int8)
@@ -201,7 +203,7 @@ def subset_leg(api, ev: HipEvents, n: int, params, reps: int) -> list:
         ms = float(np.median(time_calls(ev, lambda: api.kick(sub), reps)))
         m = last - first
         out.append({"active": m, "sources": n, "ms": round(ms, 3), "interactions_per_s": m * float(n) / (ms * 1e-3),
-                    "kernel": "kick_direct (ordered pairs)"})
+                    "kernel": "kick_sym_pairs core + kick_direct" if m >= 65536 else "kick_direct (ordered pairs)"})
         api.particle_sort_by_rung(0)
     return out
 

@@ -230,10 +230,12 @@ __global__ void zero_int(int* p) { *p = 0; }
 // initialises plane 0 of the partial sums; then launches over rounds r:
 // workgroup g pairs superblock g with J = (g + r) mod Gs (r = Gs/2 only for
 // g < Gs/2 when Gs is even), keeps its sink-side sums in registers across the
-// launch's rounds (plane 0, one read-modify-write per launch) and adds the
-// source-side sums of round r into plane 1 + (r - r0): within a launch every
-// (plane, superblock) slot has exactly one writer.  kick_finalize then adds the
-// planes in order.
+// launch's rounds (one read-modify-write per launch) and adds the source-side
+// sums of round r into plane 1 + (r - r0).  Below 1024 superblocks a launch's
+// R rounds are dealt round-robin to `groups` workgroups per superblock, group
+// q writing its sink-side sums to plane 0 (q = 0) or R + q: within a launch
+// every (plane, superblock) slot has exactly one writer.  kick_finalize then
+// adds the planes in order.
 constexpr int SYM_WAVES = 16;  // chunks of 64 per superblock (and waves of the diagonal kernel)
 constexpr int SYM_BLOCK = 64 * SYM_WAVES;
 #ifndef SYM_NS
@@ -243,11 +245,11 @@ constexpr int SYM_BLOCK = 64 * SYM_WAVES;
 #define SYM_UNROLL 1  // measured fastest: more unrolling costs occupancy
 #endif
 constexpr int SYM_THREADS = SYM_BLOCK / SYM_NS;
-// smallest core the symmetric kernels take by default: 512 superblocks (the
-// 1/8- and 1/64-active rung subsets of 2^21 particles, 256 and 32 superblocks,
-// ran 4% and 20% slower than the ordered kernel alone; half active, 1,024
-// superblocks, 11% faster)
-constexpr int SYM_MIN_CORE = 512 * SYM_BLOCK;
+// smallest core the symmetric kernels take by default: 64 superblocks.  A
+// round takes ~0.15 ms per workgroup whatever the core size, so cores of
+// 4-32 superblocks (full kicks of 4,096-32,768 particles) ran 1.2-4x slower
+// than the ordered kernel; 64 superblocks and up ran 3-22% faster.
+constexpr int SYM_MIN_CORE = 64 * SYM_BLOCK;
 static_assert(SYM_THREADS >= 256, "the chunk reduction uses 256 threads (4 components x 64 sources)");
 
 __device__ __forceinline__ float rol1(float v) {  // lane l <- lane l+1 (mod 64)
@@ -358,8 +360,11 @@ __global__ __launch_bounds__(SYM_BLOCK) void kick_sym_diag(const uint4* __restri
 // superblock's 1024), all paired with the same source at every step, so one
 // rotation of the source-side sums serves NS pairs.
 template <int KERNEL>
+// blockIdx.y = q of gridDim.y round groups: workgroup (g, q) takes rounds
+// r0 + q, r0 + q + gridDim.y, ... < r1, so that small cores still fill the
+// GPU; group q adds its sink-side sums to plane q == 0 ? 0 : R + q.
 __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __restrict__ P, int n, int ny, int gs,
-                                                              int r0, int r1, KickConst kc,
+                                                              int r0, int r1, int R, KickConst kc,
                                                               double* __restrict__ part) {
     constexpr int NWAVE = SYM_THREADS / 64;
     __shared__ uint4 S[SYM_WAVES][128];
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __res
         zi[q] = me.z;
         sx[q] = sy[q] = sz[q] = sp[q] = cx[q] = cy[q] = cz[q] = cp[q] = 0.f;
     }
-    for (int r = r0; r < r1; ++r) {
+    for (int r = r0 + (int)blockIdx.y; r < r1; r += (int)gridDim.y) {
         if ((gs & 1) == 0 && r == gs / 2 && g >= gs / 2) continue;  // that pair belongs to g - gs/2
         const long J = (g + r) % gs;
         __syncthreads();
@@ -441,13 +446,14 @@ __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __res
             }
         }
     }
+    const long sp0 = blockIdx.y == 0 ? 0 : R + blockIdx.y;  // this group's sink-side plane
 #pragma unroll
     for (int q = 0; q < SYM_NS; ++q) {
         const long i = (long)g * SYM_BLOCK + (w * SYM_NS + q) * 64 + lane;
-        part[(0L * ny) * n + i] += (double)sx[q] + (double)cx[q];
-        part[(1L * ny) * n + i] += (double)sy[q] + (double)cy[q];
-        part[(2L * ny) * n + i] += (double)sz[q] + (double)cz[q];
-        part[(3L * ny) * n + i] += (double)sp[q] + (double)cp[q];
+        part[(0L * ny + sp0) * n + i] += (double)sx[q] + (double)cx[q];
+        part[(1L * ny + sp0) * n + i] += (double)sy[q] + (double)cy[q];
+        part[(2L * ny + sp0) * n + i] += (double)sz[q] + (double)cz[q];
+        part[(3L * ny + sp0) * n + i] += (double)sp[q] + (double)cp[q];
     }
 }
 
@@ -567,7 +573,9 @@ int symmetric_pass(Context& c, int c0, int m, bool plummer, const KickConst& kc,
     const size_t plane = sizeof(double) * 4 * (size_t)m;
     const int rcap = (int)std::max<size_t>(1, ((size_t)4 << 30) / plane - 1);
     const int R = std::max(1, std::min({16, rmax, rcap}));
-    const int ny = R + 1;
+    // round groups per launch: enough workgroups (>= 1024) for small cores
+    const int groups = std::max(1, std::min(R, (1024 + gs - 1) / gs));
+    const int ny = R + groups;  // sink-side planes 0 and R+1 .. R+groups-1, source-side planes 1 .. R
     CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, plane * ny, "kick partial sums"));
     CT2_TRY(ensure_scratch((void**)&g_packed, &g_packed_bytes, sizeof(uint4) * (size_t)m, "packed positions"));
     pack_positions<<<(m + 255) / 256, 256, 0, c.stream>>>(c.x + c0, c.y + c0, c.z + c0, m, g_packed);
@@ -583,9 +591,11 @@ int symmetric_pass(Context& c, int c0, int m, bool plummer, const KickConst& kc,
     for (int r0 = 1; r0 <= rmax; r0 += R) {
         const int r1 = std::min(r0 + R, rmax + 1);
         if (plummer)
-            kick_sym_pairs<CT2_KERNEL_PLUMMER><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1, kc, c.partial);
+            kick_sym_pairs<CT2_KERNEL_PLUMMER><<<dim3(gs, groups), SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1,
+                                                                                              R, kc, c.partial);
         else
-            kick_sym_pairs<CT2_KERNEL_CT2><<<gs, SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1, kc, c.partial);
+            kick_sym_pairs<CT2_KERNEL_CT2><<<dim3(gs, groups), SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1, R,
+                                                                                          kc, c.partial);
         CT2_LAUNCH_CHECK();
     }
     *out = PartialSums{c.partial, ny, c0, m};
@@ -688,8 +698,8 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
     // last m = floor(ns / 1024) * 1024 active sinks (the core [c0, n)); the
     // ordered kernel adds the core's pairs with the particles before it and
     // the few (< 1024) active sinks before the core against everything.  A
-    // core of fewer than SYM_MIN_CORE sinks has too few superblocks (one
-    // workgroup each) to fill the GPU, and the ordered kernel alone is faster.
+    // core of fewer than SYM_MIN_CORE sinks is latency-bound (a few rounds of
+    // 1024 x 1024 pairs per workgroup), and the ordered kernel alone is faster.
     PartialSums sets[3] = {{nullptr, 0, 0, 0}, {nullptr, 0, 0, 0}, {nullptr, 0, 0, 0}};
     int m = (naive || (flags & CT2_KICK_ORDERED)) ? 0 : ns / SYM_BLOCK * SYM_BLOCK;
     if (m < SYM_MIN_CORE && !(flags & CT2_KICK_SYMMETRIC)) m = 0;

@@ -102,7 +102,7 @@ typedef struct {
 #define CT2_KICK_NAIVE 0x1 /* plain FP32 accumulation instead of compensated */
 #define CT2_KICK_STORE 0x2 /* keep per-particle force and potential for ct2_forces_download */
 #define CT2_KICK_ORDERED 0x4 /* evaluate every ordered pair (no use of Newton's third law) */
-#define CT2_KICK_SYMMETRIC 0x8 /* use Newton's third law on any core of >= 1024 sinks (default: >= 2^19) */
+#define CT2_KICK_SYMMETRIC 0x8 /* use Newton's third law on any core of >= 1024 sinks (default: >= 2^16) */
 
 /* Replaces int kick(const kick_params&) (kick.hpp:24, kick.cpp:105-125):
  * all-pairs FP32 direct sum over the whole container with compensated
@@ -110,8 +110,8 @@ typedef struct {
  * *max_rung receives the return value of kick().  The sinks are the current
  * rung range [i0, n).  The last m = floor((n - i0) / 1024) * 1024 sinks form
  * a core whose unordered pairs are each evaluated once and applied to both
- * particles (Newton's third law), when m >= 2^19 (512 superblocks, enough
- * to fill the GPU) or CT2_KICK_SYMMETRIC is set, and neither CT2_KICK_NAIVE
+ * particles (Newton's third law), when m >= 2^16 (64 superblocks, below
+ * which the ordered kernel is faster) or CT2_KICK_SYMMETRIC is set, and neither CT2_KICK_NAIVE
  * nor CT2_KICK_ORDERED is; the core's pairs with the particles before it and
  * the remaining sinks are summed pair by pair.  Without a core every sink
  * sums over every source.  All paths are deterministic. */

@@ -178,7 +178,7 @@ def test_symmetric_and_ordered_kicks_vs_fp64(ct2, orc, n, kernel):
     launches of 16 rounds); for ragged n the ordered kernel adds the pairs of
     that core with the first n % 1024 particles and those particles' own
     sums.  CT2_KICK_SYMMETRIC forces that split at these sizes (by default a
-    core below 2^19 particles is left to the ordered kernel); the forced,
+    core below 2^16 particles is left to the ordered kernel); the forced,
     ordered and default paths against the FP64 oracle, on a clustered set."""
     x, y, z = ct2.ic_plummer_mix(n, n // 2, 4, 0.02, 0.2, n + kernel)
     load(ct2, x, y, z)
