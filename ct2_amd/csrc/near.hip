@@ -22,11 +22,13 @@
 //                 (rank by counting), so the summation order never depends on
 //                 atomics, and the start of each of the cell's NEAR_SUB slices
 //   near_items    the (row, chunk) of every work item
-//   near_force    one wave per work item: 64 sinks in registers, the 9
-//                 neighbouring rows' slice ranges streamed through LDS tiles,
-//                 the filtered kernel from cubic pieces in LDS, FP32 pair
-//                 terms with tile-level TwoSum; adds into the stored force of
-//                 each particle.
+//   near_force    one wave per work item: 64 sinks in registers, the 9 (or
+//                 25) neighbouring rows' slice ranges read 64 sources at a
+//                 time; sources farther than rcut from the sinks' bounding
+//                 box are dropped and the rest gathered in an LDS ring, 64
+//                 per tile; the filtered kernel from cubic pieces in LDS,
+//                 FP32 pair terms with tile-level TwoSum; adds into the
+//                 stored force of each particle.
 // Cells have side >= rcut, so every partner of a sink lies in the 3x3x3 cells
 // around it; a work item's sinks span a run of slices along z in one row, and
 // the union of their neighbourhoods is 9 contiguous (or wrapped) slice ranges
@@ -89,6 +91,7 @@ struct NearState {
 
 struct NearConst {
     float rc2u;      // rcut^2 in 2^-64 units
+    float rcu;       // rcut in 2^-32 units
     float eps2u;     // Plummer eps^2
     float h2u, hinvu, hinv3u;  // ct2 kernel
     float tscale;    // NEAR_NT / rcut^2: r^2 -> table coordinate
@@ -298,7 +301,6 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
                                                          double pscale, double pconst, float* __restrict__ fx,
                                                          float* __restrict__ fy, float* __restrict__ fz,
                                                          float* __restrict__ pot) {
-    __shared__ uint4 tile[NEAR_BLOCK];
     __shared__ float4 lk[NEAR_NT + 1], lp[NEAR_NT + 1];  // the kernel's cubic pieces
     __shared__ int rng[2 * NEAR_MAX_ROWS][2];
     const int tid = threadIdx.x;
@@ -348,48 +350,104 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
     const int i = s0 + tid;
     const uint4 me = P[i < s1 ? i : s1 - 1];
     float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f, cx_ = 0.f, cy_ = 0.f, cz_ = 0.f, cp_ = 0.f;
+    // The sinks' bounding box: per axis the offsets [L, H] about a centre cc
+    // (exact integers).  A source whose distance to the box is >= rcut has no
+    // partner among the sinks and is dropped as its tile is loaded.  The
+    // distance is squared and summed in the pair's own FP32 operation order
+    // on per-axis gaps e <= |dx|, so a source with any r2 < rc2u always
+    // passes.  An axis whose half-extent plus rcut nears half the box is not
+    // tested (there the source's offset from cc could wrap).
+    const uint4 ref = P[s0];
+    int lo[3] = {(int32_t)(me.x - ref.x), (int32_t)(me.y - ref.y), (int32_t)(me.z - ref.z)};
+    int hi[3] = {lo[0], lo[1], lo[2]};
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], m));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], m));
+        }
+    uint32_t cc[3];
+    long long L[3], H[3];
+    bool on[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long long mid = (long long)lo[a] + (((long long)hi[a] - (long long)lo[a]) >> 1);
+        cc[a] = (a == 0 ? ref.x : a == 1 ? ref.y : ref.z) + (uint32_t)mid;
+        L[a] = (long long)lo[a] - mid;
+        H[a] = (long long)hi[a] - mid;
+        on[a] = (double)(H[a] > -L[a] ? H[a] : -L[a]) + (double)q.rcu < 0x1p31 - 0x1p24;
+    }
+    __shared__ uint4 ring[2 * NEAR_BLOCK];  // survivors, consumed NEAR_BLOCK at a time from rb
+    int fill = 0, rb = 0;
+    float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
+    auto pair = [&](const uint4 s) {
+        const float dx = (float)(int32_t)(s.x - me.x);
+        const float dy = (float)(int32_t)(s.y - me.y);
+        const float dz = (float)(int32_t)(s.z - me.z);
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (r2 < q.rc2u) {  // a source out of range of the whole wave is skipped
+            float k, p;
+            near_pair<KERNEL>(r2, q, lk, lp, k, p);
+            tx = fmaf(dx, k, tx);
+            ty = fmaf(dy, k, ty);
+            tz = fmaf(dz, k, tz);
+            tp += p;
+        }
+    };
     for (int r = 0; r < 2 * nr; ++r) {
         const int a = rng[r][0], e = rng[r][1];
         for (int j0 = a; j0 < e; j0 += NEAR_BLOCK) {
-            const int cnt = min(NEAR_BLOCK, e - j0);
-            __syncthreads();
-            if (tid < cnt) tile[tid] = P[j0 + tid];
-            __syncthreads();
-            float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
-            auto pair = [&](const uint4 s) {
-                const float dx = (float)(int32_t)(s.x - me.x);
-                const float dy = (float)(int32_t)(s.y - me.y);
-                const float dz = (float)(int32_t)(s.z - me.z);
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (r2 < q.rc2u) {  // a source out of range of the whole wave is skipped
-                    float k, p;
-                    near_pair<KERNEL>(r2, q, lk, lp, k, p);
-                    tx = fmaf(dx, k, tx);
-                    ty = fmaf(dy, k, ty);
-                    tz = fmaf(dz, k, tz);
-                    tp += p;
+            __syncthreads();  // the ring's consumed half has been read
+            bool keep = false;
+            uint4 s = make_uint4(0u, 0u, 0u, 0u);
+            if (j0 + tid < e) {
+                s = P[j0 + tid];
+                const uint32_t sv[3] = {s.x, s.y, s.z};
+                float g[3];
+#pragma unroll
+                for (int a3 = 0; a3 < 3; ++a3) {
+                    const long long d = (int32_t)(sv[a3] - cc[a3]);
+                    long long gap = L[a3] - d > d - H[a3] ? L[a3] - d : d - H[a3];
+                    gap = gap < 0 ? 0 : (gap > 0x7fffffffLL ? 0x7fffffffLL : gap);
+                    g[a3] = on[a3] ? (float)(int32_t)gap : 0.f;
                 }
-            };
-            if (cnt == NEAR_BLOCK) {
-                // four sources read together: the in-range branch inside pair()
-                // would otherwise keep each read (and its wait) next to its use
+                keep = fmaf(g[2], g[2], fmaf(g[1], g[1], g[0] * g[0])) < q.rc2u;
+            }
+            const uint64_t m = __ballot(keep);
+            const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (keep) ring[(rb + fill + below) & (2 * NEAR_BLOCK - 1)] = s;
+            fill += (int)__popcll(m);
+            if (fill >= NEAR_BLOCK) {
+                __syncthreads();
+                const uint4* T = ring + rb;
 #pragma unroll 2
                 for (int j = 0; j < NEAR_BLOCK; j += 4) {
-                    const uint4 s0 = tile[j], s1 = tile[j + 1], s2 = tile[j + 2], s3 = tile[j + 3];
-                    pair(s0);
-                    pair(s1);
-                    pair(s2);
-                    pair(s3);
+                    // four sources read together: the in-range branch inside
+                    // pair() would otherwise keep each read next to its use
+                    const uint4 u0 = T[j], u1 = T[j + 1], u2 = T[j + 2], u3 = T[j + 3];
+                    pair(u0);
+                    pair(u1);
+                    pair(u2);
+                    pair(u3);
                 }
-            } else {
-                for (int j = 0; j < cnt; ++j) pair(tile[j]);
+                two_sum(sx, cx_, tx);
+                two_sum(sy, cy_, ty);
+                two_sum(sz, cz_, tz);
+                two_sum(sp, cp_, tp);
+                tx = ty = tz = tp = 0.f;
+                rb ^= NEAR_BLOCK;
+                fill -= NEAR_BLOCK;
             }
-            two_sum(sx, cx_, tx);
-            two_sum(sy, cy_, ty);
-            two_sum(sz, cz_, tz);
-            two_sum(sp, cp_, tp);
         }
     }
+    __syncthreads();
+    for (int j = 0; j < fill; ++j) pair(ring[rb + j]);
+    two_sum(sx, cx_, tx);
+    two_sum(sy, cy_, ty);
+    two_sum(sz, cz_, tz);
+    two_sum(sp, cp_, tp);
     if (i < s1) {
         const uint32_t p = me.w;
         fx[p] += (float)(((double)sx + (double)cx_) * fscale);
@@ -564,6 +622,7 @@ int near_forces(double rs) {
     const double rsu = rs * u, rcu = rcut * u, h = c.opts.hsoft, hu = h * u;
     NearConst q;
     q.rc2u = (float)(rcu * rcu);
+    q.rcu = (float)rcu;
     q.eps2u = (float)(hu * hu);
     q.h2u = (float)(hu * hu);
     q.hinvu = (float)(1.0 / hu);
