@@ -511,10 +511,15 @@ __global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const uint32_t* __re
                                                            const bf16_t* __restrict__ W3p, const float* __restrict__ w,
                                                            long r0, int m, float unit, float* __restrict__ fx,
                                                            float* __restrict__ fy, float* __restrict__ fz) {
-    __shared__ uint4 sW2[HID * HID / 8];  // 128 KB
-    for (int e = threadIdx.x; e < HID * HID / 8; e += INF_WAVES * 64) sW2[e] = ((const uint4*)W2p)[e];
+    // W2 in LDS (128 KB): 16-byte chunk j of row n at chunk j ^ (n & 15) of
+    // the row, so that the 16 rows a lane group reads together (rows 512 B
+    // apart) fall on 16 different 4-bank groups instead of the same one
+    __shared__ uint4 sW2[HID * HID / 8];
+    for (int e = threadIdx.x; e < HID * HID / 8; e += INF_WAVES * 64) {
+        const int n = e / (HID / 8), j = e % (HID / 8);
+        sW2[n * (HID / 8) + (j ^ (n & 15))] = ((const uint4*)W2p)[e];
+    }
     __syncthreads();
-    const bf16_t* sw2 = (const bf16_t*)sW2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
     const float* b1 = w + OFF_B1;
     const float* b2 = w + OFF_B2;
@@ -593,7 +598,7 @@ __global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const uint32_t* __re
         for (int ks = 0; ks < HID / 32; ++ks) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const bf16x8 a = load8(sw2 + (16 * r + l16) * HID + 32 * ks + 8 * g);
+                const bf16x8 a = __builtin_bit_cast(bf16x8, sW2[(16 * r + l16) * (HID / 8) + ((4 * ks + g) ^ l16)]);
 #pragma unroll
                 for (int c = 0; c < INF_CB; ++c)
                     acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][ks]),

@@ -1,8 +1,9 @@
 """GPU near field of the PM force (near.hip) against the FP64 oracle and the
 periodic FP64 judge: ragged and clustered particle sets (cells holding more
 than one tile and more than one LDS key chunk), both pair kernels, a
-neighbourhood that covers whole rows (3 cells per side), determinism and the
-argument checks."""
+neighbourhood that covers whole rows (3 cells per side), sparse sets whose
+work items span whole columns (the sinks' bounding-box test then skips z, or
+every axis), determinism and the argument checks."""
 import numpy as np
 import pytest
 
@@ -45,13 +46,19 @@ CASES = [
     ("uniform_ragged", "uniform", 64, 2.0, 6.0, 0, 5),
     ("clustered_3cells", "clustered", 32, 2.0, 5.0, 0, 3),
     ("clustered_ct2_kernel", "clustered", 64, 1.5, 6.0, 1, 7),
+    # ~33 particles per (x, y) column: a work item's sinks span the whole
+    # column, so the bounding-box test skips z; with rcut = 0.331 box it skips
+    # x and y too (half-extent + rcut near half the box)
+    ("sparse_whole_columns", "sparse", 32, 2.0, 5.0, 0, 3),
+    ("sparse_no_box_test", "sparse", 32, 2.0, 5.3, 0, 3),
 ]
 
 
 def build(ct2, kind):
-    if kind == "uniform":
+    if kind in ("uniform", "sparse"):
         rng = np.random.default_rng(11)
-        return tuple(rng.integers(0, TWO32, 4099, dtype=np.uint64).astype(np.uint32) for _ in range(3))
+        n = 4099 if kind == "uniform" else 300
+        return tuple(rng.integers(0, TWO32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3))
     return ct2.ic_plummer_mix(24000, 16000, 4, 0.01, 0.1, 5)
 
 
@@ -66,7 +73,7 @@ def test_near_field_vs_fp64_oracle(ct2, orc, name, kind, ng, rs_cells, rcut_rs, 
     tot, near = near_only(ct2, rcut_rs)
     assert ct2.pm_near_cells() == ncell
     rs = rs_cells / ng
-    sinks = np.random.default_rng(2).choice(n, 2000, replace=False)
+    sinks = np.random.default_rng(2).choice(n, min(n, 2000), replace=False)
     want = np.stack(orc.pm_near(x, y, z, gm, soft, rs, rcut_rs * rs, kernel, sinks=sinks))
     # the FP32 near-field sum against FP64, on the scale of the total force
     e = np.linalg.norm(near[:3, sinks] - want[:3], axis=0) / np.linalg.norm(tot[:3, sinks], axis=0)
