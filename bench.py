@@ -413,7 +413,10 @@ def main() -> int:
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4), "traffic": KICK_TRAFFIC_BYTES.get(side),
                      "kernel": "kick_sym_pairs (+kick_sym_diag, kick_finalize): each pair evaluated once",
                      "flop_per_interaction": FLOP_PER_INTERACTION, "kick_ms": round(kick_avg, 3),
-                     "traffic_per": "kick_sym_pairs launch (64 per kick); achieved = 20 N^2 / kick time"},
+                     "traffic_per": "kick_sym_pairs launch (64 per kick); achieved = 20 N^2 / kick time",
+                     # information only: the same kick at 38 flop per interaction (the
+                     # Warren et al. 1997 / Hamada et al. 2009 count)
+                     "info_38flop_tflops": round(achieved_tflops * 38 / FLOP_PER_INTERACTION, 3)},
         "fft_fp32": fft,
         "drift": {"ms": round(drift_avg, 4), "GB/s": round(36 * n / drift_avg / 1e6, 1),
                   "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},
