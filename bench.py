@@ -106,7 +106,7 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
     periodic FP64 direct sum judging `judge` particles each step.  method
     "pm_mlp": CIC PM (split 1.25 cells) + the learned correction trained on
     this configuration's distribution (the force the configuration names);
-    "pm_near": TSC PM (split 2.5 cells) + the near field to 5.5 split radii.
+    "pm_near": TSC PM (split 2 cells) + the near field to 5.5 split radii.
     The direct kick at this N is not run (N^2 / rate at the measured rate
     would take minutes per step); its time is derived."""
     from ct2_amd import stepping
@@ -120,7 +120,7 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
         rec = stepping.run(side, 2 * side, method="pm_mlp", weights=trained[0], rs_cells=1.25, near=0.0, order=1,
                            **kw)
     else:
-        rec = stepping.run(side, 2 * side, method="pm", rs_cells=2.5, near=5.5, order=2, **kw)
+        rec = stepping.run(side, 2 * side, method="pm", rs_cells=2.0, near=5.5, order=2, **kw)
     n = side ** 3
     ms = [r["force_kick_ms"] for r in rec["records"]][1:]  # the first call includes one-time set-up
     med = [r["median_rel_err"] for r in rec["records"]]
