@@ -13,14 +13,15 @@
 // PM + near field tends to the periodic direct sum.
 //
 // Pipeline (all on the library stream, deterministic):
-//   near_bin      cell of each particle, per-cell counts and slots (one atomic
-//                 per run of equal cells in a wave)
-//   near_scan     exclusive scan of the counts -> cell starts; again over the
-//                 rows -> work-item prefix (chunks of NEAR_BLOCK sinks per row)
-//   near_scatter  (x, y, z, index) into its cell's segment (slot order)
-//   near_order    each cell's segment sorted by (z slice, particle index)
-//                 (rank by counting), so the summation order never depends on
-//                 atomics, and the start of each of the cell's NEAR_SUB slices
+//   near_key      slice key of each particle: (x, y) column, then z slice
+//                 (NEAR_SUB slices per cell), and its index
+//   radix sort    hipCUB's stable LSD sort of (key, index): the order is
+//                 (column, slice, particle index) for any scheduling and any
+//                 clustering, so the summation order never depends on atomics
+//   near_bounds   start of every slice and cell (a binary search each)
+//   near_scan     exclusive scan over the rows -> work-item prefix (chunks of
+//                 NEAR_BLOCK sinks per row)
+//   near_gather   the sorted (x, y, z, index)
 //   near_items    the (row, chunk) of every work item
 //   near_force    one wave per work item: 64 sinks in registers, the 9 (or
 //                 25) neighbouring rows' slice ranges read 64 sources at a
@@ -33,12 +34,14 @@
 // around it; a work item's sinks span a run of slices along z in one row, and
 // the union of their neighbourhoods is 9 contiguous (or wrapped) slice ranges
 // of the sorted array, each widened by the slices rcut spans.
+#include <algorithm>
 #include <cmath>
 #include <utility>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "internal.h"
-#include "wave.h"
 
 namespace ct2 {
 namespace {
@@ -74,12 +77,13 @@ struct NearState {
     double rcut_rs = 0.0;  // 0 = off
     int nc = 0;
     size_t cap = 0;                // particles the buffers hold
-    int* cell = nullptr;           // [cap]
-    int* slot = nullptr;           // [cap]
-    uint4* P0 = nullptr;           // [cap] scattered (x, y, z, index)
-    uint4* P = nullptr;            // [cap] sorted by (cell, index)
+    uint32_t* key = nullptr;       // [cap] slice key of each particle
+    int* idx = nullptr;            // [cap] particle indices (sort values)
+    uint4* P0 = nullptr;           // [cap] sorted keys and indices (8 of its 16 B per particle)
+    uint4* P = nullptr;            // [cap] sorted (x, y, z, index)
+    void* sort_tmp = nullptr;      // radix sort scratch
+    size_t sort_tmp_bytes = 0;
     size_t ncell_cap = 0, nrow_cap = 0;  // cells and (x, y) columns the index buffers hold
-    int* count = nullptr;          // [nc^3]
     int* start = nullptr;          // [nc^3 + 1]
     int* row_chunk = nullptr;      // [nc^2 + 1]
     int* subst = nullptr;          // [nc^3 NEAR_SUB + 1] sorted index of each slice's first particle
@@ -101,37 +105,64 @@ __device__ __forceinline__ uint32_t cell_of(uint32_t x, uint32_t nc) {
     return (uint32_t)(((uint64_t)x * nc) >> 32);
 }
 
-__global__ __launch_bounds__(256) void fill_int(int* __restrict__ p, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0;
+// z slice of a particle along its row: u = floor(z nzs / 2^32), nzs = nc NEAR_SUB
+// slices per row; u / NEAR_SUB is the cell along z.
+__device__ __forceinline__ uint32_t slice_of(uint32_t z, uint32_t nzs) {
+    return (uint32_t)(((uint64_t)z * nzs) >> 32);
 }
 
-// cells: nxy x nxy columns in (x, y), nc along z; index (cx nxy + cy) nc + kz
-__global__ __launch_bounds__(256) void near_bin(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
-                                                const uint32_t* __restrict__ Z, int n, uint32_t nxy, uint32_t nc,
-                                                int* __restrict__ count, int* __restrict__ cell,
-                                                int* __restrict__ slot) {
+// Sort key of each particle: its (x, y) column row = cx nxy + cy and its z
+// slice u along the row, key = row nzs + u -- the index of its slice in
+// subst (cells: (row nc + u / NEAR_SUB) = key / NEAR_SUB).
+__global__ __launch_bounds__(256) void near_key(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                const uint32_t* __restrict__ Z, int n, uint32_t nxy, uint32_t nzs,
+                                                uint32_t* __restrict__ key, int* __restrict__ idx) {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    int c = -1;
-    if (p < n) {
-        c = (int)((cell_of(X[p], nxy) * nxy + cell_of(Y[p], nxy)) * nc + cell_of(Z[p], nc));
-        cell[p] = c;
-    }
-    const int k = run_increment(count, c);  // every lane of the wave takes part
-    if (p < n) slot[p] = k;
+    if (p >= n) return;
+    key[p] = (cell_of(X[p], nxy) * nxy + cell_of(Y[p], nxy)) * nzs + slice_of(Z[p], nzs);
+    idx[p] = p;
 }
 
-// Exclusive scan of len values (one workgroup): out[0..len], out[len] = total.
-// mode 0: values are in[i]; mode 1: values are ceil(row count / NEAR_BLOCK)
-// of row i, row count = in[(i+1)*nc] - in[i*nc] (in = cell starts).
-__global__ __launch_bounds__(SCAN_THREADS) void near_scan(const int* __restrict__ in, int len, int mode, int nc,
+// After the stable sort by key: subst[q] = first sorted position with key >=
+// q (q = 0 .. nkeys, subst[nkeys] = n), and the cell starts start[c] =
+// subst[c NEAR_SUB].  One binary search per slice, so empty slices cost the
+// same as full ones.
+__global__ __launch_bounds__(256) void near_bounds(const uint32_t* __restrict__ key, int n, int nkeys,
+                                                   int* __restrict__ subst, int* __restrict__ start) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q > nkeys) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (key[mid] < (uint32_t)q)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    subst[q] = lo;
+    if (q % NEAR_SUB == 0) start[q / NEAR_SUB] = lo;
+}
+
+// The sorted particles: P[i] = (x, y, z, index) of sorted entry i.
+__global__ __launch_bounds__(256) void near_gather(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                   const uint32_t* __restrict__ Z, const int* __restrict__ idx, int n,
+                                                   uint4* __restrict__ P) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int p = idx[i];
+    P[i] = make_uint4(X[p], Y[p], Z[p], (uint32_t)p);
+}
+
+// Exclusive scan (one workgroup) of the work items per row: out[0..len],
+// out[len] = total; row i holds ceil(m / NEAR_BLOCK) items, m = in[(i+1)*nc] -
+// in[i*nc] (in = cell starts).
+__global__ __launch_bounds__(SCAN_THREADS) void near_scan(const int* __restrict__ in, int len, int nc,
                                                           int* __restrict__ out) {
     __shared__ int part[SCAN_THREADS];
     const int t = threadIdx.x;
     const int per = (len + SCAN_THREADS - 1) / SCAN_THREADS;
     const int b = min(len, t * per), e = min(len, b + per);
     auto value = [&](int i) -> int {
-        if (mode == 0) return in[i];
         const int m = in[(i + 1) * nc] - in[i * nc];
         return (m + NEAR_BLOCK - 1) / NEAR_BLOCK;
     };
@@ -151,94 +182,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void near_scan(const int* __restrict_
         run += value(i);
     }
     if (t == SCAN_THREADS - 1) out[len] = part[t];
-}
-
-__global__ __launch_bounds__(256) void near_scatter(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
-                                                    const uint32_t* __restrict__ Z, int n,
-                                                    const int* __restrict__ cell, const int* __restrict__ slot,
-                                                    const int* __restrict__ start, uint4* __restrict__ P0) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    P0[start[cell[p]] + slot[p]] = make_uint4(X[p], Y[p], Z[p], (uint32_t)p);
-}
-
-// z slice of a particle along its row: u = floor(z nzs / 2^32), nzs = nc NEAR_SUB
-// slices per row; u / NEAR_SUB is the cell along z.
-__device__ __forceinline__ uint32_t slice_of(uint32_t z, uint32_t nzs) {
-    return (uint32_t)(((uint64_t)z * nzs) >> 32);
-}
-
-// One workgroup per cell: element e goes to rank = #{f in cell : key_f < key_e},
-// key = (slice, particle index); then the cell's slice starts.  Cells of up to
-// ORDER_LDS particles are bucketed by slice in LDS first, so an element is
-// ranked only against its own slice; larger cells compare whole keys.
-constexpr int ORDER_LDS = 4096;
-__global__ __launch_bounds__(256) void near_order(const uint4* __restrict__ P0, const int* __restrict__ start,
-                                                  uint32_t nzs, int ncell, uint4* __restrict__ P,
-                                                  int* __restrict__ subst) {
-    __shared__ uint32_t bucket[ORDER_LDS];  // particle indices grouped by slice (large cells: keys' low words)
-    __shared__ uint32_t khi[1024];          // large cells: keys' slices
-    __shared__ int cnt[NEAR_SUB], off[NEAR_SUB], fill[NEAR_SUB];
-    const int c = blockIdx.x;
-    const int base = start[c], m = start[c + 1] - base;
-    const int tid = threadIdx.x;
-    if (tid < NEAR_SUB) {
-        cnt[tid] = 0;
-        fill[tid] = 0;
-    }
-    __syncthreads();
-    for (int e = tid; e < m; e += 256) atomicAdd(&cnt[slice_of(P0[base + e].z, nzs) % NEAR_SUB], 1);
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int q = 0; q < NEAR_SUB; ++q) {
-            off[q] = run;
-            subst[c * NEAR_SUB + q] = base + run;
-            run += cnt[q];
-        }
-        if (c == ncell - 1) subst[ncell * NEAR_SUB] = base + run;
-    }
-    __syncthreads();
-    if (m <= ORDER_LDS) {
-        for (int e = tid; e < m; e += 256) {
-            const uint4 v = P0[base + e];
-            const uint32_t q = slice_of(v.z, nzs) % NEAR_SUB;
-            bucket[off[q] + atomicAdd(&fill[q], 1)] = v.w;
-        }
-        __syncthreads();
-        for (int e = tid; e < m; e += 256) {
-            const uint4 v = P0[base + e];
-            const uint32_t q = slice_of(v.z, nzs) % NEAR_SUB;
-            const int b = off[q], nb = cnt[q];
-            int rank = 0;
-            for (int f = 0; f < nb; ++f) rank += bucket[b + f] < v.w;
-            P[base + b + rank] = v;
-        }
-        return;
-    }
-    for (int e0 = 0; e0 < m; e0 += 256) {
-        const int e = e0 + tid;
-        uint4 mine = make_uint4(0u, 0u, 0u, 0u);
-        uint32_t mq = 0;
-        if (e < m) {
-            mine = P0[base + e];
-            mq = slice_of(mine.z, nzs) % NEAR_SUB;
-        }
-        int rank = 0;
-        for (int f0 = 0; f0 < m; f0 += 1024) {
-            const int cntf = min(1024, m - f0);
-            __syncthreads();
-            for (int f = tid; f < cntf; f += 256) {
-                const uint4 v = P0[base + f0 + f];
-                khi[f] = slice_of(v.z, nzs) % NEAR_SUB;
-                bucket[f] = v.w;
-            }
-            __syncthreads();
-            if (e < m)
-                for (int f = 0; f < cntf; ++f) rank += khi[f] < mq || (khi[f] == mq && bucket[f] < mine.w);
-        }
-        if (e < m) P[base + rank] = mine;
-    }
 }
 
 __device__ __forceinline__ void two_sum(float& s, float& c, float v) {
@@ -458,11 +401,13 @@ __global__ __launch_bounds__(NEAR_BLOCK) void near_force(const uint4* __restrict
 }
 
 void free_near_buffers() {
-    dev_free(g_near.cell);
-    dev_free(g_near.slot);
+    dev_free(g_near.key);
+    dev_free(g_near.idx);
     dev_free(g_near.P0);
     dev_free(g_near.P);
-    dev_free(g_near.count);
+    dev_free(g_near.sort_tmp);
+    g_near.sort_tmp = nullptr;
+    g_near.sort_tmp_bytes = 0;
     dev_free(g_near.start);
     dev_free(g_near.row_chunk);
     dev_free(g_near.subst);
@@ -471,9 +416,10 @@ void free_near_buffers() {
     dev_free(g_near.table);
     g_near.table = nullptr;
     g_near.table_rsu = g_near.table_rcu = 0.0;
-    g_near.cell = g_near.slot = nullptr;
+    g_near.key = nullptr;
+    g_near.idx = nullptr;
     g_near.P0 = g_near.P = nullptr;
-    g_near.count = g_near.start = g_near.row_chunk = nullptr;
+    g_near.start = g_near.row_chunk = nullptr;
     g_near.item = nullptr;
     g_near.item_cap = 0;
     g_near.cap = 0;
@@ -482,27 +428,26 @@ void free_near_buffers() {
 
 int ensure_buffers(size_t n, size_t ncell, size_t nrow) {
     if (n > g_near.cap) {
-        dev_free(g_near.cell);
-        dev_free(g_near.slot);
+        dev_free(g_near.key);
+        dev_free(g_near.idx);
         dev_free(g_near.P0);
         dev_free(g_near.P);
-        g_near.cell = g_near.slot = nullptr;
+        g_near.key = nullptr;
+        g_near.idx = nullptr;
         g_near.P0 = g_near.P = nullptr;
         g_near.cap = 0;
-        CT2_TRY(dev_alloc((void**)&g_near.cell, n * sizeof(int), "near-field cells"));
-        CT2_TRY(dev_alloc((void**)&g_near.slot, n * sizeof(int), "near-field slots"));
-        CT2_TRY(dev_alloc((void**)&g_near.P0, n * sizeof(uint4), "near-field scatter"));
+        CT2_TRY(dev_alloc((void**)&g_near.key, n * sizeof(uint32_t), "near-field sort keys"));
+        CT2_TRY(dev_alloc((void**)&g_near.idx, n * sizeof(int), "near-field sort values"));
+        CT2_TRY(dev_alloc((void**)&g_near.P0, n * sizeof(uint4), "near-field sorted keys"));
         CT2_TRY(dev_alloc((void**)&g_near.P, n * sizeof(uint4), "near-field sorted particles"));
         g_near.cap = n;
     }
     if (ncell > g_near.ncell_cap || nrow > g_near.nrow_cap) {
-        dev_free(g_near.count);
         dev_free(g_near.start);
         dev_free(g_near.row_chunk);
         dev_free(g_near.subst);
-        g_near.count = g_near.start = g_near.row_chunk = g_near.subst = nullptr;
+        g_near.start = g_near.row_chunk = g_near.subst = nullptr;
         g_near.ncell_cap = g_near.nrow_cap = 0;
-        CT2_TRY(dev_alloc((void**)&g_near.count, ncell * sizeof(int), "near-field counts"));
         CT2_TRY(dev_alloc((void**)&g_near.start, (ncell + 1) * sizeof(int), "near-field cell starts"));
         CT2_TRY(dev_alloc((void**)&g_near.row_chunk, (nrow + 1) * sizeof(int), "near-field work items"));
         CT2_TRY(dev_alloc((void**)&g_near.subst, (ncell * NEAR_SUB + 1) * sizeof(int), "near-field slice starts"));
@@ -604,18 +549,27 @@ int near_forces(double rs) {
     g_near.nc = nc;
     const unsigned pb = (unsigned)((n + 255) / 256);
     hipStream_t st = c.stream;
-    fill_int<<<(c3 + 255) / 256, 256, 0, st>>>(g_near.count, c3);
+    // (slice key, particle index) pairs sorted stably by key: the order is
+    // (row, z slice, particle index) whatever the scheduling
+    const uint32_t nzs = (uint32_t)(nc * NEAR_SUB);
+    const int nkeys = c3 * NEAR_SUB;  // = nrow nzs < 2^25
+    int bits = 1;
+    while ((1 << bits) < nkeys) ++bits;
+    uint32_t* kin = g_near.key;
+    int* vin = g_near.idx;
+    uint32_t* kout = (uint32_t*)g_near.P0;  // P0 (16 B per particle) holds the sorted keys and indices
+    int* vout = (int*)g_near.P0 + n;
+    near_key<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, (uint32_t)nxy, nzs, kin, vin);
     CT2_LAUNCH_CHECK();
-    near_bin<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, (uint32_t)nxy, (uint32_t)nc, g_near.count, g_near.cell,
-                                 g_near.slot);
+    size_t tmp = 0;
+    CT2_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, n, 0, bits, st));
+    CT2_TRY(ensure_scratch(&g_near.sort_tmp, &g_near.sort_tmp_bytes, std::max<size_t>(tmp, 256), "near-field sort"));
+    CT2_HIP(hipcub::DeviceRadixSort::SortPairs(g_near.sort_tmp, tmp, kin, kout, vin, vout, n, 0, bits, st));
+    near_bounds<<<(unsigned)((nkeys + 256) / 256), 256, 0, st>>>(kout, n, nkeys, g_near.subst, g_near.start);
     CT2_LAUNCH_CHECK();
-    near_scan<<<1, SCAN_THREADS, 0, st>>>(g_near.count, c3, 0, nc, g_near.start);
+    near_scan<<<1, SCAN_THREADS, 0, st>>>(g_near.start, nrow, nc, g_near.row_chunk);
     CT2_LAUNCH_CHECK();
-    near_scan<<<1, SCAN_THREADS, 0, st>>>(g_near.start, nrow, 1, nc, g_near.row_chunk);
-    CT2_LAUNCH_CHECK();
-    near_scatter<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, g_near.cell, g_near.slot, g_near.start, g_near.P0);
-    CT2_LAUNCH_CHECK();
-    near_order<<<c3, 256, 0, st>>>(g_near.P0, g_near.start, (uint32_t)(nc * NEAR_SUB), c3, g_near.P, g_near.subst);
+    near_gather<<<pb, 256, 0, st>>>(c.x, c.y, c.z, vout, n, g_near.P);
     CT2_LAUNCH_CHECK();
 
     const double u = 0x1p32;

@@ -3,7 +3,8 @@ periodic FP64 judge: ragged and clustered particle sets (cells holding more
 than one tile and more than one LDS key chunk), both pair kernels, a
 neighbourhood that covers whole rows (3 cells per side), sparse sets whose
 work items span whole columns (the sinks' bounding-box test then skips z, or
-every axis), determinism and the argument checks."""
+every axis), a single sphere packing most particles into one cell,
+determinism and the argument checks."""
 import numpy as np
 import pytest
 
@@ -51,6 +52,9 @@ CASES = [
     # x and y too (half-extent + rcut near half the box)
     ("sparse_whole_columns", "sparse", 32, 2.0, 5.0, 0, 3),
     ("sparse_no_box_test", "sparse", 32, 2.0, 5.3, 0, 3),
+    # one compact sphere: ~15,000 particles share a cell, so the (cell, slice,
+    # index) order of a very full cell is exercised
+    ("dense_cell", "dense", 32, 2.0, 5.0, 0, 3),
 ]
 
 
@@ -59,6 +63,8 @@ def build(ct2, kind):
         rng = np.random.default_rng(11)
         n = 4099 if kind == "uniform" else 300
         return tuple(rng.integers(0, TWO32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3))
+    if kind == "dense":
+        return ct2.ic_plummer_mix(20000, 16000, 1, 0.003, 0.1, 7)
     return ct2.ic_plummer_mix(24000, 16000, 4, 0.01, 0.1, 5)
 
 
