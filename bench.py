@@ -184,6 +184,48 @@ def train_leg(api, ev: HipEvents, n: int) -> list:
     return out
 
 
+def pct(e) -> dict:
+    return {"p50": float(np.percentile(e, 50)), "p90": float(np.percentile(e, 90)), "p99": float(np.percentile(e, 99))}
+
+
+def precision_leg(api, ev: HipEvents, side: int, reps: int) -> dict:
+    """The clustered precision study: side^3 particles, half drawn from 64
+    Plummer spheres (scale radius 0.01, truncated at 0.1, seeded centres) and
+    half uniform, eps = 2e-4.  The FP32 kick with compensated and with naive
+    accumulation against the FP64 direct sum (minimum image) on 4,096 sampled
+    sinks; the PM force on a 512^3 grid (FP64 Green's table, FP32 FFTs) alone
+    and with the near field against the periodic FP64 direct sum on the same
+    sinks.  Replaces the particle container."""
+    from ct2_amd import _lib
+    lib = _lib.load()
+    n = side ** 3
+    x, y, z = api.ic_plummer_mix(n, n // 2, 64, 0.01, 0.1, 3)
+    api.particle_allocate(n)
+    api.particles_upload(x, y, z)
+    api.set_force_options(2e-4, 1.0 / n, api.KERNEL_PLUMMER)
+    idx = np.random.default_rng(7).choice(n, 4096, replace=False)
+    Jm = np.stack(api.kick_sample_fp64(idx))
+    out = {"n": n, "spheres": 64, "scale_radius": 0.01, "truncation_radius": 0.1, "eps": 2e-4, "sinks": 4096}
+    for name, flag in (("fp32_compensated", 0), ("fp32_naive", api.KICK_NAIVE)):
+        ms = float(time_calls(ev, lambda: api.kick(None, flag | api.KICK_STORE), 1)[0])
+        fx, fy, fz, pot = api.forces_download()
+        e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), Jm[:3])
+        out[name] = {"ms": round(ms, 1), **pct(e), "pot_p50": float(np.median(np.abs(pot[idx] / Jm[3] - 1)))}
+    J = np.stack(api.kick_sample_fp64(idx, periodic=True))
+    for name, order, rs, rc in (("pm_cic_512", 1, 1.25, 0.0), ("pm_tsc_512_near", 2, 2.0, 5.5)):
+        api.pm_assignment(order)
+        api.pm_init(4 * side, rs)
+        api.pm_near(rc)
+        _lib.check(lib.ct2_pm_forces())
+        ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), reps)))
+        fx, fy, fz, _ = api.forces_download()
+        e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
+        out[name] = {"pm_grid": 4 * side, "rs_cells": rs, "rcut_rs": rc, "ms": round(ms, 3), **pct(e)}
+        api.pm_near(0.0)
+        api.pm_assignment(1)
+    return out
+
+
 def subset_leg(api, ev: HipEvents, n: int, params, reps: int) -> list:
     """Kicks of a rung subset, as kick_step makes them (main.cpp:50-68):
     particles on rung >= 1 are partitioned to the end (particle_sort_by_rung)
@@ -384,6 +426,8 @@ def main() -> int:
     train = train_leg(api, ev, n)
     print("[bench] rung-subset kicks ...", file=sys.stderr, flush=True)
     subset = subset_leg(api, ev, n, params, args.reps)
+    print("[bench] clustered precision study ...", file=sys.stderr, flush=True)
+    precision = precision_leg(api, ev, side, args.reps)
     steps_res = None
     if args.stepping_side:
         print("[bench] stepping configuration ...", file=sys.stderr, flush=True)
@@ -423,6 +467,7 @@ def main() -> int:
         "pm_mlp_vs_direct": pmres,
         "pm_near_vs_direct": nearres,
         "rung_subset_kicks": subset,
+        "clustered_precision": precision,
         "mlp_training": train,
         "stepping": steps_res,
     }
