@@ -184,6 +184,38 @@ def train_leg(api, ev: HipEvents, n: int) -> list:
     return out
 
 
+def heldout_mlp_leg(api, side: int, ng: int, seed: int = 6) -> dict:
+    """The trained correction on particles it never saw: a lattice +
+    Gaussian(0.2 cell) set of a seed outside the training seeds (1-5), PM and
+    PM + correction against the periodic FP64 direct sum on 4,096 sinks.
+    Replaces the particle container."""
+    from ct2_amd import _lib
+    from ct2_amd.train_mlp import DEFAULT_OUT, load_weights
+
+    trained = load_weights(DEFAULT_OUT)
+    if trained is None:
+        return {"skipped": "no trained weights"}
+    lib = _lib.load()
+    n = side ** 3
+    x, y, z = api.ic_lattice(side, 0.2, seed)
+    api.particle_allocate(n)
+    api.particles_upload(x, y, z)
+    api.set_force_options(1.0 / (50 * side), 1.0 / n, api.KERNEL_PLUMMER)
+    idx = np.random.default_rng(seed).choice(n, 4096, replace=False)
+    J = np.stack(api.kick_sample_fp64(idx, periodic=True))
+    api.pm_init(ng, 1.25)
+    api.mlp_init(1)
+    api.mlp_set_weights(trained[0])
+    out = {"seed": seed, "training_seeds": trained[1]["seeds"], "pm_grid": ng}
+    for name, fn in (("pm", lib.ct2_pm_forces), ("pm_mlp", lib.ct2_pm_mlp_forces)):
+        _lib.check(fn())
+        fx, fy, fz, _ = api.forces_download()
+        e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
+        out[name + "_median_rel_err"] = float(np.median(e))
+        out[name + "_p90_rel_err"] = float(np.percentile(e, 90))
+    return out
+
+
 def pct(e) -> dict:
     return {"p50": float(np.percentile(e, 50)), "p90": float(np.percentile(e, 90)), "p99": float(np.percentile(e, 99))}
 
@@ -428,6 +460,7 @@ def main() -> int:
     subset = subset_leg(api, ev, n, params, args.reps)
     print("[bench] clustered precision study ...", file=sys.stderr, flush=True)
     precision = precision_leg(api, ev, side, args.reps)
+    pmres["held_out"] = heldout_mlp_leg(api, side, args.pm_ng)
     steps_res = None
     if args.stepping_side:
         print("[bench] stepping configuration ...", file=sys.stderr, flush=True)
