@@ -433,6 +433,22 @@ def main() -> int:
                           "note": "kick only"}), flush=True)
         return 0
 
+    # the host boundary: the container upload (25 B per particle, pageable host
+    # arrays) and the force download (16 B per particle), synchronous, wall clock
+    zf, zr = np.zeros(n, np.float32), np.zeros(n, np.int8)
+    t_up = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        api.particles_upload(x, y, z, zf, zf, zf, zr)
+        t_up.append((time.perf_counter() - t1) * 1e3)
+    t_down = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        api.forces_download()
+        t_down.append((time.perf_counter() - t1) * 1e3)
+    host_io = {"upload_ms": round(float(np.median(t_up)), 3), "upload_bytes": 25 * n,
+               "download_forces_ms": round(float(np.median(t_down)), 3), "download_bytes": 16 * n}
+
     # drift: 36 B/particle
     drift_ms = time_calls(ev, lambda: api.drift(1e-9, 1.0), 20)
     drift_avg = float(np.median(drift_ms))
@@ -500,6 +516,7 @@ def main() -> int:
         "pm_mlp_vs_direct": pmres,
         "pm_near_vs_direct": nearres,
         "rung_subset_kicks": subset,
+        "host_io": host_io,
         "clustered_precision": precision,
         "mlp_training": train,
         "stepping": steps_res,
