@@ -346,13 +346,13 @@ def cpu_case(threads: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=128, help="lattice side (N = side^3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--fft-reps", type=int, default=10)
-    ap.add_argument("--reps", type=int, default=5, help="repetitions of the PM timings")
+    ap.add_argument("--reps", type=int, default=10, help="repetitions of the PM timings")
     ap.add_argument("--pm-ng", type=int, default=256, help="PM grid of the PM + MLP leg")
     ap.add_argument("--stepping-side", type=int, default=256, help="particles^(1/3) of the stepping leg (0: skip)")
     ap.add_argument("--stepping-steps", type=int, default=10)
