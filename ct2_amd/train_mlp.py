@@ -90,8 +90,8 @@ def main() -> int:
     ap.add_argument("--samples", type=int, default=1_000_000, help="labels per particle set")
     ap.add_argument("--seeds", type=int, default=5)
     ap.add_argument("--kinds", default="lattice,clustered")
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=60000)
+    ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--init-seed", type=int, default=1)
     ap.add_argument("--out", default=DEFAULT_OUT)
