@@ -87,10 +87,10 @@ def main() -> int:
     ap.add_argument("--rs", type=float, default=1.25)
     ap.add_argument("--near", type=float, default=0.0,
                     help="near-field cutoff in units of rs (0: PM alone); labels are then judge - (PM + near field)")
-    ap.add_argument("--samples", type=int, default=1_000_000, help="labels per particle set")
+    ap.add_argument("--samples", type=int, default=2_000_000, help="labels per particle set")
     ap.add_argument("--seeds", type=int, default=5)
     ap.add_argument("--kinds", default="lattice,clustered")
-    ap.add_argument("--steps", type=int, default=60000)
+    ap.add_argument("--steps", type=int, default=150000)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--init-seed", type=int, default=1)
