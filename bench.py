@@ -47,19 +47,24 @@ def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, J, ng: in
     from ct2_amd.train_mlp import DEFAULT_OUT, load_weights
 
     lib = _lib.load()
-    api.pm_init(ng, 1.25)
-    out = {"pm_grid": ng, "rs_cells": 1.25, "direct_kick_ms": round(kick_ms, 3)}
+    trained = load_weights(DEFAULT_OUT)
+    meta = trained[1] if trained is not None else {}
+    order, rs = int(meta.get("order", 1)), float(meta.get("rs_cells", 1.25))  # the mesh the model corrects
+    api.pm_assignment(order)
+    api.pm_init(ng, rs)
+    out = {"pm_grid": ng, "assignment": {1: "cic", 2: "tsc"}[order], "rs_cells": rs,
+           "direct_kick_ms": round(kick_ms, 3)}
     _lib.check(lib.ct2_pm_forces())
     pm_ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), reps)))
     fx, fy, fz, _ = api.forces_download()
     e_pm = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
     out.update({"pm_ms": round(pm_ms, 3), "pm_median_rel_err": float(np.median(e_pm)),
                 "pm_p90_rel_err": float(np.percentile(e_pm, 90))})
-    trained = load_weights(DEFAULT_OUT)
     if trained is None:
         out["pm_mlp"] = "no trained weights (run ct2_amd/train_mlp.py)"
+        api.pm_assignment(1)
         return out
-    w, meta = trained
+    w = trained[0]
     api.mlp_init(1)
     api.mlp_set_weights(w)
     _lib.check(lib.ct2_pm_mlp_forces())
@@ -72,7 +77,9 @@ def pm_leg(api, ev: HipEvents, n: int, side: int, kick_ms: float, idx, J, ng: in
                 "mlp_flops_per_particle": 189440, "mlp_tflops": round(189440 * n / (ms - pm_ms) / 1e9, 2)
                 if ms > pm_ms else None,
                 "target_median_1e-3_met": bool(np.median(e_ml) <= 1e-3),
-                "model": {k: meta[k] for k in ("ng", "rs_cells", "kinds", "seeds", "samples_per_set", "steps")}})
+                "model": {k: meta.get(k) for k in ("ng", "rs_cells", "order", "kinds", "seeds", "samples_per_set",
+                                                   "steps", "batch")}})
+    api.pm_assignment(1)
     return out
 
 
@@ -104,8 +111,9 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
     """The stepping configuration: side^3 particles from a Gaussian random
     field, 512^3 grid, KDK leapfrog a = 0.02 -> 0.03 with FP64 factors, the
     periodic FP64 direct sum judging `judge` particles each step.  method
-    "pm_mlp": CIC PM (split 1.25 cells) + the learned correction trained on
-    this configuration's distribution (the force the configuration names);
+    "pm_mlp": PM (the assignment and split the model was trained on: TSC,
+    1.25 cells) + the learned correction trained on this configuration's
+    distribution (the force the configuration names);
     "pm_near": TSC PM (split 2 cells) + the near field to 5.5 split radii.
     The direct kick at this N is not run (N^2 / rate at the measured rate
     would take minutes per step); its time is derived."""
@@ -117,8 +125,9 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
         trained = load_weights(path)
         if trained is None:
             return {"method": method, "skipped": f"no trained weights at {path}"}
-        rec = stepping.run(side, 2 * side, method="pm_mlp", weights=trained[0], rs_cells=1.25, near=0.0, order=1,
-                           **kw)
+        meta = trained[1]
+        rec = stepping.run(side, 2 * side, method="pm_mlp", weights=trained[0],
+                           rs_cells=float(meta.get("rs_cells", 1.25)), near=0.0, order=int(meta.get("order", 1)), **kw)
     else:
         rec = stepping.run(side, 2 * side, method="pm", rs_cells=2.0, near=5.5, order=2, **kw)
     n = side ** 3
@@ -203,7 +212,8 @@ def heldout_mlp_leg(api, side: int, ng: int, seed: int = 6) -> dict:
     api.set_force_options(1.0 / (50 * side), 1.0 / n, api.KERNEL_PLUMMER)
     idx = np.random.default_rng(seed).choice(n, 4096, replace=False)
     J = np.stack(api.kick_sample_fp64(idx, periodic=True))
-    api.pm_init(ng, 1.25)
+    api.pm_assignment(int(trained[1].get("order", 1)))
+    api.pm_init(ng, float(trained[1].get("rs_cells", 1.25)))
     api.mlp_init(1)
     api.mlp_set_weights(trained[0])
     out = {"seed": seed, "training_seeds": trained[1]["seeds"], "pm_grid": ng}
@@ -213,6 +223,7 @@ def heldout_mlp_leg(api, side: int, ng: int, seed: int = 6) -> dict:
         e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
         out[name + "_median_rel_err"] = float(np.median(e))
         out[name + "_p90_rel_err"] = float(np.percentile(e, 90))
+    api.pm_assignment(1)
     return out
 
 

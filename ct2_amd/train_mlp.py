@@ -6,7 +6,7 @@ library computes the periodic FP64 direct sum (Ewald) and stores
 (judge - PM) / F_unit with the PM-grid features; training runs the
 library's BF16-MFMA forward/backward kernels and Adam.
 
-Default: N = 128^3, PM grid 256^3 (split 1.25 cells), no near field
+Default: N = 128^3, TSC PM grid 256^3 (split 1.25 cells), no near field
 (`--near` adds the direct near-field sum to the base force, and the model
 then learns the mesh's remaining error); training sets from the
 lattice+Gaussian(0.2 cell) configuration (eps = 1/(50*128)) and the
@@ -85,6 +85,7 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=128)
     ap.add_argument("--ng", type=int, default=256)
     ap.add_argument("--rs", type=float, default=1.25)
+    ap.add_argument("--order", type=int, default=2, choices=[1, 2], help="mass assignment: 1 CIC, 2 TSC")
     ap.add_argument("--near", type=float, default=0.0,
                     help="near-field cutoff in units of rs (0: PM alone); labels are then judge - (PM + near field)")
     ap.add_argument("--samples", type=int, default=2_000_000, help="labels per particle set")
@@ -105,6 +106,7 @@ def main() -> int:
     api.init(0)
     kinds = args.kinds.split(",")
     n = args.side ** 3
+    api.pm_assignment(args.order)
     api.pm_init(args.ng, args.rs)
     api.pm_near(args.near)
     api.mlp_init(args.init_seed)
@@ -112,7 +114,8 @@ def main() -> int:
     build_training_set(api, kinds, range(1, args.seeds + 1), args.side, args.samples, log)
     losses = train(api, args.steps, args.batch, args.lr, args.init_seed, log)
     w = api.mlp_get_weights()
-    meta = {"side": args.side, "n": n, "ng": args.ng, "rs_cells": args.rs, "near_rcut_rs": args.near, "kinds": kinds, "seeds": args.seeds,
+    meta = {"side": args.side, "n": n, "ng": args.ng, "rs_cells": args.rs, "order": args.order,
+            "near_rcut_rs": args.near, "kinds": kinds, "seeds": args.seeds,
             "samples_per_set": args.samples, "steps": args.steps, "batch": args.batch, "lr": args.lr,
             "final_loss": float(np.mean(losses[-200:]))}
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
