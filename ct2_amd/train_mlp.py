@@ -6,7 +6,7 @@ library computes the periodic FP64 direct sum (Ewald) and stores
 (judge - PM) / F_unit with the PM-grid features; training runs the
 library's BF16-MFMA forward/backward kernels and Adam.
 
-Default: N = 128^3, TSC PM grid 256^3 (split 1.25 cells), no near field
+Default: N = 128^3, TSC PM grid 256^3 (split 1.0 cells), no near field
 (`--near` adds the direct near-field sum to the base force, and the model
 then learns the mesh's remaining error); training sets from the
 lattice+Gaussian(0.2 cell) configuration (eps = 1/(50*128)) and the
@@ -84,7 +84,7 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--side", type=int, default=128)
     ap.add_argument("--ng", type=int, default=256)
-    ap.add_argument("--rs", type=float, default=1.25)
+    ap.add_argument("--rs", type=float, default=1.0)
     ap.add_argument("--order", type=int, default=2, choices=[1, 2], help="mass assignment: 1 CIC, 2 TSC")
     ap.add_argument("--near", type=float, default=0.0,
                     help="near-field cutoff in units of rs (0: PM alone); labels are then judge - (PM + near field)")
