@@ -112,7 +112,7 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
     field, 512^3 grid, KDK leapfrog a = 0.02 -> 0.03 with FP64 factors, the
     periodic FP64 direct sum judging `judge` particles each step.  method
     "pm_mlp": PM (the assignment and split the model was trained on: TSC,
-    1.25 cells) + the learned correction trained on this configuration's
+    1.0 cells) + the learned correction trained on this configuration's
     distribution (the force the configuration names);
     "pm_near": TSC PM (split 2 cells) + the near field to 5.5 split radii.
     The direct kick at this N is not run (N^2 / rate at the measured rate

@@ -122,9 +122,11 @@ def main() -> int:
     ap.add_argument("--judge", type=int, default=16384)
     ap.add_argument("--method", default="pm_mlp", choices=["pm_mlp", "pm", "direct"])
     ap.add_argument("--weights", default=os.path.join(ROOT, "ct2_amd", "weights", "mlp_grf_ng2x.npz"))
-    ap.add_argument("--rs", type=float, default=1.25, help="PM split radius in cells")
+    ap.add_argument("--rs", type=float, default=None,
+                    help="PM split radius in cells (pm_mlp: the weights' own; otherwise 1.25)")
     ap.add_argument("--near", type=float, default=0.0, help="near-field cutoff in units of rs (0: off)")
-    ap.add_argument("--assign", type=int, default=1, choices=[1, 2], help="mass assignment: 1 CIC, 2 TSC")
+    ap.add_argument("--assign", type=int, default=None, choices=[1, 2],
+                    help="mass assignment: 1 CIC, 2 TSC (pm_mlp: the weights' own; otherwise CIC)")
     args = ap.parse_args()
     from ct2_amd import api
     from ct2_amd.timing import HipEvents
@@ -138,6 +140,15 @@ def main() -> int:
             print(f"no weights at {args.weights}", file=sys.stderr)
             return 2
         w = t[0]
+        # the mesh the model was trained to correct
+        if args.rs is None:
+            args.rs = float(t[1].get("rs_cells", 1.25))
+        if args.assign is None:
+            args.assign = int(t[1].get("order", 1))
+    if args.rs is None:
+        args.rs = 1.25
+    if args.assign is None:
+        args.assign = 1
     ev = HipEvents(api.stream())
     rec = run(args.side, args.ng, steps=args.steps, judge=args.judge, method=args.method, weights=w, timer=ev.time,
               log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=args.rs, near=args.near, order=args.assign)
