@@ -20,7 +20,8 @@ namespace {
 size_t g_count = 0;
 std::vector<char> g_store;
 float g_eta = 0.15f;
-int g_mesh_ng = 0;  // 0: direct-sum kick
+int g_mesh_ng = 0;   // 0: direct-sum kick
+bool g_learned = false;  // with g_mesh_ng > 0: PM + learned correction instead of PM + near field
 std::FILE* g_trace = nullptr;
 
 // ct2 aborts on allocation failure (memory.cpp:7-10); the binding does the
@@ -51,11 +52,23 @@ void ct2_gpu_set_options(float hsoft, float GM, float eta) {
 void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order) {
     ensure_init();
     g_mesh_ng = ng;
+    g_learned = false;
     if (ng > 0) {
         check(ct2_pm_assignment(order), "ct2_pm_assignment");
         check(ct2_pm_init(ng, rs_cells), "ct2_pm_init");
         check(ct2_pm_near(rcut_rs), "ct2_pm_near");
     }
+}
+
+void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells, int order) {
+    ensure_init();
+    check(ct2_pm_assignment(order), "ct2_pm_assignment");
+    check(ct2_pm_init(ng, rs_cells), "ct2_pm_init");
+    check(ct2_pm_near(0.0), "ct2_pm_near");
+    check(ct2_mlp_init(1), "ct2_mlp_init");
+    check(ct2_mlp_set_weights(weights, n), "ct2_mlp_set_weights");
+    g_mesh_ng = ng;
+    g_learned = true;
 }
 
 void ct2_gpu_trace(std::FILE* f) { g_trace = f; }
@@ -84,7 +97,9 @@ int kick(const kick_params& p) {
           "upload");
     const ct2_kick_params q{p.ascending, p.descending, p.top, p.first, p.rung, p.t0, p.eta, p.max_dt, p.scale};
     int max_rung = 0;
-    if (g_mesh_ng > 0)
+    if (g_mesh_ng > 0 && g_learned)
+        check(ct2_kick_pm_mlp(&q, CT2_KICK_STORE, &max_rung), "ct2_kick_pm_mlp");
+    else if (g_mesh_ng > 0)
         check(ct2_kick_pm(&q, CT2_KICK_STORE, &max_rung), "ct2_kick_pm");
     else
         check(ct2_kick(&q, CT2_KICK_STORE, &max_rung), "ct2_kick");

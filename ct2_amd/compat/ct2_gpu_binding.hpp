@@ -58,6 +58,9 @@ void ct2_gpu_set_options(float hsoft, float GM, float eta = 0.15f);
 // kick() takes the direct sum by default; this switches it to PM + near field
 // (ct2_pm_assignment, ct2_pm_init, ct2_pm_near); ng = 0 switches back.
 void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order);
+// ... or to PM + the learned correction with the given weights (n =
+// ct2_mlp_num_params(); trained for that grid, split and assignment).
+void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells, int order);
 
 // kick_step (kick.hpp:23, main.cpp:8-92) over the functions above, for
 // callers that do not link ct2's main.cpp (which runs unchanged on them).

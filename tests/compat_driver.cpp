@@ -2,13 +2,16 @@
 // ct2's kick_step would (main.cpp:62-81) and prints the resulting state, for
 // tests/test_gpu_compat.py to compare with the CPU oracle.  With the argument
 // "fft" it drives the FFT API the way fft_test.cpp does (scramble, then
-// fft_real; fft_r2c_3d, then fft_c2r_3d) and prints the results.
+// fft_real; fft_r2c_3d, then fft_c2r_3d) and prints the results.  With
+// "learned <weights file>" it switches kick() to PM + the learned correction
+// (raw FP32 weights from the file) and prints the velocities of one kick.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "ct2_gpu_binding.hpp"
+#include "ct2hip.h"
 
 static double lcg_unit(unsigned& s) {
     s = s * 1664525u + 1013904223u;
@@ -74,7 +77,39 @@ static int rungs_mode() {
     return 0;
 }
 
+static int learned_mode(const char* path) {
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return 2;
+    std::vector<float> w(ct2_mlp_num_params());
+    const size_t got = std::fread(w.data(), sizeof(float), w.size(), f);
+    std::fclose(f);
+    if (got != w.size()) return 3;
+    const int n = 4096;
+    particle_allocate(n);
+    ct2_gpu_set_options(0.01f, 1.0f / n);
+    unsigned s = 777u;
+    for (int i = 0; i < n; i++) {
+        s = s * 1664525u + 1013904223u;
+        particle_x_ptr[i] = s;
+        s = s * 1664525u + 1013904223u;
+        particle_y_ptr[i] = s;
+        s = s * 1664525u + 1013904223u;
+        particle_z_ptr[i] = s;
+        particle_vx_ptr[i] = particle_vy_ptr[i] = particle_vz_ptr[i] = 0.0f;
+        particle_rung_ptr[i] = 0;
+    }
+    ct2_gpu_use_learned(w.data(), w.size(), 32, 1.0, 2);
+    kick_params kp{true, true, true, false, 0, 0.01f, 0.15f, 0.01f, 1.0f};
+    const int max_rung = kick(kp);
+    std::printf("max_rung %d\n", max_rung);
+    for (int i = 0; i < n; i++)
+        std::printf("%.9e %.9e %.9e %d\n", particle_vx_ptr[i], particle_vy_ptr[i], particle_vz_ptr[i],
+                    (int)particle_rung_ptr[i]);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 2 && std::strcmp(argv[1], "learned") == 0) return learned_mode(argv[2]);
     if (argc > 1 && std::strcmp(argv[1], "fft") == 0) return fft_mode();
     if (argc > 1 && std::strcmp(argv[1], "rungs") == 0) return rungs_mode();
     const int n = argc > 1 ? std::atoi(argv[1]) : 512;

@@ -13,8 +13,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "_bin", "compat_driver")
 
 
-def lcg_positions(n):
-    s = 12345
+def lcg_positions(n, seed=12345):
+    s = seed
     out = np.empty((n, 3), np.uint32)
     for i in range(n):
         for d in range(3):
@@ -72,3 +72,43 @@ def test_fft_api_through_ct2_signatures(orc):
     back, v = v[:M ** 3], v[M ** 3:]
     assert np.allclose(back / M ** 3, cube.ravel(), atol=1e-14)
     assert np.allclose(v, orc.twiddles(8)[0], atol=1e-16)
+
+
+def test_learned_correction_through_kick(ct2, tmp_path):
+    """kick() switched to PM + the learned correction (ct2_gpu_use_learned,
+    TSC 32^3, split 1 cell) gives, bit for bit, the velocities and rungs of
+    the library's own ct2_kick_pm_mlp on the same particles and weights."""
+    nw = ct2.mlp_num_params()
+    w = (np.random.default_rng(5).standard_normal(nw) * 0.05).astype(np.float32)
+    path = tmp_path / "weights.f32"
+    w.tofile(path)
+    r = subprocess.run([EXE, "learned", str(path)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    max_rung = int(lines[0].split()[1])
+    rows = [l.split() for l in lines[1:]]
+    got = np.array([[float(v) for v in row[:3]] for row in rows], np.float32).T
+    got_rung = np.array([int(row[3]) for row in rows])
+    n = got.shape[1]
+    x, y, z = lcg_positions(n, 777)
+    zf = np.zeros(n, np.float32)
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z, zf, zf, zf, np.zeros(n, np.int8))
+    ct2.set_force_options(float(np.float32(0.01)), float(np.float32(1.0 / n)), 0)
+    try:
+        ct2.pm_assignment(2)
+        ct2.pm_init(32, 1.0)
+        ct2.pm_near(0.0)
+        ct2.mlp_init(1)
+        ct2.mlp_set_weights(w)
+        params = ct2.kick_params(ascending=True, descending=True, top=True, first=False, rung=0, t0=0.01, eta=0.15,
+                                 max_dt=0.01, scale=1.0)
+        want_max = ct2.kick_pm_mlp(params)
+        p = ct2.particles_download()
+    finally:
+        ct2.pm_assignment(1)
+    assert max_rung == want_max
+    assert np.array_equal(got, np.stack([p.vx, p.vy, p.vz]))
+    assert np.array_equal(got_rung, p.rung.astype(int))
+    assert np.abs(got).max() > 0
