@@ -6,8 +6,13 @@ lattice displaced by Gaussian noise of 0.2 cell (seed 1), Plummer softening
 eps = 1/(50*128), GM = 1/N.  One step is one all-pairs FP32 compensated kick
 (N^2 = 4.398e12 interactions, with the half-kick/rung epilogue) over inputs
 already resident in HBM.  The same run times the 3-D complex FP32 FFT
-(256^3 round trip of that configuration and a 512^3 forward transform) and
-the drift.
+(256^3 round trip of that configuration and a 512^3 forward transform), the
+drift, the host upload and download, PM alone and with the learned
+correction (also on a held-out seed), PM + the direct near field in three
+settings, rung-subset kicks, the clustered precision study, training steps
+of the learned correction, and the stepping configuration (2^24 particles,
+10 KDK steps) with each force method, every force judged against the FP64
+direct sum; and the CPU baseline on the host's cores.
 
 Prints one JSON line.  value = interactions/s of the kick.
 """
