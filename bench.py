@@ -132,7 +132,8 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
             return {"method": method, "skipped": f"no trained weights at {path}"}
         meta = trained[1]
         rec = stepping.run(side, 2 * side, method="pm_mlp", weights=trained[0],
-                           rs_cells=float(meta.get("rs_cells", 1.25)), near=0.0, order=int(meta.get("order", 1)), **kw)
+                           rs_cells=float(meta.get("rs_cells", 1.25)), near=0.0, order=int(meta.get("order", 1)),
+                           meta=meta, **kw)
     else:
         rec = stepping.run(side, 2 * side, method="pm", rs_cells=2.0, near=5.5, order=2, **kw)
     n = side ** 3
@@ -318,7 +319,20 @@ def cpu_baseline(x, y, z, soft, gm, target_s: float):
     return {"value": m * n / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
             "sample": f"FP64 all-pairs kick of {m} random sinks against all {n} sources "
                       f"({m * n:.3e} interactions, {dt:.1f} s, OpenMP threads={threads})",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "cpus_usable": len(os.sched_getaffinity(0)),
             "cpu_case": cpu_case(threads)}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (as lscpu reports it, from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_case(threads: int) -> dict:

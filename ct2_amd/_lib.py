@@ -26,6 +26,11 @@ KICK_NAIVE = 0x1
 KICK_STORE = 0x2
 KICK_ORDERED = 0x4
 KICK_SYMMETRIC = 0x8
+PATH_NONE = 0
+PATH_DIRECT = 1
+PATH_PM = 2
+PATH_PM_NEAR = 3
+PATH_PM_MLP = 4
 
 
 class Ct2Error(RuntimeError):
@@ -90,6 +95,8 @@ _SIGS = {
     "ct2_particle_current_range": ([_P, _P], C.c_int),
     "ct2_particle_rung_counts": ([_P, C.c_int, _P], C.c_int),
     "ct2_particle_set_minrung": ([C.c_int], C.c_int),
+    "ct2_particles_to_grid": ([C.c_int, _P], C.c_int),
+    "ct2_debug_fail_allocations": ([C.c_int, C.c_int], C.c_int),
     "ct2_pm_near_cells": ([], C.c_int),
     "ct2_near_kernel_table": ([C.c_double, C.c_double, C.c_int, _P, _P], C.c_int),
     "ct2_pm_assignment": ([C.c_int], C.c_int),
@@ -108,6 +115,8 @@ _SIGS = {
     "ct2_mlp_data_append": ([_P, C.c_size_t], C.c_int),
     "ct2_mlp_train": ([C.c_int, C.c_int, C.c_float, C.c_uint64, _P], C.c_int),
     "ct2_pm_mlp_forces": ([], C.c_int),
+    "ct2_mlp_set_mesh": ([C.c_double, C.c_int, C.c_double, C.c_double], C.c_int),
+    "ct2_last_force_path": ([], C.c_int),
     "ct2_kick_pm_mlp": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_fft3d_c2c": ([_P, C.c_int, C.c_int], C.c_int),
     "ct2_fft3d_c2c_f64": ([_P, C.c_int, C.c_int], C.c_int),

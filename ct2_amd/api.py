@@ -21,6 +21,7 @@ KICK_NAIVE = L.KICK_NAIVE
 KICK_STORE = L.KICK_STORE
 KICK_ORDERED = L.KICK_ORDERED
 KICK_SYMMETRIC = L.KICK_SYMMETRIC
+PATH_DIRECT, PATH_PM, PATH_PM_NEAR, PATH_PM_MLP = L.PATH_DIRECT, L.PATH_PM, L.PATH_PM_NEAR, L.PATH_PM_MLP
 
 
 def _p(a: np.ndarray | None):
@@ -164,6 +165,20 @@ def particle_set_minrung(minrung: int) -> None:
     L.check(L.load().ct2_particle_set_minrung(minrung))
 
 
+def particles_to_grid(ncell: int) -> np.ndarray:
+    """Sorts the current rung range by cell (ct2_particles_to_grid); returns
+    the ncell^3 + 1 cell starts (container indices)."""
+    starts = np.zeros(ncell ** 3 + 1, np.uint64)
+    L.check(L.load().ct2_particles_to_grid(ncell, _p(starts)))
+    return starts
+
+
+def debug_fail_allocations(after: int, count: int) -> None:
+    """After `after` more device allocations succeed, the next `count` fail
+    with CT2_ERR_NOMEM (test hook; count 0 turns it off)."""
+    L.check(L.load().ct2_debug_fail_allocations(after, count))
+
+
 # ---------------------------------------------------------------- particle mesh
 _pm_ng = 0
 
@@ -246,6 +261,24 @@ def mlp_get_weights() -> np.ndarray:
 def mlp_set_weights(w) -> None:
     w = np.ascontiguousarray(w, dtype=np.float32)
     L.check(L.load().ct2_mlp_set_weights(_p(w), w.size))
+
+
+def mlp_set_mesh(rs_cells: float, order: int, rcut_rs: float = 0.0, particles_per_node: float = 0.0) -> None:
+    """Records the mesh the current weights were trained for (ct2_mlp_set_mesh):
+    ct2_kick_pm_mlp falls back to the direct sum on any other mesh."""
+    L.check(L.load().ct2_mlp_set_mesh(rs_cells, order, rcut_rs, particles_per_node))
+
+
+def mlp_mesh_of(meta: dict) -> tuple[float, int, float, float]:
+    """(rs_cells, order, rcut_rs, particles per node) of a trained model's
+    metadata (train_mlp.py)."""
+    ppn = (meta["side"] / meta["ng"]) ** 3 if meta.get("side") else 0.0
+    return float(meta["rs_cells"]), int(meta.get("order", 1)), float(meta.get("near_rcut_rs", 0.0)), float(ppn)
+
+
+def last_force_path() -> int:
+    """CT2_PATH_* of the last kick (ct2_last_force_path)."""
+    return int(L.load().ct2_last_force_path())
 
 
 def mlp_forward(X) -> np.ndarray:

@@ -1,6 +1,7 @@
 // ct2_gpu_binding.cpp -- see ct2_gpu_binding.hpp.
 #include "ct2_gpu_binding.hpp"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -20,6 +21,9 @@ namespace {
 size_t g_count = 0;
 std::vector<char> g_store;
 float g_eta = 0.15f;
+int g_ppcell = 256, g_bwidth = 3;  // options.cpp:61,54
+int g_ncell = 0;                   // the last particle_to_grid
+std::vector<uint64_t> g_starts;    // its cell starts, g_ncell^3 + 1
 int g_mesh_ng = 0;   // 0: direct-sum kick
 bool g_learned = false;  // with g_mesh_ng > 0: PM + learned correction instead of PM + near field
 std::FILE* g_trace = nullptr;
@@ -42,11 +46,20 @@ void ensure_init() {
 }
 }  // namespace
 
-void ct2_gpu_set_options(float hsoft, float GM, float eta) {
+void ct2_gpu_set_options(float hsoft, float GM, float eta, int kernel) {
     ensure_init();
-    ct2_force_options o{hsoft, GM, CT2_KERNEL_PLUMMER};
+    ct2_force_options o{hsoft, GM, kernel};
     check(ct2_set_force_options(&o), "ct2_set_force_options");
     g_eta = eta;
+}
+
+void ct2_gpu_set_grid(int ppcell, int bwidth) {
+    if (ppcell <= 0 || bwidth < 0) {
+        std::fprintf(stderr, "ct2 GPU path: bad grid options ppcell %d bwidth %d\n", ppcell, bwidth);
+        std::abort();
+    }
+    g_ppcell = ppcell;
+    g_bwidth = bwidth;
 }
 
 void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order) {
@@ -60,13 +73,15 @@ void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order) {
     }
 }
 
-void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells, int order) {
+void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells, int order,
+                         double particles_per_node) {
     ensure_init();
     check(ct2_pm_assignment(order), "ct2_pm_assignment");
     check(ct2_pm_init(ng, rs_cells), "ct2_pm_init");
     check(ct2_pm_near(0.0), "ct2_pm_near");
     check(ct2_mlp_init(1), "ct2_mlp_init");
     check(ct2_mlp_set_weights(weights, n), "ct2_mlp_set_weights");
+    check(ct2_mlp_set_mesh(rs_cells, order, 0.0, particles_per_node), "ct2_mlp_set_mesh");
     g_mesh_ng = ng;
     g_learned = true;
 }
@@ -90,6 +105,18 @@ void particle_allocate(size_t count) {
 }
 
 size_t particle_count() { return g_count; }
+
+void particles_init() {
+    // ct2 draws libc rand() uniforms (particle.cpp:364-374, util.cpp:3-5); the
+    // library's seeded generator replaces them, so a run is reproducible
+    ensure_init();
+    check(ct2_ic_plummer_mix(g_count, 0, 0, 0.0, 0.0, 1, particle_x_ptr, particle_y_ptr, particle_z_ptr),
+          "ct2_ic_plummer_mix");
+    for (size_t i = 0; i < g_count; ++i) {
+        particle_vx_ptr[i] = particle_vy_ptr[i] = particle_vz_ptr[i] = 0.0f;
+        particle_rung_ptr[i] = 0;
+    }
+}
 
 int kick(const kick_params& p) {
     check(ct2_particles_upload(particle_x_ptr, particle_y_ptr, particle_z_ptr, particle_vx_ptr, particle_vy_ptr,
@@ -164,7 +191,43 @@ std::pair<size_t, size_t> particle_current_range() {
     return {first, last};
 }
 
-void particle_to_grid() {}
+void particle_to_grid() {
+    const std::pair<size_t, size_t> rng = particle_current_range();
+    const size_t count = rng.second - rng.first;
+    const int ncell = (int)(std::pow((double)(count / (size_t)g_ppcell), 1.0 / 3.0) + 4);
+    upload_all();
+    g_starts.assign((size_t)ncell * ncell * ncell + 1, 0);
+    check(ct2_particles_to_grid(ncell, g_starts.data()), "ct2_particles_to_grid");
+    check(ct2_particles_download(particle_x_ptr, particle_y_ptr, particle_z_ptr, particle_vx_ptr, particle_vy_ptr,
+                                 particle_vz_ptr, (int8_t*)particle_rung_ptr),
+          "download");
+    g_ncell = ncell;
+    if (g_trace) std::fprintf(g_trace, "to_grid %d\n", ncell);
+}
+
+int particle_cell_dim() { return g_ncell; }
+
+std::pair<size_t, size_t> particle_get_cell(int i, int j, int k) {
+    const int n = g_ncell;
+    if (n <= 0 || i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) {
+        std::fprintf(stderr, "ct2 GPU path: cell (%d, %d, %d) outside the %d^3 grid\n", i, j, k, n);
+        std::abort();
+    }
+    const size_t c = ((size_t)i * n + j) * n + k;
+    return {(size_t)g_starts[c], (size_t)g_starts[c + 1]};
+}
+
+std::vector<std::pair<size_t, size_t>> particle_cell_neighbors(int i, int j, int k) {
+    const int n = g_ncell, b = g_bwidth;
+    std::vector<std::pair<size_t, size_t>> out;
+    for (int i0 = -b; i0 <= b; ++i0)
+        for (int j0 = -b; j0 <= b; ++j0)
+            for (int k0 = -b; k0 <= b; ++k0)
+                if (i0 * i0 + j0 * j0 + k0 * k0 <= b * b)
+                    out.push_back(particle_get_cell(((i + i0) % n + n) % n, ((j + j0) % n + n) % n,
+                                                    ((k + k0) % n + n) % n));
+    return out;
+}
 
 void particle_pop_rungs() {
     check(ct2_particle_pop_rungs(), "ct2_particle_pop_rungs");
@@ -272,7 +335,7 @@ void ct2_gpu_fft_complex_lanes4(double* X, int N, int dir) {
     d.to_host(X);
 }
 
-void fft_r2c_3d(const fft_float* in, fft_float* out, int N) {
+void fft_r2c_3d(fft_float* in, fft_float* out, int N) {
     ensure_init();
     const size_t n3 = (size_t)N * N * N, half = (size_t)N * N * (N / 2 + 1);
     DeviceCopy a(in, n3), b(nullptr, 2 * half);
@@ -281,7 +344,7 @@ void fft_r2c_3d(const fft_float* in, fft_float* out, int N) {
     b.to_host(out);
 }
 
-void fft_c2r_3d(const fft_float* in, fft_float* out, int N) {
+void fft_c2r_3d(fft_float* in, fft_float* out, int N) {
     ensure_init();
     const size_t n3 = (size_t)N * N * N, half = (size_t)N * N * (N / 2 + 1);
     DeviceCopy a(in, 2 * half), b(nullptr, n3);
@@ -306,7 +369,7 @@ void transpose_yz(fft_float* X, int N) {
     d.to_host(X);
 }
 
-void transpose_r2c(const fft_float* in, fft_float* out, int N) {
+void transpose_r2c(fft_float* in, fft_float* out, int N) {
     ensure_init();
     DeviceCopy a(in, (size_t)N * N * N), b(nullptr, 2 * (size_t)N * N * (N / 2 + 1));
     check(ct2_transpose_r2c(a.ptr(), b.ptr(), N), "ct2_transpose_r2c");
@@ -314,7 +377,7 @@ void transpose_r2c(const fft_float* in, fft_float* out, int N) {
     b.to_host(out);
 }
 
-void transpose_c2r(const fft_float* in, fft_float* out, int N) {
+void transpose_c2r(fft_float* in, fft_float* out, int N) {
     ensure_init();
     DeviceCopy a(in, 2 * (size_t)N * N * (N / 2 + 1)), b(out, (size_t)N * N * N);
     check(ct2_transpose_c2r(a.ptr(), b.ptr(), N), "ct2_transpose_c2r");
@@ -339,6 +402,6 @@ const std::vector<double>& twiddle_table(int N, int which) {
 }
 }  // namespace
 
-const std::vector<double>& cos_twiddles(int N) { return twiddle_table(N, 0); }
-const std::vector<double>& sin_twiddles(int N) { return twiddle_table(N, 1); }
-const std::vector<double>& inv_sin_twiddles(int N) { return twiddle_table(N, 2); }
+const std::vector<fft_float>& cos_twiddles(int N) { return twiddle_table(N, 0); }
+const std::vector<fft_float>& sin_twiddles(int N) { return twiddle_table(N, 1); }
+const std::vector<fft_float>& inv_sin_twiddles(int N) { return twiddle_table(N, 2); }

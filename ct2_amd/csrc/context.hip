@@ -32,9 +32,18 @@ int require_particles() {
     return CT2_OK;
 }
 
+static int g_fail_after = 0, g_fail_allocs = 0;  // ct2_debug_fail_allocations
+
 int dev_alloc(void** p, size_t bytes, const char* what) {
     *p = nullptr;
     if (bytes == 0) return CT2_OK;
+    if (g_fail_allocs > 0 && g_fail_after > 0) {
+        --g_fail_after;
+    } else if (g_fail_allocs > 0) {
+        --g_fail_allocs;
+        return set_error(CT2_ERR_NOMEM, "device allocation of %zu bytes for %s failed: injected by the test hook",
+                         bytes, what);
+    }
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
@@ -119,6 +128,7 @@ int ct2_finalize(void) {
     release_near();
     release_fft_ct2();
     release_rungs();
+    release_grid();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device = -1;
@@ -223,6 +233,15 @@ int ct2_forces_download(float* fx, float* fy, float* fz, float* pot) {
     for (int a = 0; a < 4; ++a)
         if (dst[a]) CT2_HIP(hipMemcpyAsync(dst[a], src[a], b4, hipMemcpyDeviceToHost, c.stream));
     CT2_HIP(hipStreamSynchronize(c.stream));
+    return CT2_OK;
+}
+
+int ct2_last_force_path(void) { return g_ctx.last_path; }
+
+int ct2_debug_fail_allocations(int after, int count) {
+    if (after < 0 || count < 0) return set_error(CT2_ERR_INVALID, "negative count");
+    g_fail_after = after;
+    g_fail_allocs = count;
     return CT2_OK;
 }
 

@@ -36,6 +36,9 @@ struct Context {
     size_t partial_bytes = 0;
     int* maxrung = nullptr;
 
+    // the force path the last kick took (CT2_PATH_*)
+    int last_path = 0;
+
     // rung bookkeeping (particle.cpp:20-58): kicks act on the sinks
     // [rung_begin, n); push/pop save and restore rung_begin
     size_t rung_begin = 0;
@@ -64,12 +67,16 @@ int fft3d_density(const unsigned long long* W, void* out, int n);
 int pm_forces();
 int pm_require();
 void pm_grids(int* g, const unsigned long long** W, const float2** A, const float2** B);
+// The mesh of the PM force: grid side (0 before ct2_pm_init), split radius in
+// cells, assignment order, near-field cutoff in split radii (0 = off).
+void pm_config(int* ng, double* rs_cells, int* order, double* rcut_rs);
 
 // Near field of the PM force (near.hip): direct pairs closer than
 // rcut_rs * rs with the softened kernel minus the mesh's filtered one, added
 // to the stored force.  rs in box units.
 int near_set(double rcut_rs);
 bool near_enabled();
+double near_rcut_rs();
 int near_forces(double rs);
 int near_cells();
 
@@ -85,6 +92,7 @@ void release_mlp();
 void release_near();
 void release_fft_ct2();
 void release_rungs();
+void release_grid();
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(long v) {

@@ -680,6 +680,7 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
     if (!c.maxrung) CT2_TRY(dev_alloc((void**)&c.maxrung, sizeof(int), "max rung"));
     if (ns <= 0) {  // no active sink: nothing to kick
         if (max_rung) *max_rung = 0;
+        c.last_path = CT2_PATH_DIRECT;
         return CT2_OK;
     }
     const double two32 = 4294967296.0;
@@ -720,6 +721,7 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
                                                           c.vy, c.vz, c.rung, c.fx, c.fy, c.fz, c.pot, c.maxrung);
     CT2_LAUNCH_CHECK();
     c.forces_valid = true;
+    c.last_path = CT2_PATH_DIRECT;
     if (max_rung) {
         CT2_HIP(hipMemcpyAsync(max_rung, c.maxrung, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         CT2_HIP(hipStreamSynchronize(c.stream));

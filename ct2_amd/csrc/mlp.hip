@@ -635,6 +635,16 @@ __global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const uint32_t* __re
 // ------------------------------------------------------------------ state
 struct Mlp {
     bool ready = false;
+    // the weights are trained (ct2_mlp_set_weights or ct2_mlp_train), and the
+    // mesh they were trained to correct, when known
+    bool trained = false;
+    // (the features are in grid units, so the grid side itself may differ)
+    bool mesh_known = false;
+    int mesh_order = 0;
+    double mesh_rs = 0.0, mesh_rcut = 0.0, mesh_ppn = 0.0;
+    // the mesh of the training set (data_order 0: unknown or mixed)
+    int data_order = 0;
+    double data_rs = 0.0, data_rcut = 0.0, data_ppn = 0.0;
     float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     bf16_t *W1t = nullptr, *W2t = nullptr, *W3t = nullptr, *W2n = nullptr, *W3n = nullptr;
     bf16_t *W2p = nullptr, *W3p = nullptr;  // k-permuted copies for the fused inference kernel
@@ -901,6 +911,20 @@ extern "C" int ct2_mlp_set_weights(const float* w, size_t n) {
     CT2_TRY(repack(c.stream));
     CT2_HIP(hipStreamSynchronize(c.stream));
     g_mlp.adam_t = 0;
+    g_mlp.trained = true;
+    g_mlp.mesh_known = false;  // until ct2_mlp_set_mesh says what these weights were trained for
+    return CT2_OK;
+}
+
+extern "C" int ct2_mlp_set_mesh(double rs_cells, int order, double rcut_rs, double particles_per_node) {
+    CT2_TRY(require_mlp());
+    if (!(rs_cells > 0.0) || (order != 1 && order != 2) || !(rcut_rs >= 0.0) || !(particles_per_node >= 0.0))
+        return set_error(CT2_ERR_INVALID, "bad mesh description");
+    g_mlp.mesh_known = true;
+    g_mlp.mesh_rs = rs_cells;
+    g_mlp.mesh_order = order;
+    g_mlp.mesh_rcut = rcut_rs;
+    g_mlp.mesh_ppn = particles_per_node;
     return CT2_OK;
 }
 
@@ -1016,6 +1040,16 @@ extern "C" int ct2_mlp_data_append(const int64_t* idx, size_t m) {
     CT2_LAUNCH_CHECK();
     CT2_HIP(hipStreamSynchronize(c.stream));
     g_mlp.dsize += (long)m;
+    // the mesh the training set corrects (the same for every append, or unknown)
+    int ng, order;
+    double rs, rcut;
+    pm_config(&ng, &rs, &order, &rcut);
+    const double ppn = (double)c.n / ((double)ng * ng * ng);
+    if (off == 0) {
+        g_mlp.data_rs = rs, g_mlp.data_order = order, g_mlp.data_rcut = rcut, g_mlp.data_ppn = ppn;
+    } else if (g_mlp.data_rs != rs || g_mlp.data_order != order || g_mlp.data_rcut != rcut || g_mlp.data_ppn != ppn) {
+        g_mlp.data_order = 0;
+    }
     return CT2_OK;
 }
 
@@ -1052,6 +1086,10 @@ extern "C" int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, floa
         }
     }
     CT2_HIP(hipStreamSynchronize(st));
+    g_mlp.trained = true;
+    g_mlp.mesh_known = g_mlp.data_order > 0;
+    g_mlp.mesh_rs = g_mlp.data_rs, g_mlp.mesh_order = g_mlp.data_order;
+    g_mlp.mesh_rcut = g_mlp.data_rcut, g_mlp.mesh_ppn = g_mlp.data_ppn;
     return CT2_OK;
 }
 
@@ -1061,12 +1099,31 @@ extern "C" int ct2_pm_mlp_forces(void) {
     return mlp_correct_forces();
 }
 
+// Whether the learned correction can correct the current PM force: trained
+// weights, and the mesh they were trained for (when known) is the current
+// one: split radius in cells, assignment, near-field cutoff and particles per
+// node (the grid side may differ: every feature is in grid units).
+bool mlp_usable() {
+    if (!g_mlp.ready || !g_mlp.trained) return false;
+    int ng, order;
+    double rs, rcut;
+    pm_config(&ng, &rs, &order, &rcut);
+    if (ng == 0) return false;
+    if (!g_mlp.mesh_known) return true;
+    const double ppn = (double)ctx().n / ((double)ng * ng * ng);
+    auto close = [](double a, double b) { return std::fabs(a - b) <= 1e-9 * std::max(std::fabs(a), std::fabs(b)); };
+    return order == g_mlp.mesh_order && close(rs, g_mlp.mesh_rs) &&
+           close(rcut, g_mlp.mesh_rcut) && (g_mlp.mesh_ppn == 0.0 || close(ppn, g_mlp.mesh_ppn));
+}
+
 extern "C" int ct2_kick_pm_mlp(const ct2_kick_params* p, unsigned flags, int* max_rung) {
-    CT2_TRY(pm_require());
-    CT2_TRY(require_mlp());
+    CT2_TRY(require_particles());
     CT2_TRY(check_kick_params(p));
     if (flags & ~(unsigned)CT2_KICK_STORE) return set_error(CT2_ERR_INVALID, "unknown flags 0x%x", flags);
+    // without usable weights the direct-sum kick is the fallback
+    if (!mlp_usable()) return ct2_kick(p, flags, max_rung);
     CT2_TRY(pm_forces());
     CT2_TRY(mlp_correct_forces());
+    ctx().last_path = CT2_PATH_PM_MLP;
     return run_epilogue(p, max_rung);
 }

@@ -533,6 +533,7 @@ void near_kernel_table(double rsu, double rcu, int nt, float* k4, float* p4) {
 }
 
 bool near_enabled() { return g_near.rcut_rs > 0.0; }
+double near_rcut_rs() { return g_near.rcut_rs; }
 
 // Adds the near field to the stored PM force (rs in box units).
 int near_forces(double rs) {

@@ -515,6 +515,13 @@ int pm_forces() {
 }
 
 // Device grids for the learned correction's feature gather.
+void pm_config(int* ng, double* rs_cells, int* order, double* rcut_rs) {
+    *ng = g_pm.ng;
+    *rs_cells = g_pm.rs_cells;
+    *order = g_assign;
+    *rcut_rs = near_rcut_rs();
+}
+
 void pm_grids(int* g, const unsigned long long** W, const float2** A, const float2** B) {
     *g = g_pm.g;
     *W = g_pm.W;
@@ -585,6 +592,7 @@ extern "C" int ct2_kick_pm(const ct2_kick_params* p, unsigned flags, int* max_ru
     CT2_TRY(check_kick_params(p));
     if (flags & ~(unsigned)CT2_KICK_STORE) return set_error(CT2_ERR_INVALID, "unknown flags 0x%x", flags);
     CT2_TRY(pm_forces());
+    ctx().last_path = near_enabled() ? CT2_PATH_PM_NEAR : CT2_PATH_PM;
     return run_epilogue(p, max_rung);
 }
 

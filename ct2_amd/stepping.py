@@ -39,9 +39,12 @@ def rel_err(F, J):
 
 
 def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, judge=16384, method="pm_mlp",
-        weights=None, timer=None, log=print, rs_cells=1.25, near=0.0, order=1):
+        weights=None, timer=None, log=print, rs_cells=1.25, near=0.0, order=1, meta=None):
     """Runs the loop; returns a record with per-step force/step times and the
-    judged force errors."""
+    judged force errors.  `meta` (pm_mlp) is the trained model's metadata: the
+    mesh it records is handed to the library, and the run stops if the kick
+    does not take the learned path (ct2_kick_pm_mlp falls back to the direct
+    sum on another mesh)."""
     from ct2_amd import _lib, api
 
     lib = _lib.load()
@@ -60,6 +63,8 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
             raise ValueError("pm_mlp needs trained weights")
         api.mlp_init(1)
         api.mlp_set_weights(weights)
+        if meta is not None:
+            api.mlp_set_mesh(*api.mlp_mesh_of(meta))
     kick_fn = {"pm_mlp": lib.ct2_kick_pm_mlp, "pm": lib.ct2_kick_pm, "direct": lib.ct2_kick}[method]
 
     a = np.linspace(a_start, a_end, steps + 1)
@@ -74,6 +79,8 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
         ms_force = timer(lambda: _lib.check(kick_fn(params, api.KICK_STORE, None))) if timer else None
         if timer is None:
             _lib.check(kick_fn(params, api.KICK_STORE, None))
+        if method == "pm_mlp" and api.last_force_path() != api.PATH_PM_MLP:
+            raise RuntimeError("the learned correction does not fit this mesh; the kick fell back to the direct sum")
         rec = {"step": i, "a": float(a[i]), "kick_factor": float(pending), "force_kick_ms": ms_force}
         if judge:
             idx = rng.choice(n, judge, replace=False)
@@ -117,7 +124,8 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
 def main() -> int:
     ap = argparse.ArgumentParser(description="global-step leapfrog on one GPU")
     ap.add_argument("--side", type=int, default=256)
-    ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--ng", type=int, default=None,
+                    help="PM grid side (pm_mlp: the weights' grid-to-lattice ratio times --side; otherwise 2 side)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--judge", type=int, default=16384)
     ap.add_argument("--method", default="pm_mlp", choices=["pm_mlp", "pm", "direct"])
@@ -133,25 +141,38 @@ def main() -> int:
     from ct2_amd.train_mlp import load_weights
 
     api.init(0)
-    w = None
+    w = meta = None
     if args.method == "pm_mlp":
         t = load_weights(args.weights)
         if t is None:
             print(f"no weights at {args.weights}", file=sys.stderr)
             return 2
-        w = t[0]
-        # the mesh the model was trained to correct
+        w, meta = t
+        # the mesh the model was trained to correct; another one is refused
+        ratio = meta["ng"] / meta["side"]
+        if args.ng is None:
+            args.ng = int(round(args.side * ratio))
         if args.rs is None:
-            args.rs = float(t[1].get("rs_cells", 1.25))
+            args.rs = float(meta.get("rs_cells", 1.25))
         if args.assign is None:
-            args.assign = int(t[1].get("order", 1))
+            args.assign = int(meta.get("order", 1))
+        want = (ratio, float(meta.get("rs_cells", 1.25)), int(meta.get("order", 1)),
+                float(meta.get("near_rcut_rs", 0.0)))
+        have = (args.ng / args.side, args.rs, args.assign, args.near)
+        if any(abs(a - b) > 1e-9 * max(abs(a), abs(b), 1.0) for a, b in zip(want, have)):
+            print(f"the weights were trained for (grid/lattice, rs, assignment, near) = {want}, not {have}",
+                  file=sys.stderr)
+            return 2
+    if args.ng is None:
+        args.ng = 2 * args.side
     if args.rs is None:
         args.rs = 1.25
     if args.assign is None:
         args.assign = 1
     ev = HipEvents(api.stream())
     rec = run(args.side, args.ng, steps=args.steps, judge=args.judge, method=args.method, weights=w, timer=ev.time,
-              log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=args.rs, near=args.near, order=args.assign)
+              log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=args.rs, near=args.near, order=args.assign,
+              meta=meta)
     print(json.dumps(rec))
     api.finalize()
     return 0

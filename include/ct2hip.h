@@ -210,10 +210,29 @@ size_t ct2_mlp_data_size(void);
 int ct2_mlp_data_append(const int64_t* idx, size_t m);
 /* `steps` Adam steps on random batches; losses[steps] optional. */
 int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, float* losses);
+/* The mesh the current weights were trained to correct: split radius in
+ * cells, assignment order (1 CIC, 2 TSC), near-field cutoff in split radii
+ * (0: none) and particles per grid node (0: not checked).  The grid side
+ * itself may differ (every feature is in grid units).  ct2_mlp_train
+ * records the mesh of its training set itself; ct2_mlp_set_weights forgets
+ * it until this call. */
+int ct2_mlp_set_mesh(double rs_cells, int order, double rcut_rs, double particles_per_node);
 /* PM force plus the learned correction into the stored-force arrays, and the
- * kick() built on it (epilogue of kick.cpp:75-99). */
+ * kick() built on it (epilogue of kick.cpp:75-99).  ct2_kick_pm_mlp falls
+ * back to the direct-sum kick (ct2_kick with the same parameters and flags)
+ * when the correction is not usable: no trained weights (only
+ * ct2_mlp_init), no PM mesh, or a mesh other than the one recorded for the
+ * weights; ct2_last_force_path says which ran. */
 int ct2_pm_mlp_forces(void);
 int ct2_kick_pm_mlp(const ct2_kick_params* params, unsigned flags, int* max_rung);
+
+/* The force path the last kick took. */
+#define CT2_PATH_NONE 0
+#define CT2_PATH_DIRECT 1  /* all-pairs direct sum (ct2_kick) */
+#define CT2_PATH_PM 2      /* PM alone */
+#define CT2_PATH_PM_NEAR 3 /* PM + near field */
+#define CT2_PATH_PM_MLP 4  /* PM + learned correction */
+int ct2_last_force_path(void);
 
 /* ------------------------------------------------------------ rungs */
 /* ct2's rung bookkeeping (particle.cpp:20-58, 320-347), the primitives of
@@ -240,6 +259,24 @@ int ct2_particle_pop_rungs(void);
 int ct2_particle_current_range(size_t* first, size_t* last);
 int ct2_particle_rung_counts(uint64_t* counts, int max_rungs, int* nrungs);
 int ct2_particle_set_minrung(int minrung);
+
+/* Replaces particle_to_grid() (particle.cpp:59-114, 349-362) and the cell
+ * accessors it fills (particle_get_cell, particle_cell_dim,
+ * particle.cpp:290-296): sorts the current rung range [first, n) by cell,
+ * cell (ix, iy, iz) with ix = floor(x ncell / 2^32), in the order
+ * (ix ncell + iy) ncell + iz, permuting every container array.  The sort is
+ * stable (ct2's recursive partition is not).  starts (host, ncell^3 + 1
+ * entries, may be NULL) receives each cell's first container index: cell c
+ * holds [starts[c], starts[c+1]).  The stored forces are invalid until the
+ * next kick. */
+#define CT2_GRID_MAX_NCELL 1024
+int ct2_particles_to_grid(int ncell, uint64_t* starts);
+
+/* Test hook for failure handling (no reference counterpart: ct2 aborts on a
+ * failed allocation, memory.cpp:7-10): after `after` more device
+ * allocations succeed, the next `count` the library makes fail with
+ * CT2_ERR_NOMEM without calling hipMalloc (count 0 turns the hook off). */
+int ct2_debug_fail_allocations(int after, int count);
 
 /* ------------------------------------------------------------ drift */
 /* Replaces void drift(float dt, float scale) (drift.hpp:12, drift.cpp:4-18):

@@ -108,13 +108,50 @@ static int learned_mode(const char* path) {
     return 0;
 }
 
+// particles_init, then particle_to_grid over the rung range [first, n) left by
+// particle_sort_by_rung(1) (rungs 0/1 alternate in blocks of 7); prints the
+// range, the grid's dimension, each cell's range, the neighbour count of a
+// corner cell, and the particles in container order.
+static int grid_mode() {
+    const int n = 20000;
+    particle_allocate(n);
+    particles_init();
+    for (int i = 0; i < n; i++) {
+        particle_vx_ptr[i] = (float)i;  // a tag that follows each particle
+        particle_rung_ptr[i] = (char)((i / 7) % 2);
+    }
+    ct2_gpu_set_grid(40, 2);
+    particle_sort_by_rung(1);
+    const std::pair<size_t, size_t> rng = particle_current_range();
+    particle_to_grid();
+    const int nc = particle_cell_dim();
+    std::printf("range %zu %zu ncell %d neighbours %zu\n", rng.first, rng.second, nc,
+                particle_cell_neighbors(0, 0, 0).size());
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++)
+            for (int k = 0; k < nc; k++) {
+                const std::pair<size_t, size_t> c = particle_get_cell(i, j, k);
+                std::printf("%zu %zu\n", c.first, c.second);
+            }
+    for (int i = 0; i < n; i++)
+        std::printf("%u %u %u %.1f %d\n", particle_x_ptr[i], particle_y_ptr[i], particle_z_ptr[i], particle_vx_ptr[i],
+                    (int)particle_rung_ptr[i]);
+    return 0;
+}
+
+// Default mode: `compat_driver n [kernel]`; without a kernel argument the
+// binding's default (ct2's uniform-density sphere, kick.cpp:52-60) is used.
 int main(int argc, char** argv) {
     if (argc > 2 && std::strcmp(argv[1], "learned") == 0) return learned_mode(argv[2]);
     if (argc > 1 && std::strcmp(argv[1], "fft") == 0) return fft_mode();
     if (argc > 1 && std::strcmp(argv[1], "rungs") == 0) return rungs_mode();
+    if (argc > 1 && std::strcmp(argv[1], "grid") == 0) return grid_mode();
     const int n = argc > 1 ? std::atoi(argv[1]) : 512;
     particle_allocate(n);
-    ct2_gpu_set_options(0.01f, 1.0f / n);
+    if (argc > 2)
+        ct2_gpu_set_options(0.01f, 1.0f / n, 0.15f, std::atoi(argv[2]));
+    else
+        ct2_gpu_set_options(0.01f, 1.0f / n);
     unsigned s = 12345u;
     for (int i = 0; i < n; i++) {
         s = s * 1664525u + 1013904223u;

@@ -23,9 +23,13 @@ def lcg_positions(n, seed=12345):
     return out.T
 
 
-def test_kick_and_drift_through_ct2_signatures(orc):
+@pytest.mark.parametrize("kernel", [None, 0])
+def test_kick_and_drift_through_ct2_signatures(orc, kernel):
+    """Without a kernel argument ct2_gpu_set_options selects ct2's own
+    uniform-density-sphere kernel (kick.cpp:52-60); 0 selects Plummer."""
     n = 512
-    r = subprocess.run([EXE, str(n)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    args = [EXE, str(n)] + ([] if kernel is None else [str(kernel)])
+    r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.strip().splitlines()
     max_rung = int(lines[0].split()[1])
@@ -35,7 +39,7 @@ def test_kick_and_drift_through_ct2_signatures(orc):
     rung = np.array([int(v[6]) for v in rows])
     x0, y0, z0 = lcg_positions(n)
     gm, soft = float(np.float32(1.0 / n)), float(np.float32(0.01))
-    ax, ay, az, _ = orc.kick(x0, y0, z0, gm, soft, orc.KERNEL_PLUMMER)
+    ax, ay, az, _ = orc.kick(x0, y0, z0, gm, soft, orc.KERNEL_CT2 if kernel is None else orc.KERNEL_PLUMMER)
     # ascending (not first) and descending both half-kick: v = 2 F dt/2
     dtk = float(np.float32(0.5) * np.float32(0.01))
     V = np.stack([vx, vy, vz]).astype(np.float64)
@@ -95,7 +99,7 @@ def test_learned_correction_through_kick(ct2, tmp_path):
     zf = np.zeros(n, np.float32)
     ct2.particle_allocate(n)
     ct2.particles_upload(x, y, z, zf, zf, zf, np.zeros(n, np.int8))
-    ct2.set_force_options(float(np.float32(0.01)), float(np.float32(1.0 / n)), 0)
+    ct2.set_force_options(float(np.float32(0.01)), float(np.float32(1.0 / n)), 1)  # the binding's default kernel
     try:
         ct2.pm_assignment(2)
         ct2.pm_init(32, 1.0)
@@ -112,3 +116,43 @@ def test_learned_correction_through_kick(ct2, tmp_path):
     assert np.array_equal(got, np.stack([p.vx, p.vy, p.vz]))
     assert np.array_equal(got_rung, p.rung.astype(int))
     assert np.abs(got).max() > 0
+
+
+def test_grid_accessors_through_ct2_signatures(ct2):
+    """particles_init, particle_sort_by_rung, particle_to_grid and the cell
+    accessors of particle.hpp:58-70 through the binding: the rung range is
+    sorted stably by cell (ct2's Ncell = cbrt(count / ppcell) + 4), each cell's
+    range holds exactly its particles, and the neighbour list of a cell has
+    one entry per offset with i0^2 + j0^2 + k0^2 <= bwidth^2."""
+    r = subprocess.run([EXE, "grid"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    head = lines[0].split()
+    first, last, nc, nnb = int(head[1]), int(head[2]), int(head[4]), int(head[6])
+    n = 20000
+    count = last - first
+    assert nc == int((count // 40) ** (1.0 / 3.0) + 4)
+    b = 2
+    assert nnb == sum(1 for i in range(-b, b + 1) for j in range(-b, b + 1) for k in range(-b, b + 1)
+                      if i * i + j * j + k * k <= b * b)
+    cells = np.array([[int(t) for t in l.split()] for l in lines[1:1 + nc ** 3]])
+    rows = [l.split() for l in lines[1 + nc ** 3:]]
+    x, y, z = (np.array([int(v[a]) for v in rows], np.uint64) for a in range(3))
+    tag = np.array([float(v[3]) for v in rows]).astype(np.int64)
+    rung = np.array([int(v[4]) for v in rows])
+    # particles_init: the library's seeded uniform set
+    x0, y0, z0 = ct2.ic_plummer_mix(n, 0, 0, 0.0, 0.0, 1)
+    assert np.array_equal(x, x0[tag]) and np.array_equal(y, y0[tag]) and np.array_equal(z, z0[tag])
+    r0 = (np.arange(n) // 7) % 2
+    assert first == int((r0 == 0).sum()) and last == n
+    # the slow particles keep the stable partition's order, the range is in cell order
+    assert np.array_equal(tag[:first], np.flatnonzero(r0 == 0)) and (rung[first:] == 1).all()
+    key = ((x[first:] * nc >> 32) * nc + (y[first:] * nc >> 32)) * nc + (z[first:] * nc >> 32)
+    assert np.all(np.diff(key.astype(np.int64)) >= 0)
+    part = np.flatnonzero(r0 == 1)
+    kk0 = ((x0[part].astype(np.uint64) * nc >> 32) * nc + (y0[part].astype(np.uint64) * nc >> 32)) * nc + \
+        (z0[part].astype(np.uint64) * nc >> 32)
+    assert np.array_equal(tag[first:], part[np.argsort(kk0, kind="stable")])
+    starts = first + np.searchsorted(key, np.arange(nc ** 3), side="left")
+    ends = first + np.searchsorted(key, np.arange(nc ** 3), side="right")
+    assert np.array_equal(cells[:, 0], starts) and np.array_equal(cells[:, 1], ends)

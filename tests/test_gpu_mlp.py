@@ -163,3 +163,55 @@ def test_fused_inference_matches_gemm_forward(ct2, orc):
     for ref in (gemm, want):
         err = np.abs(got - ref)
         assert np.median(err) < 2e-3 * scale and np.max(err) < 3e-2 * scale, (np.median(err) / scale, err.max() / scale)
+
+
+def test_kick_pm_mlp_falls_back_to_the_direct_sum(ct2):
+    """The direct-sum kick is the fallback of the learned correction:
+    ct2_kick_pm_mlp runs ct2_kick (bit for bit the same velocities and rungs)
+    when no trained weights are loaded (ct2_mlp_init alone) or the mesh
+    recorded for the weights is not the current one (split radius,
+    assignment, near field, particles per node); with the recorded mesh it
+    takes the learned path.  ct2_last_force_path reports which ran."""
+    side = 16
+    n = side ** 3
+    x, y, z = ct2.ic_lattice(side, 0.2, 3)
+    rng = np.random.default_rng(3)
+    v0 = [rng.standard_normal(n).astype(np.float32) * 0.01 for _ in range(3)]
+    params = ct2.kick_params(ascending=True, descending=True, top=True, rung=0, t0=0.01, eta=0.15, max_dt=0.01)
+
+    def reset():
+        ct2.particles_upload(x, y, z, *v0, np.zeros(n, np.int8))
+
+    ct2.particle_allocate(n)
+    ct2.set_force_options(float(np.float32(1.0 / (50 * side))), float(np.float32(1.0 / n)), 0)
+    reset()
+    want_max = ct2.kick(params, ct2.KICK_STORE)
+    assert ct2.last_force_path() == ct2.PATH_DIRECT
+    want = ct2.particles_download()
+    try:
+        ct2.pm_assignment(2)
+        ct2.pm_init(2 * side, 1.0)
+        ct2.mlp_init(4)  # random weights, never trained
+        for setup in ("untrained", "other split", "other density", "other assignment"):
+            if setup == "other split":
+                ct2.mlp_set_weights(ct2.mlp_get_weights())
+                ct2.mlp_set_mesh(1.25, 2, 0.0, 1.0 / 8)
+            elif setup == "other density":
+                ct2.mlp_set_mesh(1.0, 2, 0.0, 1.0)
+            elif setup == "other assignment":
+                ct2.mlp_set_mesh(1.0, 1, 0.0, 1.0 / 8)
+            reset()
+            mr = ct2.kick_pm_mlp(params, ct2.KICK_STORE)
+            got = ct2.particles_download()
+            assert ct2.last_force_path() == ct2.PATH_DIRECT, setup
+            assert mr == want_max
+            for a in ("vx", "vy", "vz", "rung"):
+                assert np.array_equal(getattr(got, a), getattr(want, a)), (setup, a)
+        ct2.mlp_set_mesh(1.0, 2, 0.0, 1.0 / 8)  # the mesh in use: 16^3 particles on 32^3 nodes
+        reset()
+        ct2.kick_pm_mlp(params, ct2.KICK_STORE)
+        assert ct2.last_force_path() == ct2.PATH_PM_MLP
+        ct2.kick_pm(params, ct2.KICK_STORE)
+        assert ct2.last_force_path() == ct2.PATH_PM
+    finally:
+        ct2.pm_assignment(1)

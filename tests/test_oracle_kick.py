@@ -164,6 +164,33 @@ def test_epilogue_max_dt_clips(orc):
     assert rung[0] == 1  # t0/max_dt = 5 > 2^0
 
 
+def test_epilogue_rung_test_is_exact_at_the_power_of_two(orc):
+    """kick.cpp:92 promotes when ceilf(log2f(t0/dt)) > rung.  Here the test is
+    the exact t0/dt > 2^rung.  They differ only where t0/dt lies within a few
+    FP32 ulp above 2^rung: there log2f rounds down to rung itself and ct2 does
+    not promote.  This case records that intended divergence: t0/dt =
+    16 (1 + 2^-23) with rung 4 promotes here, and the FP32 log2 of the same
+    quotient is exactly 4.0."""
+    t0 = np.float32(1.0)
+    q = np.float32(16.0) * (np.float32(1.0) + np.float32(2.0 ** -23))
+    dt = np.float32(t0 / q)
+    if np.float32(t0 / dt) != q:  # nudge dt by one ulp until the FP32 quotient is q
+        dt = np.nextafter(dt, np.float32(0.0)) if np.float32(t0 / dt) < q else np.nextafter(dt, np.float32(1.0))
+    assert np.float32(t0 / dt) == q and q > np.float32(16.0)
+    assert np.ceil(np.log2(q)) == np.float32(4.0)  # the reference's float evaluation: no promotion
+    vx, vy, vz, rung = _state(1)
+    rung[:] = 0
+    zero = np.zeros(1, np.float32)  # F = 0: dt_i = min(inf, t0, max_dt) = max_dt
+    p = orc.KickParams(ascending=0, descending=1, top=1, first=0, rung=4, t0=float(t0), eta=1.0,
+                       max_dt=float(dt), scale=1.0)
+    orc.kick_epilogue(zero, zero, zero, vx, vy, vz, rung, p, 1.0)
+    assert rung[0] == 5  # exact test: 16 (1 + 2^-23) > 2^4
+    rung[:] = 0
+    p.max_dt = float(np.float32(1.0 / 16.0))  # t0/dt = 16 exactly: no promotion in either form
+    orc.kick_epilogue(zero, zero, zero, vx, vy, vz, rung, p, 1.0)
+    assert rung[0] == 4
+
+
 # ------------------------------------------------------------------ drift
 def exact_fixed32(s: np.float32) -> int:
     s = float(s)
