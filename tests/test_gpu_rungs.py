@@ -158,6 +158,7 @@ def replay_kick_step(tr, minrung, max_rung, first, t0, minrung0, nocrop):
             if len(counts) > minrung0 + 1 and 3 * sum(counts[minrung0 + 1:]) > counts[minrung0]:
                 clip_top = not nocrop
         clipped = clip_top and top
+        tr.expect("to_grid ")  # main.cpp:49: the cells of the range are rebuilt before every kick
         f = tr.expect(f"kick rung {level} ").split()
         got = {f[k]: int(f[k + 1]) for k in (3, 5, 7, 9)}
         assert got == {"asc": int(ascending or top), "desc": int(False if clipped else (not ascending or top)),
