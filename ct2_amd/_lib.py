@@ -83,6 +83,7 @@ _SIGS = {
     "ct2_particles_download": ([_P] * 7, C.c_int),
     "ct2_particles_device_ptrs": ([C.POINTER(ParticlePtrs)], C.c_int),
     "ct2_kick": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
+    "ct2_kick_noalloc": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_forces_download": ([_P] * 4, C.c_int),
     "ct2_kick_sample_fp64": ([_P, C.c_size_t, C.c_int, _P, _P, _P, _P], C.c_int),
     "ct2_drift": ([C.c_float, C.c_float], C.c_int),

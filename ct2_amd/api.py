@@ -115,6 +115,19 @@ def kick(params: KickParams | None = None, flags: int = 0) -> int:
     return mr.value
 
 
+def kick_noalloc(params: KickParams | None = None, flags: int = 0) -> int:
+    """ct2_kick_noalloc: the ordered kick with no device allocation (at most
+    NOALLOC_MAX_SINKS active sinks); returns the max rung."""
+    if params is None:
+        params = kick_params()
+    mr = C.c_int(0)
+    L.check(L.load().ct2_kick_noalloc(C.byref(params), flags, C.byref(mr)))
+    return mr.value
+
+
+NOALLOC_MAX_SINKS = 1 << 18
+
+
 def forces_download():
     n = particle_count()
     out = [np.empty(n, np.float32) for _ in range(4)]

@@ -110,6 +110,7 @@ int ct2_init(int device) {
         return set_error(CT2_ERR_NODEVICE, "device %d is %s; this library is built for gfx950",
                          device, prop.gcnArchName);
     CT2_HIP(hipSetDevice(device));
+    CT2_TRY(kick_noalloc_setup(device));
     CT2_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.device = device;
     return CT2_OK;

@@ -85,6 +85,9 @@ int ewald_judge(const long* d_idx, int m, double* d_out);
 
 // Called by ct2_finalize to release module-owned buffers.
 void release_kick_buffers();
+// Looks up this device's copy of the no-allocation kick's static scratch
+// (the device must be current); ct2_init calls it once per device.
+int kick_noalloc_setup(int device);
 void release_fft_plans();
 void release_pm();
 void release_ewald();

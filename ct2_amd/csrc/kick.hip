@@ -537,18 +537,29 @@ size_t g_part_b_bytes = 0;
 // Ordered pass: sinks [i0, i1) against sources [0, nsrc), source chunks
 // sized so that the grid has enough workgroups to fill 256 CUs; partial sums
 // into *buf (grown as needed).  Returns the set's descriptor.
+// With fixed != nullptr the partial sums go to that buffer of fixed_bytes
+// instead (no allocation): the source split is reduced until they fit.
 int ordered_pass(Context& c, int i0, int i1, int nsrc, bool naive, bool plummer, const KickConst& kc, double** buf,
-                 size_t* bytes, PartialSums* out) {
+                 size_t* bytes, PartialSums* out, double* fixed = nullptr, size_t fixed_bytes = 0) {
     const int ns = i1 - i0;
     const int nb = (ns + KICK_BLOCK - 1) / KICK_BLOCK;
     const int max_split = std::max(1, nsrc / (4 * KICK_TILE));
-    const int nsplit = std::min(max_split, std::max(1, (4096 + nb - 1) / nb));
+    int nsplit = std::min(max_split, std::max(1, (4096 + nb - 1) / nb));
+    const size_t per_split = sizeof(double) * 4 * (size_t)ns;
+    if (fixed) nsplit = (int)std::max<size_t>(1, std::min<size_t>(nsplit, fixed_bytes / per_split));
     int jchunk = (nsrc + nsplit - 1) / nsplit;
     jchunk = (jchunk + KICK_TILE - 1) / KICK_TILE * KICK_TILE;
     const int nsplit_eff = (nsrc + jchunk - 1) / jchunk;
-    CT2_TRY(ensure_scratch((void**)buf, bytes, sizeof(double) * 4 * (size_t)nsplit_eff * ns, "kick partial sums"));
+    double* part = fixed;
+    if (fixed) {
+        if (per_split * nsplit_eff > fixed_bytes)
+            return set_error(CT2_ERR_INVALID, "kick partial sums of %d sinks exceed the fixed %zu-byte buffer", ns,
+                             fixed_bytes);
+    } else {
+        CT2_TRY(ensure_scratch((void**)buf, bytes, per_split * nsplit_eff, "kick partial sums"));
+        part = *buf;
+    }
     const dim3 grid(nb, nsplit_eff);
-    double* part = *buf;
     if (plummer) {
         if (naive)
             kick_direct<CT2_KERNEL_PLUMMER, true><<<grid, KICK_BLOCK, 0, c.stream>>>(c.x, c.y, c.z, i0, i1, nsrc, jchunk, kc, part);
@@ -601,12 +612,70 @@ int symmetric_pass(Context& c, int c0, int m, bool plummer, const KickConst& kc,
     *out = PartialSums{c.partial, ny, c0, m};
     return CT2_OK;
 }
+// The no-allocation kick's scratch: partial sums of up to
+// NOALLOC_MAX_SINKS sinks with one source split (8 MiB), then the max-rung
+// counter.  One copy per GPU, held until the process ends; its address on
+// each device is looked up once at ct2_init (g_scratch_ptr).
+constexpr int NOALLOC_MAX_SINKS = CT2_NOALLOC_MAX_SINKS;
+constexpr size_t NOALLOC_PART_BYTES = sizeof(double) * 4 * (size_t)NOALLOC_MAX_SINKS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_PART_BYTES / sizeof(double) + 32];
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
 double* g_sample_out = nullptr;
 size_t g_sample_bytes = 0;
 long* g_sample_idx = nullptr;
 size_t g_sample_idx_bytes = 0;
 
+KickConst kick_const(double h) {
+    const double two32 = 4294967296.0;
+    KickConst kc;
+    kc.eps2u = (float)((h * two32) * (h * two32));
+    kc.h2u = kc.eps2u;
+    kc.hinvu = (float)(1.0 / (h * two32));
+    kc.hinv3u = (float)(1.0 / ((h * two32) * (h * two32) * (h * two32)));
+    return kc;
+}
+
+// Sums the partial-sum sets, stores the force and runs the epilogue, then
+// hands back kick()'s return value from the device counter maxrung.
+int finish_kick(Context& c, const ct2_kick_params* p, const PartialSums* sets, int n, int i0, int* maxrung,
+                int* max_rung) {
+    const double two32 = 4294967296.0;
+    const EpiConst e = make_epilogue(p, c.opts.hsoft);
+    // force in box units: sum(dxu * w3u) * 2^64 * GM; potential: sum(wu) * 2^32 * GM
+    const double fscale = (double)c.opts.GM * two32 * two32;
+    const double pscale = (double)c.opts.GM * two32;
+    kick_finalize<<<(n - i0 + 255) / 256, 256, 0, c.stream>>>(sets[0], sets[1], sets[2], n, i0, fscale, pscale, e,
+                                                              c.vx, c.vy, c.vz, c.rung, c.fx, c.fy, c.fz, c.pot,
+                                                              maxrung);
+    CT2_LAUNCH_CHECK();
+    c.forces_valid = true;
+    c.last_path = CT2_PATH_DIRECT;
+    if (max_rung) {
+        CT2_HIP(hipMemcpyAsync(max_rung, maxrung, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        CT2_HIP(hipStreamSynchronize(c.stream));
+    }
+    return CT2_OK;
+}
+
 }  // namespace
+
+int kick_noalloc_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return set_error(CT2_ERR_INVALID, "device %d beyond the %d devices the no-allocation kick supports", device,
+                         NOALLOC_MAX_DEVICES);
+    if (g_scratch_ptr[device]) return CT2_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return set_error(CT2_ERR_HIP, "hipGetSymbolAddress of the no-allocation kick scratch: %s",
+                         hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = ptr;
+    return CT2_OK;
+}
 
 void release_kick_buffers() {
     Context& c = ctx();
@@ -683,14 +752,7 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
         c.last_path = CT2_PATH_DIRECT;
         return CT2_OK;
     }
-    const double two32 = 4294967296.0;
-    const double h = c.opts.hsoft;
-    KickConst kc;
-    kc.eps2u = (float)((h * two32) * (h * two32));
-    kc.h2u = kc.eps2u;
-    kc.hinvu = (float)(1.0 / (h * two32));
-    kc.hinv3u = (float)(1.0 / ((h * two32) * (h * two32) * (h * two32)));
-
+    const KickConst kc = kick_const(c.opts.hsoft);
     const bool naive = flags & CT2_KICK_NAIVE;
     const bool plummer = c.opts.kernel == CT2_KERNEL_PLUMMER;
     zero_int<<<1, 1, 0, c.stream>>>(c.maxrung);
@@ -713,21 +775,40 @@ extern "C" int ct2_kick(const ct2_kick_params* p, unsigned flags, int* max_rung)
         CT2_TRY(ordered_pass(c, i0, n, n, naive, plummer, kc, &c.partial, &c.partial_bytes, &sets[0]));
     }
 
-    const EpiConst e = make_epilogue(p, c.opts.hsoft);
-    // force in box units: sum(dxu * w3u) * 2^64 * GM; potential: sum(wu) * 2^32 * GM
-    const double fscale = (double)c.opts.GM * two32 * two32;
-    const double pscale = (double)c.opts.GM * two32;
-    kick_finalize<<<(ns + 255) / 256, 256, 0, c.stream>>>(sets[0], sets[1], sets[2], n, i0, fscale, pscale, e, c.vx,
-                                                          c.vy, c.vz, c.rung, c.fx, c.fy, c.fz, c.pot, c.maxrung);
-    CT2_LAUNCH_CHECK();
-    c.forces_valid = true;
-    c.last_path = CT2_PATH_DIRECT;
-    if (max_rung) {
-        CT2_HIP(hipMemcpyAsync(max_rung, c.maxrung, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        CT2_HIP(hipStreamSynchronize(c.stream));
-    }
+    CT2_TRY(finish_kick(c, p, sets, n, i0, c.maxrung, max_rung));
     (void)(flags & CT2_KICK_STORE);  // forces are always kept; the flag documents intent
     return CT2_OK;
+}
+
+// ct2_kick's ordered path with every buffer in the static scratch: no
+// allocation.  Calls that overlap on one GPU share the scratch and race.
+extern "C" int ct2_kick_noalloc(const ct2_kick_params* p, unsigned flags, int* max_rung) {
+    CT2_TRY(require_particles());
+    CT2_TRY(check_kick_params(p));
+    if (flags & ~(unsigned)(CT2_KICK_NAIVE | CT2_KICK_STORE | CT2_KICK_ORDERED))
+        return set_error(CT2_ERR_INVALID, "ct2_kick_noalloc takes CT2_KICK_NAIVE, CT2_KICK_STORE and CT2_KICK_ORDERED only"
+                                          " (flags 0x%x)", flags);
+    Context& c = ctx();
+    void* base = (c.device >= 0 && c.device < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[c.device] : nullptr;
+    if (!base) return set_error(CT2_ERR_STATE, "no-allocation scratch not set up on device %d", c.device);
+    const int n = (int)c.n;
+    const int i0 = (int)c.rung_begin;
+    const int ns = n - i0;
+    if (ns > NOALLOC_MAX_SINKS)
+        return set_error(CT2_ERR_INVALID, "%d active sinks exceed the no-allocation kick's maximum of %d", ns,
+                         NOALLOC_MAX_SINKS);
+    if (ns <= 0) {
+        if (max_rung) *max_rung = 0;
+        c.last_path = CT2_PATH_DIRECT;
+        return CT2_OK;
+    }
+    int* maxrung = (int*)((char*)base + NOALLOC_PART_BYTES);
+    zero_int<<<1, 1, 0, c.stream>>>(maxrung);
+    CT2_LAUNCH_CHECK();
+    PartialSums sets[3] = {{nullptr, 0, 0, 0}, {nullptr, 0, 0, 0}, {nullptr, 0, 0, 0}};
+    CT2_TRY(ordered_pass(c, i0, n, n, flags & CT2_KICK_NAIVE, c.opts.kernel == CT2_KERNEL_PLUMMER,
+                         kick_const(c.opts.hsoft), nullptr, nullptr, &sets[0], (double*)base, NOALLOC_PART_BYTES));
+    return finish_kick(c, p, sets, n, i0, maxrung, max_rung);
 }
 
 extern "C" int ct2_kick_sample_fp64(const int64_t* idx, size_t m, int periodic, double* ax, double* ay,

@@ -116,6 +116,21 @@ typedef struct {
  * the remaining sinks are summed pair by pair.  Without a core every sink
  * sums over every source.  All paths are deterministic. */
 int ct2_kick(const ct2_kick_params* params, unsigned flags, int* max_rung);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same kick as ct2_kick with CT2_KICK_ORDERED (every
+ * sink sums over every source, same kernels, the same epilogue and return
+ * value) whose partial sums and max-rung counter live in a static 8 MiB
+ * device array instead of allocated buffers; it calls no allocator.  At most
+ * CT2_NOALLOC_MAX_SINKS active sinks (the rung range [i0, n); sources are all
+ * n particles), else CT2_ERR_INVALID and nothing runs.  Flags: CT2_KICK_NAIVE,
+ * CT2_KICK_STORE, CT2_KICK_ORDERED.  Each GPU holds one copy of the array for
+ * the life of the process and every call on that GPU shares it: calls that
+ * overlap on one GPU (other threads, other streams) race.  Results equal
+ * ct2_kick(CT2_KICK_ORDERED) bit for bit wherever ct2_kick's partial sums of
+ * the same sinks fit in 8 MiB, and agree to FP32 rounding otherwise. */
+#define CT2_NOALLOC_MAX_SINKS (1 << 18)
+int ct2_kick_noalloc(const ct2_kick_params* params, unsigned flags, int* max_rung);
 /* The force (GM-scaled, toward the sources) and positive potential of the
  * last ct2_kick made with CT2_KICK_STORE.  NULL skips an array. */
 int ct2_forces_download(float* fx, float* fy, float* fz, float* pot);
