@@ -19,6 +19,7 @@ Prints one JSON line.  value = interactions/s of the kick.
 from __future__ import annotations
 
 import argparse
+import subprocess
 import ctypes as C
 import json
 import os
@@ -38,7 +39,11 @@ HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
 # HBM bytes per kick_sym_pairs launch (64 per kick) at N = 128^3 from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE runs scaled by a plain 1 GiB copy (profiles/kick_sym_n128cubed.md); None at other sizes
-KICK_TRAFFIC_BYTES = {128: 1.77e9}
+KICK_TRAFFIC_BYTES = {128: 1.82e9}
+# FP32 flop per unordered pair evaluation counted by SQ_INSTS_VALU_FLOPS_FP32 over kick_sym_pairs,
+# and VALU instructions per pair evaluation (profiles/kick_sym_valu_counters_n128cubed.md)
+KICK_HW_FLOP_PER_EVAL = 26.19
+KICK_VALU_PER_EVAL = 19.07
 
 
 def rel_err(F, J):
@@ -324,15 +329,31 @@ def cpu_baseline(x, y, z, soft, gm, target_s: float):
 
 
 def cpu_model() -> str:
-    """The host CPU's model name (as lscpu reports it, from /proc/cpuinfo)."""
+    """The host CPU's model name as `lscpu` reports it (else /proc/cpuinfo),
+    with the family/model/stepping numbers where the name is generic."""
+    name, fms = "unknown", []
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                name = v
+            elif k in ("CPU family", "Model", "Stepping"):
+                fms.append(f"{k.lower()} {v}")
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    if name == "unknown":
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        name = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return f"{name} ({', '.join(fms)})" if fms else name
 
 
 def cpu_case(threads: int) -> dict:
@@ -539,7 +560,15 @@ def main() -> int:
                      "traffic_per": "kick_sym_pairs launch (64 per kick); achieved = 20 N^2 / kick time",
                      # information only: the same kick at 38 flop per interaction (the
                      # Warren et al. 1997 / Hamada et al. 2009 count)
-                     "info_38flop_tflops": round(achieved_tflops * 38 / FLOP_PER_INTERACTION, 3)},
+                     "info_38flop_tflops": round(achieved_tflops * 38 / FLOP_PER_INTERACTION, 3),
+                     # information only: the work the hardware executes, each unordered pair once
+                     # (N^2 / 2 evaluations) at the counted flop and VALU per evaluation
+                     "info_executed": {"pair_evaluations": inter / 2,
+                                       "hw_flop_per_evaluation": KICK_HW_FLOP_PER_EVAL,
+                                       "valu_per_evaluation": KICK_VALU_PER_EVAL,
+                                       "hw_tflops": round(KICK_HW_FLOP_PER_EVAL * inter / 2 / (kick_avg * 1e-3) / 1e12, 3),
+                                       "hw_frac": round(KICK_HW_FLOP_PER_EVAL * inter / 2 / (kick_avg * 1e-3) / 1e12
+                                                        / FP32_PEAK_TFLOPS, 4)}},
         "fft_fp32": fft,
         "drift": {"ms": round(drift_avg, 4), "GB/s": round(36 * n / drift_avg / 1e6, 1),
                   "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <hipcub/hipcub.hpp>
+
 #include "epilogue.h"
 #include "internal.h"
 
@@ -241,9 +243,6 @@ constexpr int SYM_BLOCK = 64 * SYM_WAVES;
 #ifndef SYM_NS
 #define SYM_NS 4  // sinks per lane in the off-diagonal rounds
 #endif
-#ifndef SYM_UNROLL
-#define SYM_UNROLL 1  // measured fastest: unroll 2 (124 VGPRs, still 4 waves/SIMD) 1,018 ms vs 988 per kick
-#endif
 constexpr int SYM_THREADS = SYM_BLOCK / SYM_NS;
 // smallest core the symmetric kernels take by default: 64 superblocks.  A
 // round takes ~0.15 ms per workgroup whatever the core size, so cores of
@@ -356,18 +355,152 @@ __global__ __launch_bounds__(SYM_BLOCK) void kick_sym_diag(const uint4* __restri
     part[(3L * ny) * n + i] = (double)sp + (double)cp + jacc[3][tid];
 }
 
-// Off-diagonal rounds.  Each lane carries NS sinks (the wave 64*NS of the
-// superblock's 1024), all paired with the same source at every step, so one
-// rotation of the source-side sums serves NS pairs.
+// Off-diagonal rounds on spatially ordered superblocks.  Each lane carries NS
+// sinks (the wave 64*NS consecutive particles of the superblock's 1024), all
+// paired with the same source at every step, so one rotation of the
+// source-side sums serves NS pairs.  Sources stream through LDS one chunk of
+// 64 at a time (double-buffered: the next chunk is fetched during the current
+// one, one barrier per chunk).
+//
+// Separations in FP32 offsets.  Every chunk has a reference point (the
+// centre of its bounding box) and the exact integer range [lo, hi] of its
+// sources' offsets from it (ChunkRef).  Per chunk a wave forms its sinks'
+// offsets u = (int32)(x_i - ref) and the sources' v = (int32)(x_j - ref)
+// are in LDS as floats, so a separation is one FP32 subtraction v - u
+// instead of an integer subtraction and a conversion.  That is the exact
+// minimum image whenever v - u cannot leave [-2^31, 2^31) for any pair of
+// the (wave, chunk) block; the wave decides that per axis from its sinks'
+// integer range (computed once) and the chunk's, in scalar arithmetic, and
+// takes the exact integer separation on any axis where it could (MASK bit).
+// The offsets are rounded to FP32 once each, so a separation carries an
+// error of at most 2^-24 (|u| + |v|): relative to the separation, 2^-24
+// times (1 + twice the chunk's half extent / separation).
+struct ChunkRef {
+    uint4 ref;  // x, y, z reference (fixed32), w unused
+    int4 lo;    // per-axis minimum of (int32)(x_j - ref) over the chunk
+    int4 hi;    // per-axis maximum
+};
+
+template <int KERNEL>
+__device__ __forceinline__ void pair_kd(float dx, float dy, float dz, const KickConst& kc, float& k, float& p) {
+    if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, kc.eps2u)));
+        const float w = __builtin_amdgcn_rsqf(r2);
+        k = (w * w) * w;
+        p = w;
+    } else {
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const bool near = r2 < kc.h2u;
+        const float w = __builtin_amdgcn_rsqf(near ? kc.h2u : r2);
+        k = near ? kc.hinv3u : (w * w) * w;
+        p = near ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : w;
+    }
+}
+
+// Measured on gfx950 at N = 2^21 (ms per kick, one box): as written 915-941;
+// unrolling the step loop by 2 or 4, keeping the softening constant in a VGPR
+// (v_fma_f32 with three VGPR operands) or rotating the source-side sums with
+// ds_bpermute instead of DPP: 977-1,099 in every combination.
+#ifndef SYM_STEP_UNROLL
+#define SYM_STEP_UNROLL 1
+#endif
+
+// The 64 steps of one (wave, chunk) block.  MASK bit a: axis a takes the
+// exact integer separation (int sources Si, int sinks xi) instead of v - u.
+template <int KERNEL, int MASK>
+__device__ __forceinline__ void sym_chunk(const float4* Sf, const uint4* Si, const float (&ux)[SYM_NS],
+                                          const float (&uy)[SYM_NS], const float (&uz)[SYM_NS],
+                                          const uint32_t (&xi)[SYM_NS], const uint32_t (&yi)[SYM_NS],
+                                          const uint32_t (&zi)[SYM_NS], const KickConst& kc, float (&tx)[SYM_NS],
+                                          float (&ty)[SYM_NS], float (&tz)[SYM_NS], float (&tp)[SYM_NS], float& ax,
+                                          float& ay, float& az, float& ap) {
+#pragma unroll SYM_STEP_UNROLL
+    for (int t = 0; t < 64; ++t) {
+        const float4 sf = Sf[t];
+        uint4 si = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (MASK != 0) si = Si[t];
+        float gx, gy, gz, gp;  // source-side sum of this step, started by sink 0
+#pragma unroll
+        for (int q = 0; q < SYM_NS; ++q) {
+            const float dx = (MASK & 1) ? (float)(int32_t)(si.x - xi[q]) : sf.x - ux[q];
+            const float dy = (MASK & 2) ? (float)(int32_t)(si.y - yi[q]) : sf.y - uy[q];
+            const float dz = (MASK & 4) ? (float)(int32_t)(si.z - zi[q]) : sf.z - uz[q];
+            float k, p;
+            pair_kd<KERNEL>(dx, dy, dz, kc, k, p);
+            tx[q] = fmaf(dx, k, tx[q]);
+            ty[q] = fmaf(dy, k, ty[q]);
+            tz[q] = fmaf(dz, k, tz[q]);
+            tp[q] += p;
+            if (q == 0) {
+                gx = dx * k;
+                gy = dy * k;
+                gz = dz * k;
+                gp = p;
+            } else {
+                gx = fmaf(dx, k, gx);
+                gy = fmaf(dy, k, gy);
+                gz = fmaf(dz, k, gz);
+                gp += p;
+            }
+        }
+        ax = rol1(ax) - gx;
+        ay = rol1(ay) - gy;
+        az = rol1(az) - gz;
+        ap = rol1(ap) + gp;
+    }
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Whether v - u stays in [-2^31, 2^31) for every sink offset u and source
+// offset v of a (wave, chunk) block on one axis: the wave's sinks lie at
+// [ulo, uhi] from its own reference rw, the chunk's sources at [vlo, vhi]
+// from ref (all exact integers, scalar).
+__device__ __forceinline__ bool axis_float_safe(uint32_t rw, int ulo, int uhi, uint32_t ref, int vlo, int vhi) {
+    const long d = (long)(int32_t)(ref - rw);
+    const long lo = (long)ulo - d, hi = (long)uhi - d;  // the sinks' offsets from ref, unwrapped
+    if (lo < -2147483648L || hi > 2147483647L) return false;
+    return (long)vhi - lo < 2147483648L && (long)vlo - hi >= -2147483648L;
+}
+
+// Stores chunk (J, c) of P into LDS buffer (float offsets from its reference
+// and the integers, each twice in a row): lane l of one wave, source l.
+__device__ __forceinline__ void sym_store_chunk(const uint4 src, const ChunkRef& cr, float4* Sf, uint4* Si,
+                                                int lane) {
+    const float4 f = make_float4((float)(int32_t)(src.x - cr.ref.x), (float)(int32_t)(src.y - cr.ref.y),
+                                 (float)(int32_t)(src.z - cr.ref.z), 0.0f);
+    Sf[lane] = f;
+    Sf[lane + 64] = f;
+    Si[lane] = src;
+    Si[lane + 64] = src;
+}
+
 template <int KERNEL>
 // blockIdx.y = q of gridDim.y round groups: workgroup (g, q) takes rounds
 // r0 + q, r0 + q + gridDim.y, ... < r1, so that small cores still fill the
 // GPU; group q adds its sink-side sums to plane q == 0 ? 0 : R + q.
-__global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __restrict__ P, int n, int ny, int gs,
-                                                              int r0, int r1, int R, KickConst kc,
+__global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __restrict__ P,
+                                                              const ChunkRef* __restrict__ refs, int n, int ny,
+                                                              int gs, int r0, int r1, int R, KickConst kc,
                                                               double* __restrict__ part) {
     constexpr int NWAVE = SYM_THREADS / 64;
-    __shared__ uint4 S[SYM_WAVES][128];
+    __shared__ float4 Sf[2][128];
+    __shared__ uint4 Si[2][128];
     __shared__ float4 jr[2][NWAVE][64];  // double-buffered: one barrier per chunk
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int g = blockIdx.x;
@@ -381,51 +514,84 @@ __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __res
         zi[q] = me.z;
         sx[q] = sy[q] = sz[q] = sp[q] = cx[q] = cy[q] = cz[q] = cp[q] = 0.f;
     }
-    for (int r = r0 + (int)blockIdx.y; r < r1; r += (int)gridDim.y) {
-        if ((gs & 1) == 0 && r == gs / 2 && g >= gs / 2) continue;  // that pair belongs to g - gs/2
+    // the wave's sinks: integer offsets from its first sink, range per axis
+    const uint32_t rwx = (uint32_t)__builtin_amdgcn_readfirstlane((int)xi[0]);
+    const uint32_t rwy = (uint32_t)__builtin_amdgcn_readfirstlane((int)yi[0]);
+    const uint32_t rwz = (uint32_t)__builtin_amdgcn_readfirstlane((int)zi[0]);
+    int ulx = INT32_MAX, uhx = INT32_MIN, uly = INT32_MAX, uhy = INT32_MIN, ulz = INT32_MAX, uhz = INT32_MIN;
+#pragma unroll
+    for (int q = 0; q < SYM_NS; ++q) {
+        const int ux = (int32_t)(xi[q] - rwx), uy = (int32_t)(yi[q] - rwy), uz = (int32_t)(zi[q] - rwz);
+        ulx = min(ulx, ux);
+        uhx = max(uhx, ux);
+        uly = min(uly, uy);
+        uhy = max(uhy, uy);
+        ulz = min(ulz, uz);
+        uhz = max(uhz, uz);
+    }
+    ulx = wave_min_i32(ulx);
+    uhx = wave_max_i32(uhx);
+    uly = wave_min_i32(uly);
+    uhy = wave_max_i32(uhy);
+    ulz = wave_min_i32(ulz);
+    uhz = wave_max_i32(uhz);
+
+    const bool even = (gs & 1) == 0;
+    auto valid = [&](int r) { return !(even && r == gs / 2 && g >= gs / 2); };  // that pair belongs to g - gs/2
+    auto next_round = [&](int r) {
+        r += (int)gridDim.y;
+        while (r < r1 && !valid(r)) r += (int)gridDim.y;
+        return r;
+    };
+    int r = r0 + (int)blockIdx.y;
+    while (r < r1 && !valid(r)) r += (int)gridDim.y;
+    if (r < r1 && w == 0) {
         const long J = (g + r) % gs;
-        __syncthreads();
-        for (int e = tid; e < SYM_BLOCK; e += SYM_THREADS) {
-            const uint4 src = P[J * SYM_BLOCK + e];
-            S[e >> 6][e & 63] = src;
-            S[e >> 6][(e & 63) + 64] = src;
-        }
-        __syncthreads();
+        sym_store_chunk(P[J * SYM_BLOCK + lane], refs[J * SYM_WAVES], Sf[0], Si[0], lane);
+    }
+    __syncthreads();
+    int b = 0;
+    for (; r < r1; r = next_round(r)) {
+        const long J = (g + r) % gs;
         for (int c = 0; c < SYM_WAVES; ++c) {
+            // the next chunk, fetched now and stored after this one's steps
+            const int rn = c + 1 < SYM_WAVES ? r : next_round(r);
+            const int cn = c + 1 < SYM_WAVES ? c + 1 : 0;
+            const long Jn = (g + rn) % gs;
+            uint4 pre = make_uint4(0u, 0u, 0u, 0u);
+            if (rn < r1 && w == 0) pre = P[Jn * SYM_BLOCK + cn * 64 + lane];
+            const ChunkRef cr = refs[J * SYM_WAVES + c];
+            const int mask = (axis_float_safe(rwx, ulx, uhx, cr.ref.x, cr.lo.x, cr.hi.x) ? 0 : 1) |
+                             (axis_float_safe(rwy, uly, uhy, cr.ref.y, cr.lo.y, cr.hi.y) ? 0 : 2) |
+                             (axis_float_safe(rwz, ulz, uhz, cr.ref.z, cr.lo.z, cr.hi.z) ? 0 : 4);
+            float ux[SYM_NS], uy[SYM_NS], uz[SYM_NS];
+#pragma unroll
+            for (int q = 0; q < SYM_NS; ++q) {
+                ux[q] = (float)(int32_t)(xi[q] - cr.ref.x);
+                uy[q] = (float)(int32_t)(yi[q] - cr.ref.y);
+                uz[q] = (float)(int32_t)(zi[q] - cr.ref.z);
+            }
             float tx[SYM_NS], ty[SYM_NS], tz[SYM_NS], tp[SYM_NS];
 #pragma unroll
             for (int q = 0; q < SYM_NS; ++q) tx[q] = ty[q] = tz[q] = tp[q] = 0.f;
             float ax = 0.f, ay = 0.f, az = 0.f, ap = 0.f;
-            const uint4* Sl = S[c] + lane;
-#pragma unroll SYM_UNROLL
-            for (int t = 0; t < 64; ++t) {
-                const uint4 src = Sl[t];
-                float gx, gy, gz, gp;  // source-side sum of this step, started by sink 0
-#pragma unroll
-                for (int q = 0; q < SYM_NS; ++q) {
-                    float dx, dy, dz, k, p;
-                    pair_k<KERNEL>(src, xi[q], yi[q], zi[q], kc, dx, dy, dz, k, p);
-                    tx[q] = fmaf(dx, k, tx[q]);
-                    ty[q] = fmaf(dy, k, ty[q]);
-                    tz[q] = fmaf(dz, k, tz[q]);
-                    tp[q] += p;
-                    if (q == 0) {
-                        gx = dx * k;
-                        gy = dy * k;
-                        gz = dz * k;
-                        gp = p;
-                    } else {
-                        gx = fmaf(dx, k, gx);
-                        gy = fmaf(dy, k, gy);
-                        gz = fmaf(dz, k, gz);
-                        gp += p;
-                    }
-                }
-                ax = rol1(ax) - gx;
-                ay = rol1(ay) - gy;
-                az = rol1(az) - gz;
-                ap = rol1(ap) + gp;
+            const float4* Sl = Sf[b] + lane;
+            const uint4* Il = Si[b] + lane;
+#define CT2_SYM_CASE(M)                                                                                          \
+    case M:                                                                                                      \
+        sym_chunk<KERNEL, M>(Sl, Il, ux, uy, uz, xi, yi, zi, kc, tx, ty, tz, tp, ax, ay, az, ap);                \
+        break;
+            switch (mask) {
+                CT2_SYM_CASE(0)
+                CT2_SYM_CASE(1)
+                CT2_SYM_CASE(2)
+                CT2_SYM_CASE(3)
+                CT2_SYM_CASE(4)
+                CT2_SYM_CASE(5)
+                CT2_SYM_CASE(6)
+                CT2_SYM_CASE(7)
             }
+#undef CT2_SYM_CASE
 #pragma unroll
             for (int q = 0; q < SYM_NS; ++q) {
                 two_sum(sx[q], cx[q], tx[q]);
@@ -433,17 +599,20 @@ __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __res
                 two_sum(sz[q], cz[q], tz[q]);
                 two_sum(sp[q], cp[q], tp[q]);
             }
-            jr[c & 1][w][(lane + 63) & 63] = make_float4(ax, ay, az, ap);
+            jr[b][w][(lane + 63) & 63] = make_float4(ax, ay, az, ap);
+            // buffer b^1 held the previous chunk, whose steps ended before the last barrier
+            if (rn < r1 && w == 0) sym_store_chunk(pre, refs[Jn * SYM_WAVES + cn], Sf[b ^ 1], Si[b ^ 1], lane);
             __syncthreads();
             if (tid < 256) {
                 const int comp = tid >> 6, jj = tid & 63;
                 double acc = 0.0;
 #pragma unroll
-                for (int v = 0; v < NWAVE; ++v) acc += (double)((const float*)&jr[c & 1][v][jj])[comp];
+                for (int v = 0; v < NWAVE; ++v) acc += (double)((const float*)&jr[b][v][jj])[comp];
                 // the only update of this slot in this launch: a no-return FP64
                 // atomic keeps the wave from waiting on the read-modify-write
                 unsafeAtomicAdd(part + ((long)comp * ny + 1 + (r - r0)) * n + J * SYM_BLOCK + c * 64 + jj, acc);
             }
+            b ^= 1;
         }
     }
     const long sp0 = blockIdx.y == 0 ? 0 : R + blockIdx.y;  // this group's sink-side plane
@@ -457,10 +626,63 @@ __global__ __launch_bounds__(SYM_THREADS) void kick_sym_pairs(const uint4* __res
     }
 }
 
-__global__ __launch_bounds__(256) void pack_positions(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
-                                                      const uint32_t* __restrict__ Z, int n, uint4* __restrict__ P) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) P[i] = make_uint4(X[i], Y[i], Z[i], 0u);
+// Spatial order of the symmetric core: the key of particle s is the Morton
+// index of its cell in a 1024^3 grid (10 bits per axis, x highest).
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void sym_keys(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                const uint32_t* __restrict__ Z, int m, uint32_t* __restrict__ key,
+                                                uint32_t* __restrict__ idx) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= m) return;
+    key[s] = (spread3(X[s] >> 22) << 2) | (spread3(Y[s] >> 22) << 1) | spread3(Z[s] >> 22);
+    idx[s] = (uint32_t)s;
+}
+
+// P[s] = (x, y, z, core index) of the particle at sorted slot s.
+__global__ __launch_bounds__(256) void sym_gather(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                  const uint32_t* __restrict__ Z, const uint32_t* __restrict__ idx,
+                                                  int m, uint4* __restrict__ P) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= m) return;
+    const uint32_t i = idx[s];
+    P[s] = make_uint4(X[i], Y[i], Z[i], i);
+}
+
+// One wave per chunk of 64: its bounding-box centre and the exact range of
+// the offsets from it.
+__global__ __launch_bounds__(64) void sym_chunk_refs(const uint4* __restrict__ P, ChunkRef* __restrict__ refs) {
+    const uint4 p = P[(long)blockIdx.x * 64 + threadIdx.x];
+    const uint32_t lx = wave_min_u32(p.x), hx = wave_max_u32(p.x);
+    const uint32_t ly = wave_min_u32(p.y), hy = wave_max_u32(p.y);
+    const uint32_t lz = wave_min_u32(p.z), hz = wave_max_u32(p.z);
+    const uint4 ref = make_uint4(lx + (hx - lx) / 2, ly + (hy - ly) / 2, lz + (hz - lz) / 2, 0u);
+    const int vx = (int32_t)(p.x - ref.x), vy = (int32_t)(p.y - ref.y), vz = (int32_t)(p.z - ref.z);
+    const int4 lo = make_int4(wave_min_i32(vx), wave_min_i32(vy), wave_min_i32(vz), 0);
+    const int4 hi = make_int4(wave_max_i32(vx), wave_max_i32(vy), wave_max_i32(vz), 0);
+    if (threadIdx.x == 0) refs[blockIdx.x] = ChunkRef{ref, lo, hi};
+}
+
+// Sums the planes of the core's partial sums in plane order for each sorted
+// slot and writes the totals to the core index (out[c * m + i]).
+__global__ __launch_bounds__(256) void sym_unsort(const double* __restrict__ part, int ny, int m,
+                                                  const uint4* __restrict__ P, double* __restrict__ out) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= m) return;
+    const uint32_t i = P[s].w;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double v = 0.0;
+        for (int y = 0; y < ny; ++y) v += part[((long)c * ny + y) * m + s];
+        out[(long)c * m + i] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void fill_f64(double* __restrict__ p, long n) {
@@ -527,8 +749,16 @@ __global__ __launch_bounds__(256) void kick_sample_fp64(const uint32_t* __restri
     }
 }
 
-uint4* g_packed = nullptr;
+uint4* g_packed = nullptr;  // the symmetric core in spatial order: (x, y, z, core index)
 size_t g_packed_bytes = 0;
+ChunkRef* g_refs = nullptr;  // per chunk of 64 of g_packed
+size_t g_refs_bytes = 0;
+uint32_t* g_sort_kv = nullptr;  // keys in/out, indices in/out of the spatial sort
+size_t g_sort_kv_bytes = 0;
+void* g_sort_tmp = nullptr;
+size_t g_sort_tmp_bytes = 0;
+double* g_sym_out = nullptr;  // the core's summed partials, by core index
+size_t g_sym_out_bytes = 0;
 double* g_part_a = nullptr;  // ordered pass beside the symmetric core: core sinks x other sources
 size_t g_part_a_bytes = 0;
 double* g_part_b = nullptr;  // ordered pass of the sinks before the core x all sources
@@ -589,7 +819,21 @@ int symmetric_pass(Context& c, int c0, int m, bool plummer, const KickConst& kc,
     const int ny = R + groups;  // sink-side planes 0 and R+1 .. R+groups-1, source-side planes 1 .. R
     CT2_TRY(ensure_scratch((void**)&c.partial, &c.partial_bytes, plane * ny, "kick partial sums"));
     CT2_TRY(ensure_scratch((void**)&g_packed, &g_packed_bytes, sizeof(uint4) * (size_t)m, "packed positions"));
-    pack_positions<<<(m + 255) / 256, 256, 0, c.stream>>>(c.x + c0, c.y + c0, c.z + c0, m, g_packed);
+    CT2_TRY(ensure_scratch((void**)&g_refs, &g_refs_bytes, sizeof(ChunkRef) * (size_t)(m / 64), "chunk references"));
+    CT2_TRY(ensure_scratch((void**)&g_sort_kv, &g_sort_kv_bytes, sizeof(uint32_t) * 4 * (size_t)m, "spatial sort"));
+    CT2_TRY(ensure_scratch((void**)&g_sym_out, &g_sym_out_bytes, plane, "core partial sums"));
+    // spatial order: (Morton key, core index) pairs sorted stably by key
+    uint32_t *kin = g_sort_kv, *kout = g_sort_kv + m, *vin = g_sort_kv + 2L * m, *vout = g_sort_kv + 3L * m;
+    const unsigned pb = (unsigned)((m + 255) / 256);
+    sym_keys<<<pb, 256, 0, c.stream>>>(c.x + c0, c.y + c0, c.z + c0, m, kin, vin);
+    CT2_LAUNCH_CHECK();
+    size_t tmp = 0;
+    CT2_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, m, 0, 30, c.stream));
+    CT2_TRY(ensure_scratch(&g_sort_tmp, &g_sort_tmp_bytes, std::max<size_t>(tmp, 256), "spatial sort"));
+    CT2_HIP(hipcub::DeviceRadixSort::SortPairs(g_sort_tmp, tmp, kin, kout, vin, vout, m, 0, 30, c.stream));
+    sym_gather<<<pb, 256, 0, c.stream>>>(c.x + c0, c.y + c0, c.z + c0, vout, m, g_packed);
+    CT2_LAUNCH_CHECK();
+    sym_chunk_refs<<<m / 64, 64, 0, c.stream>>>(g_packed, g_refs);
     CT2_LAUNCH_CHECK();
     const long total = 4L * ny * m;
     fill_f64<<<(unsigned)((total + 255) / 256), 256, 0, c.stream>>>(c.partial, total);
@@ -602,14 +846,16 @@ int symmetric_pass(Context& c, int c0, int m, bool plummer, const KickConst& kc,
     for (int r0 = 1; r0 <= rmax; r0 += R) {
         const int r1 = std::min(r0 + R, rmax + 1);
         if (plummer)
-            kick_sym_pairs<CT2_KERNEL_PLUMMER><<<dim3(gs, groups), SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1,
-                                                                                              R, kc, c.partial);
+            kick_sym_pairs<CT2_KERNEL_PLUMMER><<<dim3(gs, groups), SYM_THREADS, 0, c.stream>>>(
+                g_packed, g_refs, m, ny, gs, r0, r1, R, kc, c.partial);
         else
-            kick_sym_pairs<CT2_KERNEL_CT2><<<dim3(gs, groups), SYM_THREADS, 0, c.stream>>>(g_packed, m, ny, gs, r0, r1, R,
-                                                                                          kc, c.partial);
+            kick_sym_pairs<CT2_KERNEL_CT2><<<dim3(gs, groups), SYM_THREADS, 0, c.stream>>>(
+                g_packed, g_refs, m, ny, gs, r0, r1, R, kc, c.partial);
         CT2_LAUNCH_CHECK();
     }
-    *out = PartialSums{c.partial, ny, c0, m};
+    sym_unsort<<<pb, 256, 0, c.stream>>>(c.partial, ny, m, g_packed, g_sym_out);
+    CT2_LAUNCH_CHECK();
+    *out = PartialSums{g_sym_out, 1, c0, m};
     return CT2_OK;
 }
 // The no-allocation kick's scratch: partial sums of up to
@@ -684,9 +930,11 @@ void release_kick_buffers() {
     c.partial_bytes = 0;
     dev_free(c.maxrung);
     c.maxrung = nullptr;
-    dev_free(g_packed);
-    g_packed = nullptr;
-    g_packed_bytes = 0;
+    for (void** p : {(void**)&g_packed, (void**)&g_refs, (void**)&g_sort_kv, &g_sort_tmp, (void**)&g_sym_out}) {
+        dev_free(*p);
+        *p = nullptr;
+    }
+    g_packed_bytes = g_refs_bytes = g_sort_kv_bytes = g_sort_tmp_bytes = g_sym_out_bytes = 0;
     dev_free(g_part_a);
     g_part_a = nullptr;
     g_part_a_bytes = 0;
