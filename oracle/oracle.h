@@ -93,6 +93,19 @@ void orc_ewald_kick(size_t n, const uint32_t* x, const uint32_t* y, const uint32
                     const int64_t* sinks, double gm, double soft, double alpha, int nimg, int kmax,
                     double* ax, double* ay, double* az, double* pot);
 
+/* ---- multipole mesh (particle.cpp:116-271, kick.cpp:11-29), Cartesian order P <= 8 ---- */
+int orc_mm_ncomp(int P);
+void orc_mm_index(int P, int* idx); /* [ncomp][3] exponents */
+/* D_n(rho) = d^n of the periodic Green's function (cell units), minus the
+ * nearest image 1/|rho| if near; rho anywhere, k-space summed directly */
+void orc_mm_green_point(int P, int nc, double beta, int near, const double* rho, double* D);
+/* the transformed table Dt[c][q] (Re or Im of D^_c by degree parity, / nc^3) */
+void orc_mm_green_table(int P, int nc, double beta, int near2, double* Dt);
+/* parts: 1 near-set direct pairs, 2 far field, 3 both */
+void orc_mm_forces(size_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int nc, int P, double beta,
+                   int near2, double gm, double soft, int kernel, int parts, double* fx, double* fy, double* fz,
+                   double* pot);
+
 /* ---- learned short-range correction (no ct2 counterpart) ---- */
 size_t orc_mlp_num_params(void);
 double orc_bf16_round(double v);

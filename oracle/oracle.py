@@ -78,6 +78,12 @@ def _declare(L):
     L.orc_pm_forces.argtypes = [sz, _P, _P, _P, i, d, d, i, _P, _P, _P, _P]
     L.orc_ewald_kick.argtypes = [sz, _P, _P, _P, sz, _P, d, d, d, i, i, _P, _P, _P, _P]
     L.orc_pm_near.argtypes = [sz, _P, _P, _P, sz, _P, d, d, i, d, d, _P, _P, _P, _P]
+    L.orc_mm_ncomp.argtypes = [i]
+    L.orc_mm_ncomp.restype = i
+    L.orc_mm_index.argtypes = [i, _P]
+    L.orc_mm_green_point.argtypes = [i, i, d, i, _P, _P]
+    L.orc_mm_green_table.argtypes = [i, i, d, i, _P]
+    L.orc_mm_forces.argtypes = [sz, _P, _P, _P, i, i, d, i, d, d, i, i, _P, _P, _P, _P]
     L.orc_mlp_num_params.restype = sz
     L.orc_bf16_round.argtypes = [d]
     L.orc_bf16_round.restype = d
@@ -283,6 +289,42 @@ def ewald_kick(x, y, z, gm, soft, sinks=None, alpha=2.5, nimg=2, kmax=5):
         m, sp = sinks.size, _p(sinks)
     out = [np.empty(m) for _ in range(4)]
     lib().orc_ewald_kick(x.size, _p(x), _p(y), _p(z), m, sp, gm, soft, alpha, nimg, kmax, *[_p(o) for o in out])
+    return tuple(out)
+
+
+# ---------------------------------------------------------------- multipole mesh
+MM_BETA = 0.5  # Ewald split parameter per cell of the derivative table
+MM_NEAR2 = 10  # |cell offset|^2 of the direct set (particle.cpp:136)
+
+
+def mm_index(P):
+    k = lib().orc_mm_ncomp(P)
+    out = np.empty(3 * k, np.int32)
+    lib().orc_mm_index(P, _p(out))
+    return out.reshape(k, 3)
+
+
+def mm_green_point(P, nc, rho, near=False, beta=MM_BETA):
+    k = lib().orc_mm_ncomp(P)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    D = np.empty(k)
+    lib().orc_mm_green_point(P, nc, beta, int(near), _p(rho), _p(D))
+    return D
+
+
+def mm_green_table(P, nc, near2=MM_NEAR2, beta=MM_BETA):
+    k = lib().orc_mm_ncomp(P)
+    Dt = np.empty(k * nc ** 3)
+    lib().orc_mm_green_table(P, nc, beta, near2, _p(Dt))
+    return Dt.reshape(k, nc, nc, nc)
+
+
+def mm_forces(x, y, z, nc, P, gm, soft, kernel=KERNEL_PLUMMER, parts=3, near2=MM_NEAR2, beta=MM_BETA):
+    """FP64 multipole-mesh force and potential (box units): (fx, fy, fz, pot)."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    out = [np.empty(x.size) for _ in range(4)]
+    lib().orc_mm_forces(x.size, _p(x), _p(y), _p(z), nc, P, beta, near2, gm, soft, kernel, parts,
+                        *[_p(o) for o in out])
     return tuple(out)
 
 
