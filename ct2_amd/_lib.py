@@ -31,6 +31,7 @@ PATH_DIRECT = 1
 PATH_PM = 2
 PATH_PM_NEAR = 3
 PATH_PM_MLP = 4
+PATH_MM = 5
 
 
 class Ct2Error(RuntimeError):
@@ -84,6 +85,10 @@ _SIGS = {
     "ct2_particles_device_ptrs": ([C.POINTER(ParticlePtrs)], C.c_int),
     "ct2_kick": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_kick_noalloc": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
+    "ct2_mm_init": ([C.c_int, C.c_int], C.c_int),
+    "ct2_mm_forces": ([], C.c_int),
+    "ct2_kick_mm": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
+    "ct2_mm_table_download": ([_P], C.c_int),
     "ct2_forces_download": ([_P] * 4, C.c_int),
     "ct2_kick_sample_fp64": ([_P, C.c_size_t, C.c_int, _P, _P, _P, _P], C.c_int),
     "ct2_drift": ([C.c_float, C.c_float], C.c_int),

@@ -22,6 +22,7 @@ KICK_STORE = L.KICK_STORE
 KICK_ORDERED = L.KICK_ORDERED
 KICK_SYMMETRIC = L.KICK_SYMMETRIC
 PATH_DIRECT, PATH_PM, PATH_PM_NEAR, PATH_PM_MLP = L.PATH_DIRECT, L.PATH_PM, L.PATH_PM_NEAR, L.PATH_PM_MLP
+PATH_MM = L.PATH_MM
 
 
 def _p(a: np.ndarray | None):
@@ -239,6 +240,49 @@ def kick_pm(params: KickParams | None = None, flags: int = 0) -> int:
     mr = C.c_int(0)
     L.check(L.load().ct2_kick_pm(C.byref(params), flags, C.byref(mr)))
     return mr.value
+
+
+# ---------------------------------------------------------------- multipole mesh
+_mm = {"nc": 0, "order": 0}
+
+
+def mm_ncomp(order: int) -> int:
+    return (order + 1) * (order + 2) * (order + 3) // 6
+
+
+def mm_cells_for(n: int, per_cell: int = 64) -> int:
+    """The power-of-two cell grid with about `per_cell` particles per cell (>= 8)."""
+    nc = 8
+    while (2 * nc) ** 3 * per_cell <= n and nc < 256:
+        nc *= 2
+    return nc
+
+
+def mm_init(nc: int, order: int) -> None:
+    """ct2's multipole-mesh far field on nc^3 cells with Cartesian order `order`."""
+    L.check(L.load().ct2_mm_init(nc, order))
+    _mm.update(nc=nc, order=order)
+
+
+def mm_forces():
+    """Multipole mesh + near-set direct pairs for every particle: (fx, fy, fz, pot)."""
+    L.check(L.load().ct2_mm_forces())
+    return forces_download()
+
+
+def kick_mm(params: KickParams | None = None, flags: int = 0) -> int:
+    if params is None:
+        params = kick_params()
+    mr = C.c_int(0)
+    L.check(L.load().ct2_kick_mm(C.byref(params), flags, C.byref(mr)))
+    return mr.value
+
+
+def mm_table_download() -> np.ndarray:
+    nc, k = _mm["nc"], mm_ncomp(_mm["order"])
+    t = np.empty(k * nc ** 3)
+    L.check(L.load().ct2_mm_table_download(_p(t)))
+    return t.reshape(k, nc, nc, nc)
 
 
 def pm_density_download() -> np.ndarray:

@@ -130,6 +130,7 @@ int ct2_finalize(void) {
     release_fft_ct2();
     release_rungs();
     release_grid();
+    release_mm();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device = -1;

@@ -96,6 +96,7 @@ void release_near();
 void release_fft_ct2();
 void release_rungs();
 void release_grid();
+void release_mm();
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(long v) {

@@ -1,0 +1,729 @@
+// mm.hip -- ct2's multipole-mesh force (particle.cpp:116-271, kick.cpp:11-29,
+// 105-125) on gfx950.
+//
+// ct2's kick() sums, for the particles of each cell, direct pairs with the
+// cells of its neighbour list and a local expansion of everything else about
+// the cell centre; the expansions are a convolution over the cell grid done
+// with one FFT per multipole component.  The reference's P2M, MG2L and L2P
+// come from sfmm (spherical harmonics, PORDER = 8), which is not in the tree;
+// this module uses Cartesian Taylor expansions of total order P <= 8.
+//
+//   cells      nc^3, nc a power of two in [8, 256]; lengths below in cells;
+//   P2M        M_l(c) = sum_{j in c} (-b_j)^l / l!   (b = x - centre, unit mass);
+//   table      D_n(rho) = d^n G_far(rho) at the cell offsets rho, G_far the
+//              periodic Green's function minus the nearest image 1/|rho| on the
+//              near set |rho|^2 <= 10 (particle.cpp:136), transformed once:
+//              Dt[n][q] = D^_n(q) / nc^3 (real for even |n|, i x real for odd);
+//   MG2L       L^_k(q) = sum_{|k|+|l|<=P} M^_l(q) D^_{k+l}(q), per wave vector;
+//   L2P        pot = sum_k a^k/k! L_k,  F_q = sum_k a^{k-e_q}/(k-e_q)! L_k;
+//   near       softened direct pairs with the particles of the near set's cells.
+//
+// Box units: force GM (near + nc^2 far), potential GM (near + nc far).
+//
+// The table: G = G_short + G_long (Ewald split beta = 0.5 per cell).
+// G_short = sum_m erfc(beta |rho + nc m|)/|rho + nc m| is expanded on the host in
+// FP64: for a radial f(s), s = |rho + h|^2, the Taylor coefficients in h are
+// sum_j f^(j)(s0)/j! [h^n] (2 rho.h + h.h)^j with f^(j) = (-1/2)^j B_j and
+// B_0 = erfc(beta r)/r, B_j = ((2j-1) B_{j-1} + (2 beta^2)^j e^{-beta^2 r^2}/(beta sqrt pi))/r^2
+// (Smith 1998); a near offset's own image is -erf(beta r)/r, a power series in s.
+// Its samples are transformed with the FP64 3-D FFT.  G_long's sampled
+// transform is analytic: sum over aliases kappa = q/nc + m of
+// (2 pi i kappa)^n e^{-pi^2 kappa^2/beta^2} / (pi kappa^2), and -pi/beta^2 at 0.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "epilogue.h"
+#include "internal.h"
+
+namespace ct2 {
+namespace {
+
+constexpr int MM_PMAX = 8;
+constexpr int MM_KMAX = 165;       // components at P = 8
+constexpr int MM_NEAR2 = 10;       // |cell offset|^2 of the direct set (particle.cpp:136)
+constexpr double MM_BETA = 0.5;    // Ewald split per cell
+constexpr int MM_QB = 16;          // wave vectors per MG2L workgroup
+constexpr int MM_SLOTS = 16;       // MG2L work lists per wave vector (256 threads)
+constexpr int MM_SCHED_MAX = 320;  // entries per list
+constexpr int MM_NEAR_MAX = 160;   // offsets of the near set (147 at |d|^2 <= 10)
+constexpr int MM_WAVES = 4;        // waves per near-field workgroup
+
+// MG2L list entry: k | l << 8 | n << 16 | odd(n) << 24 | pad << 25 | flush << 26
+constexpr uint32_t E_ODD = 1u << 24, E_PAD = 1u << 25, E_FLUSH = 1u << 26;
+
+__constant__ int c_e[MM_KMAX][3];         // exponents of component c
+__constant__ float c_ifact[MM_KMAX];      // 1 / (a! b! c!)
+__constant__ uint32_t c_sched[MM_SLOTS][MM_SCHED_MAX];
+__constant__ int c_near[MM_NEAR_MAX][3];  // near-set cell offsets
+
+struct MMIndex {
+    int P = 0, K = 0;
+    int e[MM_KMAX][3];
+    int id[MM_PMAX + 1][MM_PMAX + 1][MM_PMAX + 1];
+    double fact[MM_KMAX];
+    std::vector<int> pa, pb, pc;  // products within degree P: (i, j) -> i + j
+    void init(int p) {
+        P = p;
+        K = (P + 1) * (P + 2) * (P + 3) / 6;
+        std::memset(id, -1, sizeof id);
+        int c = 0;
+        for (int d = 0; d <= P; ++d)
+            for (int a = d; a >= 0; --a)
+                for (int b = d - a; b >= 0; --b) {
+                    e[c][0] = a;
+                    e[c][1] = b;
+                    e[c][2] = d - a - b;
+                    id[a][b][d - a - b] = c;
+                    fact[c] = std::tgamma(a + 1.0) * std::tgamma(b + 1.0) * std::tgamma(d - a - b + 1.0);
+                    ++c;
+                }
+        pa.clear();
+        pb.clear();
+        pc.clear();
+        for (int i = 0; i < K; ++i)
+            for (int j = 0; j < K; ++j) {
+                const int a = e[i][0] + e[j][0], b = e[i][1] + e[j][1], d = e[i][2] + e[j][2];
+                if (a + b + d > P) continue;
+                pa.push_back(i);
+                pb.push_back(j);
+                pc.push_back(id[a][b][d]);
+            }
+    }
+    int deg(int c) const { return e[c][0] + e[c][1] + e[c][2]; }
+};
+
+struct MMState {
+    int nc = 0, P = 0, K = 0, nz = 0;  // nz = complex grids (K/2 rounded up)
+    int slen = 0, nnear = 0;
+    double* Dt = nullptr;  // [K][nc^3]
+    size_t Dt_bytes = 0;
+    float2* Zm = nullptr;  // multipole grids [nz][nc^3], then their transforms
+    float2* Zl = nullptr;  // local-expansion grids
+    size_t Z_bytes = 0;
+    uint32_t* keys = nullptr;  // sort keys / values in and out (4 n)
+    size_t keys_bytes = 0;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    uint4* Ps = nullptr;  // sorted (x, y, z, index)
+    size_t Ps_bytes = 0;
+    int* start = nullptr;  // cell starts (nc^3 + 1)
+    size_t start_bytes = 0;
+    int* items = nullptr;  // near-field work items (cell, first, count) and their count
+    size_t items_bytes = 0;
+    int* nitems = nullptr;
+    int* cnt = nullptr;  // items per cell, then their exclusive sum
+    size_t cnt_bytes = 0;
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    double2* tabz = nullptr;  // one FP64 complex grid while the table is built
+    size_t tabz_bytes = 0;
+};
+MMState g_mm;
+
+// ------------------------------------------------------------ the table (host, FP64)
+
+// s-derivatives of erfc(beta sqrt s)/sqrt s at s = r^2
+void erfc_sderiv(int P, double r, double* fs) {
+    const double b = MM_BETA, r2 = r * r, ex = std::exp(-b * b * r2) / (b * std::sqrt(M_PI));
+    double B = std::erfc(b * r) / r, tb = 1.0, sgn = 1.0;
+    fs[0] = B;
+    for (int j = 1; j <= P; ++j) {
+        tb *= 2.0 * b * b;
+        B = ((2 * j - 1) * B + tb * ex) / r2;
+        sgn *= -0.5;
+        fs[j] = sgn * B;
+    }
+}
+
+// s-derivatives of -erf(beta sqrt s)/sqrt s = -(2 beta/sqrt pi) sum_i (-beta^2 s)^i/(i! (2i+1))
+void erf_sderiv(int P, double s, double* fs) {
+    const double b2 = MM_BETA * MM_BETA, c0 = -2.0 * MM_BETA / std::sqrt(M_PI);
+    for (int j = 0; j <= P; ++j) {
+        double sum = 0.0, term = std::pow(-b2, j);
+        for (int i = j; i < j + 400; ++i) {
+            const double t = term / (2 * i + 1);
+            sum += t;
+            if (std::fabs(t) < 1e-19 * std::fabs(sum) && i > j + 4) break;
+            term *= -b2 * s / (double)(i - j + 1);
+        }
+        fs[j] = c0 * sum;
+    }
+}
+
+// T += Taylor coefficients (d^n f / n!) at rho of f(|rho + h|^2), fs = f^(j)(|rho|^2)
+void radial_taylor_add(const MMIndex& I, const double* rho, const double* fs, double* T) {
+    double delta[MM_KMAX] = {}, pw[MM_KMAX] = {}, nx[MM_KMAX];
+    for (int q = 0; q < 3; ++q) {
+        int a[3] = {0, 0, 0};
+        a[q] = 1;
+        delta[I.id[a[0]][a[1]][a[2]]] = 2.0 * rho[q];
+        a[q] = 2;
+        if (I.P >= 2) delta[I.id[a[0]][a[1]][a[2]]] = 1.0;
+    }
+    pw[0] = 1.0;
+    T[0] += fs[0];
+    double jf = 1.0;
+    const size_t np = I.pa.size();
+    for (int j = 1; j <= I.P; ++j) {
+        std::fill(nx, nx + I.K, 0.0);
+        for (size_t p = 0; p < np; ++p) nx[I.pc[p]] += pw[I.pa[p]] * delta[I.pb[p]];
+        std::copy(nx, nx + I.K, pw);
+        jf *= j;
+        const double c = fs[j] / jf;
+        for (int k = 0; k < I.K; ++k) T[k] += c * pw[k];
+    }
+}
+
+int wrap_offset(int i, int nc) { return i < nc / 2 ? i : i - nc; }
+
+// D_n of G_short (less the near set's nearest image) at every cell, [K][nc^3];
+// cells beyond the cutoff stay zero.  Split over host threads.
+void short_samples(const MMIndex& I, int nc, std::vector<double>& S) {
+    const size_t n3 = (size_t)nc * nc * nc;
+    S.assign(n3 * I.K, 0.0);
+    const double rc = 8.0 / MM_BETA;  // erfc(beta r)/r and its derivatives < 1e-16 of their size beyond
+    const int mr = (int)std::ceil(rc / nc) + 1;
+    auto work = [&](size_t c0, size_t c1) {
+        double fs[MM_PMAX + 1], T[MM_KMAX];
+        for (size_t cell = c0; cell < c1; ++cell) {
+            const int o[3] = {wrap_offset((int)(cell / ((size_t)nc * nc)), nc), wrap_offset((int)(cell / nc % nc), nc),
+                              wrap_offset((int)(cell % nc), nc)};
+            const bool near = o[0] * o[0] + o[1] * o[1] + o[2] * o[2] <= MM_NEAR2;
+            bool any = false;
+            std::fill(T, T + I.K, 0.0);
+            for (int m0 = -mr; m0 <= mr; ++m0)
+                for (int m1 = -mr; m1 <= mr; ++m1)
+                    for (int m2 = -mr; m2 <= mr; ++m2) {
+                        const double r[3] = {(double)o[0] + nc * m0, (double)o[1] + nc * m1, (double)o[2] + nc * m2};
+                        const double s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                        if (near && !m0 && !m1 && !m2) {
+                            erf_sderiv(I.P, s, fs);
+                        } else {
+                            if (s >= rc * rc) continue;
+                            erfc_sderiv(I.P, std::sqrt(s), fs);
+                        }
+                        radial_taylor_add(I, r, fs, T);
+                        any = true;
+                    }
+            if (any)
+                for (int k = 0; k < I.K; ++k) S[(size_t)k * n3 + cell] = T[k] * I.fact[k];
+        }
+    };
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) th.emplace_back(work, n3 * t / nt, n3 * (t + 1) / nt);
+    for (auto& t : th) t.join();
+}
+
+// Dt[c][q] = (parity part of the transformed short samples + the analytic
+// long-range transform) / nc^3 for one component c of exponents (a, b, d).
+__global__ __launch_bounds__(256) void mm_table_finish(const double2* __restrict__ Z, int nc, int a, int b, int d,
+                                                       double* __restrict__ Dt) {
+    const size_t n3 = (size_t)nc * nc * nc;
+    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n3) return;
+    const int deg = a + b + d;
+    const int q0 = (int)(q / ((size_t)nc * nc)), q1 = (int)(q / nc % nc), q2 = (int)(q % nc);
+    const double k0 = (double)(q0 < nc / 2 ? q0 : q0 - nc) / nc, k1 = (double)(q1 < nc / 2 ? q1 : q1 - nc) / nc,
+                 k2 = (double)(q2 < nc / 2 ? q2 : q2 - nc) / nc;
+    const double b2 = MM_BETA * MM_BETA, tp = 2.0 * M_PI;
+    double lr = 0.0;
+    for (int m0 = -1; m0 <= 1; ++m0)
+        for (int m1 = -1; m1 <= 1; ++m1)
+            for (int m2 = -1; m2 <= 1; ++m2) {
+                const double x = k0 + m0, y = k1 + m1, z = k2 + m2, kk = x * x + y * y + z * z;
+                if (kk == 0.0) continue;
+                double mono = exp(-M_PI * M_PI * kk / b2) / (M_PI * kk);
+                for (int i = 0; i < a; ++i) mono *= tp * x;
+                for (int i = 0; i < b; ++i) mono *= tp * y;
+                for (int i = 0; i < d; ++i) mono *= tp * z;
+                lr += mono;
+            }
+    const double s = ((deg >> 1) & 1) ? -1.0 : 1.0;  // i^deg = s or i s
+    double v = (deg & 1) ? Z[q].y + s * lr : Z[q].x + s * lr;
+    if (deg == 0 && q == 0) v -= M_PI / b2;
+    Dt[q] = v / (double)n3;
+}
+
+// ------------------------------------------------------------ binning
+
+__global__ __launch_bounds__(256) void mm_keys(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                               const uint32_t* __restrict__ Z, int n, int lg, uint32_t* __restrict__ key,
+                                               uint32_t* __restrict__ idx) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int sh = 32 - lg;
+    key[i] = (((X[i] >> sh) << lg | (Y[i] >> sh)) << lg) | (Z[i] >> sh);
+    idx[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void mm_gather(const uint32_t* __restrict__ X, const uint32_t* __restrict__ Y,
+                                                 const uint32_t* __restrict__ Z, const uint32_t* __restrict__ idx, int n,
+                                                 uint4* __restrict__ P) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = idx[s];
+    P[s] = make_uint4(X[i], Y[i], Z[i], i);
+}
+
+// start[c] = first sorted slot of cell c (lower bound of key c), start[n3] = n
+__global__ __launch_bounds__(256) void mm_bounds(const uint32_t* __restrict__ key, int n, int n3,
+                                                 int* __restrict__ start) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c > n3) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (key[mid] < (uint32_t)c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    start[c] = lo;
+}
+
+__global__ __launch_bounds__(256) void mm_item_counts(const int* __restrict__ start, int n3, int* __restrict__ cnt) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c < n3) cnt[c] = (start[c + 1] - start[c] + 63) >> 6;
+}
+
+// items[3 i + {0,1,2}] = (cell, first sorted slot, sinks <= 64); cnt holds the exclusive sum
+__global__ __launch_bounds__(256) void mm_items(const int* __restrict__ start, const int* __restrict__ first, int n3,
+                                                int* __restrict__ items, int* __restrict__ nitems) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n3) return;
+    const int s0 = start[c], s1 = start[c + 1];
+    int it = first[c];
+    for (int s = s0; s < s1; s += 64, ++it) {
+        items[3 * it] = c;
+        items[3 * it + 1] = s;
+        items[3 * it + 2] = min(64, s1 - s);
+    }
+    if (c == n3 - 1) *nitems = it;
+}
+
+// ------------------------------------------------------------ P2M
+
+// One workgroup per cell: powers of -b of up to 256 particles in LDS, one
+// thread per component; FP64 sums.  Component c goes to grid c/2, real part
+// for even c, imaginary for odd (an odd K leaves the last imaginary part 0).
+__global__ __launch_bounds__(256) void mm_p2m(const uint4* __restrict__ P, const int* __restrict__ start, int lg, int Pord,
+                                              int K, size_t n3, float2* __restrict__ Zm) {
+    __shared__ float pw[256][3][MM_PMAX + 1];
+    const int cell = blockIdx.x, t = threadIdx.x;
+    const int s0 = start[cell], s1 = start[cell + 1];
+    const int sh = 32 - lg;
+    const uint32_t mask = (1u << sh) - 1u, half = 1u << (sh - 1);
+    const float inv = ldexpf(1.0f, -sh);
+    double acc = 0.0;
+    const int ea = t < K ? c_e[t][0] : 0, eb = t < K ? c_e[t][1] : 0, ec = t < K ? c_e[t][2] : 0;
+    for (int base = s0; base < s1; base += 256) {
+        __syncthreads();
+        if (base + t < s1) {
+            const uint4 p = P[base + t];
+            const uint32_t v[3] = {p.x, p.y, p.z};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float mb = -(float)(int32_t)((v[q] & mask) - half) * inv;  // -b in cells
+                float w = 1.0f;
+                pw[t][q][0] = 1.0f;
+                for (int e = 1; e <= Pord; ++e) {
+                    w *= mb;
+                    pw[t][q][e] = w;
+                }
+            }
+        }
+        __syncthreads();
+        const int cnt = min(256, s1 - base);
+        if (t < K)
+            for (int j = 0; j < cnt; ++j) acc += (double)(pw[j][0][ea] * pw[j][1][eb] * pw[j][2][ec]);
+    }
+    if (t < K) {
+        float* z = (float*)(Zm + (size_t)(t >> 1) * n3 + cell);
+        z[t & 1] = (float)(acc * (double)c_ifact[t]);
+    } else if (t == K && (K & 1)) {
+        ((float*)(Zm + (size_t)(t >> 1) * n3 + cell))[1] = 0.0f;
+    }
+}
+
+// ------------------------------------------------------------ MG2L
+
+// MM_QB wave vectors per workgroup: the transformed multipoles unpacked from
+// the paired grids (Z(q) and Z(-q)) and the table rows in LDS; thread (slot,
+// qq) walks list `slot` of (k, l, k+l) products, all lists of equal length,
+// every k in one list, and writes L^_k at its last product; the local
+// expansions leave paired the same way for the inverse transforms.
+__global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, const double* __restrict__ Dt, int nc,
+                                               int K, int nz, int slen, float2* __restrict__ Zl) {
+    __shared__ float2 Mh[MM_KMAX][MM_QB];
+    __shared__ double Dq[MM_KMAX][MM_QB];
+    __shared__ float2 Lh[MM_KMAX][MM_QB];
+    const size_t n3 = (size_t)nc * nc * nc;
+    const size_t q0 = (size_t)blockIdx.x * MM_QB;
+    const int t = threadIdx.x;
+    const int m = nc - 1;
+    for (int e = t; e < nz * MM_QB; e += 256) {
+        const int j = e / MM_QB, qq = e % MM_QB;
+        const size_t q = q0 + qq;
+        const int i0 = (int)(q / ((size_t)nc * nc)), i1 = (int)(q / nc % nc), i2 = (int)(q % nc);
+        const size_t qn = ((size_t)((-i0) & m) * nc + (size_t)((-i1) & m)) * nc + (size_t)((-i2) & m);
+        const float2 a = Zm[(size_t)j * n3 + q], b = Zm[(size_t)j * n3 + qn];
+        Mh[2 * j][qq] = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));          // (Z(q) + conj Z(-q)) / 2
+        if (2 * j + 1 < K) Mh[2 * j + 1][qq] = make_float2(0.5f * (a.y + b.y), 0.5f * (b.x - a.x));  // / 2i
+    }
+    for (int e = t; e < K * MM_QB; e += 256) {
+        const int c = e / MM_QB, qq = e % MM_QB;
+        Dq[c][qq] = Dt[(size_t)c * n3 + q0 + qq];
+    }
+    __syncthreads();
+    const int qq = t & (MM_QB - 1), slot = t / MM_QB;
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i < slen; ++i) {
+        const uint32_t w = c_sched[slot][i];
+        if (w & E_PAD) continue;
+        const float2 mv = Mh[(w >> 8) & 255][qq];
+        const double d = Dq[(w >> 16) & 255][qq];
+        if (w & E_ODD) {  // M (i d)
+            ar -= (double)mv.y * d;
+            ai += (double)mv.x * d;
+        } else {
+            ar += (double)mv.x * d;
+            ai += (double)mv.y * d;
+        }
+        if (w & E_FLUSH) {
+            Lh[w & 255][qq] = make_float2((float)ar, (float)ai);
+            ar = ai = 0.0;
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < nz * MM_QB; e += 256) {
+        const int j = e / MM_QB, qq2 = e % MM_QB;
+        const float2 a = Lh[2 * j][qq2];
+        const float2 b = 2 * j + 1 < K ? Lh[2 * j + 1][qq2] : make_float2(0.0f, 0.0f);
+        Zl[(size_t)j * n3 + q0 + qq2] = make_float2(a.x - b.y, a.y + b.x);  // L_a + i L_b
+    }
+}
+
+// ------------------------------------------------------------ near pairs + L2P
+
+struct NearK {
+    float eps2u, h2u, hinvu, hinv3u;  // as the direct kick, in 2^-32 units
+};
+
+__device__ __forceinline__ void two_sum_f(float& s, float& c, float v) {
+#pragma clang fp contract(off)
+    const float t = s + v;
+    const float bp = t - s;
+    c += (s - (t - bp)) + (v - bp);
+    s = t;
+}
+
+// One wave per work item (up to 64 sinks of one cell), items taken in turn
+// by the waves of a persistent grid.  Sources of each near-set cell stream
+// through the wave's LDS slot as FP32 offsets from the sink cell's centre
+// (the cell offsets are at most 3, so every such difference is the minimum
+// image); per cell their sum is folded in with TwoSum.  Then the local
+// expansion of the cell at the sink's offset.
+template <int KERNEL>
+__global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __restrict__ P, const int* __restrict__ start,
+                                                             const int* __restrict__ items, const int* __restrict__ nitems,
+                                                             const float2* __restrict__ Zl, int lg, int Pord, int K,
+                                                             int nnear, NearK kc, double fnear, double pnear,
+                                                             double ffar, double pfar, float* __restrict__ fx,
+                                                             float* __restrict__ fy, float* __restrict__ fz,
+                                                             float* __restrict__ pot) {
+    __shared__ float4 S[MM_WAVES][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nc = 1 << lg, sh = 32 - lg, m = nc - 1;
+    const size_t n3 = (size_t)nc * nc * nc;
+    const int total = *nitems;
+    for (int item = blockIdx.x * MM_WAVES + w; item < total; item += gridDim.x * MM_WAVES) {
+        const int cell = items[3 * item], first = items[3 * item + 1], cnt = items[3 * item + 2];
+        const int cx = cell >> (2 * lg), cy = (cell >> lg) & m, cz = cell & m;
+        const uint32_t half = 1u << (sh - 1);
+        const uint32_t rx = ((uint32_t)cx << sh) + half, ry = ((uint32_t)cy << sh) + half, rz = ((uint32_t)cz << sh) + half;
+        const bool live = lane < cnt;
+        const uint4 me = P[first + (live ? lane : 0)];
+        const float ux = (float)(int32_t)(me.x - rx), uy = (float)(int32_t)(me.y - ry), uz = (float)(int32_t)(me.z - rz);
+        float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f, cxs = 0.f, cys = 0.f, czs = 0.f, cps = 0.f;
+        for (int o = 0; o < nnear; ++o) {
+            const int sc = ((((cx + c_near[o][0]) & m) << lg | ((cy + c_near[o][1]) & m)) << lg) | ((cz + c_near[o][2]) & m);
+            const int j0 = start[sc], j1 = start[sc + 1];
+            float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
+            for (int jb = j0; jb < j1; jb += 64) {
+                const int nj = min(64, j1 - jb);
+                if (lane < nj) {
+                    const uint4 s = P[jb + lane];
+                    S[w][lane] = make_float4((float)(int32_t)(s.x - rx), (float)(int32_t)(s.y - ry),
+                                             (float)(int32_t)(s.z - rz), 0.0f);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int j = 0; j < nj; ++j) {
+                    const float4 s = S[w][j];
+                    const float dx = s.x - ux, dy = s.y - uy, dz = s.z - uz;
+                    float k, p;
+                    if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, kc.eps2u)));
+                        const float wv = __builtin_amdgcn_rsqf(r2);
+                        k = (wv * wv) * wv;
+                        p = wv;
+                    } else {
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        const bool in = r2 < kc.h2u;
+                        const float wv = __builtin_amdgcn_rsqf(in ? kc.h2u : r2);
+                        k = in ? kc.hinv3u : (wv * wv) * wv;
+                        p = in ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : wv;
+                    }
+                    tx = fmaf(dx, k, tx);
+                    ty = fmaf(dy, k, ty);
+                    tz = fmaf(dz, k, tz);
+                    tp += p;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            two_sum_f(sx, cxs, tx);
+            two_sum_f(sy, cys, ty);
+            two_sum_f(sz, czs, tz);
+            two_sum_f(sp, cps, tp);
+        }
+        // the local expansion at a = offset in cells
+        const float inv = ldexpf(1.0f, -sh);
+        const float ax = ux * inv, ay = uy * inv, az = uz * inv;
+        float pa[MM_PMAX + 1], pb[MM_PMAX + 1], pc[MM_PMAX + 1];
+        pa[0] = pb[0] = pc[0] = 1.0f;
+        for (int e = 1; e <= Pord; ++e) {
+            pa[e] = pa[e - 1] * ax;
+            pb[e] = pb[e - 1] * ay;
+            pc[e] = pc[e - 1] * az;
+        }
+        float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+        for (int c = 0; c < K; ++c) {
+            const float2 z = Zl[(size_t)(c >> 1) * n3 + cell];
+            const float L = ((c & 1) ? z.y : z.x) * c_ifact[c];
+            const int a = c_e[c][0], b = c_e[c][1], d = c_e[c][2];
+            phi = fmaf(L, pa[a] * pb[b] * pc[d], phi);
+            if (a) gx = fmaf(L * (float)a, pa[a - 1] * pb[b] * pc[d], gx);
+            if (b) gy = fmaf(L * (float)b, pa[a] * pb[b - 1] * pc[d], gy);
+            if (d) gz = fmaf(L * (float)d, pa[a] * pb[b] * pc[d - 1], gz);
+        }
+        if (live) {
+            const uint32_t i = me.w;
+            fx[i] = (float)(((double)sx + (double)cxs) * fnear + (double)gx * ffar);
+            fy[i] = (float)(((double)sy + (double)cys) * fnear + (double)gy * ffar);
+            fz[i] = (float)(((double)sz + (double)czs) * fnear + (double)gz * ffar);
+            pot[i] = (float)(((double)sp + (double)cps) * pnear + (double)phi * pfar);
+        }
+    }
+}
+
+int mm_require() {
+    CT2_TRY(require_particles());
+    if (!g_mm.nc) return set_error(CT2_ERR_STATE, "ct2_mm_init has not been called");
+    return CT2_OK;
+}
+
+int mm_forces() {
+    Context& c = ctx();
+    MMState& s = g_mm;
+    const int n = (int)c.n, nc = s.nc, lg = ilog2(nc);
+    const int n3 = nc * nc * nc;
+    hipStream_t st = c.stream;
+    CT2_TRY(ensure_scratch((void**)&s.keys, &s.keys_bytes, sizeof(uint32_t) * 4 * (size_t)n, "multipole-mesh keys"));
+    CT2_TRY(ensure_scratch((void**)&s.Ps, &s.Ps_bytes, sizeof(uint4) * (size_t)n, "multipole-mesh particles"));
+    CT2_TRY(ensure_scratch((void**)&s.start, &s.start_bytes, sizeof(int) * ((size_t)n3 + 1), "cell starts"));
+    CT2_TRY(ensure_scratch((void**)&s.cnt, &s.cnt_bytes, sizeof(int) * 2 * (size_t)n3, "item counts"));
+    const size_t max_items = (size_t)n / 64 + (size_t)n3 + 1;
+    CT2_TRY(ensure_scratch((void**)&s.items, &s.items_bytes, sizeof(int) * (3 * max_items + 1), "work items"));
+    s.nitems = s.items + 3 * max_items;
+    uint32_t *kin = s.keys, *kout = s.keys + n, *vin = s.keys + 2L * n, *vout = s.keys + 3L * n;
+    const unsigned pb = (unsigned)((n + 255) / 256);
+    mm_keys<<<pb, 256, 0, st>>>(c.x, c.y, c.z, n, lg, kin, vin);
+    CT2_LAUNCH_CHECK();
+    size_t tmp = 0;
+    CT2_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, n, 0, 3 * lg, st));
+    CT2_TRY(ensure_scratch(&s.sort_tmp, &s.sort_tmp_bytes, std::max<size_t>(tmp, 256), "multipole-mesh sort"));
+    CT2_HIP(hipcub::DeviceRadixSort::SortPairs(s.sort_tmp, tmp, kin, kout, vin, vout, n, 0, 3 * lg, st));
+    mm_gather<<<pb, 256, 0, st>>>(c.x, c.y, c.z, vout, n, s.Ps);
+    CT2_LAUNCH_CHECK();
+    mm_bounds<<<(unsigned)((n3 + 256) / 256), 256, 0, st>>>(kout, n, n3, s.start);
+    CT2_LAUNCH_CHECK();
+    mm_item_counts<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(s.start, n3, s.cnt);
+    CT2_LAUNCH_CHECK();
+    int* first = s.cnt + n3;
+    tmp = 0;
+    CT2_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, s.cnt, first, n3, st));
+    CT2_TRY(ensure_scratch(&s.scan_tmp, &s.scan_tmp_bytes, std::max<size_t>(tmp, 256), "item scan"));
+    CT2_HIP(hipcub::DeviceScan::ExclusiveSum(s.scan_tmp, tmp, s.cnt, first, n3, st));
+    mm_items<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(s.start, first, n3, s.items, s.nitems);
+    CT2_LAUNCH_CHECK();
+    // far field
+    mm_p2m<<<(unsigned)n3, 256, 0, st>>>(s.Ps, s.start, lg, s.P, s.K, (size_t)n3, s.Zm);
+    CT2_LAUNCH_CHECK();
+    for (int j = 0; j < s.nz; ++j) CT2_TRY(fft3d(s.Zm + (size_t)j * n3, nc, 1, false));
+    mm_mg2l<<<(unsigned)(n3 / MM_QB), 256, 0, st>>>(s.Zm, s.Dt, nc, s.K, s.nz, s.slen, s.Zl);
+    CT2_LAUNCH_CHECK();
+    for (int j = 0; j < s.nz; ++j) CT2_TRY(fft3d(s.Zl + (size_t)j * n3, nc, -1, false));
+    // near pairs + L2P
+    const double two32 = 4294967296.0, h = c.opts.hsoft, hu = h * two32, gm = c.opts.GM;
+    NearK kc;
+    kc.eps2u = (float)(hu * hu);
+    kc.h2u = kc.eps2u;
+    kc.hinvu = (float)(1.0 / hu);
+    kc.hinv3u = (float)(1.0 / (hu * hu * hu));
+    const double fnear = gm * two32 * two32, pnear = gm * two32;
+    const double ffar = gm * (double)nc * nc, pfar = gm * (double)nc;
+    const unsigned grid = 2048;
+    if (c.opts.kernel == CT2_KERNEL_PLUMMER)
+        mm_near_l2p<CT2_KERNEL_PLUMMER><<<grid, 64 * MM_WAVES, 0, st>>>(s.Ps, s.start, s.items, s.nitems, s.Zl, lg, s.P,
+                                                                       s.K, s.nnear, kc, fnear, pnear, ffar, pfar, c.fx,
+                                                                       c.fy, c.fz, c.pot);
+    else
+        mm_near_l2p<CT2_KERNEL_CT2><<<grid, 64 * MM_WAVES, 0, st>>>(s.Ps, s.start, s.items, s.nitems, s.Zl, lg, s.P, s.K,
+                                                                   s.nnear, kc, fnear, pnear, ffar, pfar, c.fx, c.fy,
+                                                                   c.fz, c.pot);
+    CT2_LAUNCH_CHECK();
+    c.forces_valid = true;
+    return CT2_OK;
+}
+
+}  // namespace
+
+void release_mm() {
+    MMState& s = g_mm;
+    for (void* p : {(void*)s.Dt, (void*)s.Zm, (void*)s.Zl, (void*)s.keys, s.sort_tmp, (void*)s.Ps, (void*)s.start,
+                    (void*)s.items, (void*)s.cnt, s.scan_tmp, (void*)s.tabz})
+        dev_free(p);
+    s = MMState();
+}
+
+}  // namespace ct2
+
+using namespace ct2;
+
+extern "C" int ct2_mm_init(int nc, int order) {
+    CT2_TRY(require_init());
+    if (!is_pow2(nc) || nc < 8 || nc > 256) return set_error(CT2_ERR_INVALID, "cell grid %d: a power of two in [8, 256]", nc);
+    if (order < 1 || order > MM_PMAX) return set_error(CT2_ERR_INVALID, "expansion order %d outside [1, %d]", order, MM_PMAX);
+    MMIndex I;
+    I.init(order);
+    const size_t n3 = (size_t)nc * nc * nc;
+    const size_t dt_bytes = sizeof(double) * n3 * I.K;
+    if (dt_bytes > ((size_t)8 << 30))
+        return set_error(CT2_ERR_INVALID, "the derivative table of order %d on %d^3 cells (%zu bytes) exceeds 8 GiB", order,
+                         nc, dt_bytes);
+    Context& c = ctx();
+    MMState& s = g_mm;
+    hipStream_t st = c.stream;
+    s.nc = 0;  // unusable until the table is complete
+    // the MG2L lists: products grouped by k, the k spread over MM_SLOTS lists
+    // by descending length (each to the shortest list), padded to equal length
+    std::vector<std::vector<uint32_t>> byk(I.K);
+    for (size_t p = 0; p < I.pa.size(); ++p) {
+        const int k = I.pa[p], l = I.pb[p], kl = I.pc[p];
+        byk[k].push_back((uint32_t)k | (uint32_t)l << 8 | (uint32_t)kl << 16 | ((I.deg(kl) & 1) ? E_ODD : 0u));
+    }
+    std::vector<int> korder(I.K);
+    for (int k = 0; k < I.K; ++k) korder[k] = k;
+    std::stable_sort(korder.begin(), korder.end(), [&](int a, int b) { return byk[a].size() > byk[b].size(); });
+    std::vector<std::vector<uint32_t>> lists(MM_SLOTS);
+    for (int k : korder) {
+        auto it = std::min_element(lists.begin(), lists.end(),
+                                   [](const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) { return a.size() < b.size(); });
+        byk[k].back() |= E_FLUSH;
+        it->insert(it->end(), byk[k].begin(), byk[k].end());
+    }
+    size_t slen = 0;
+    for (auto& l : lists) slen = std::max(slen, l.size());
+    if (slen > (size_t)MM_SCHED_MAX) return set_error(CT2_ERR_INVALID, "MG2L list of %zu entries exceeds %d", slen, MM_SCHED_MAX);
+    std::vector<uint32_t> sched((size_t)MM_SLOTS * MM_SCHED_MAX, E_PAD);
+    for (int sl = 0; sl < MM_SLOTS; ++sl) std::copy(lists[sl].begin(), lists[sl].end(), sched.begin() + (size_t)sl * MM_SCHED_MAX);
+    int ce[MM_KMAX][3] = {};
+    float cf[MM_KMAX] = {};
+    for (int k = 0; k < I.K; ++k) {
+        for (int q = 0; q < 3; ++q) ce[k][q] = I.e[k][q];
+        cf[k] = (float)(1.0 / I.fact[k]);
+    }
+    int cn[MM_NEAR_MAX][3] = {};
+    int nnear = 0;
+    for (int a = -3; a <= 3; ++a)
+        for (int b = -3; b <= 3; ++b)
+            for (int d = -3; d <= 3; ++d)
+                if (a * a + b * b + d * d <= MM_NEAR2) {
+                    cn[nnear][0] = a;
+                    cn[nnear][1] = b;
+                    cn[nnear][2] = d;
+                    ++nnear;
+                }
+    CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_e), ce, sizeof ce, 0, hipMemcpyHostToDevice, st));
+    CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ifact), cf, sizeof cf, 0, hipMemcpyHostToDevice, st));
+    CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sched), sched.data(), sizeof(uint32_t) * sched.size(), 0,
+                                   hipMemcpyHostToDevice, st));
+    CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_near), cn, sizeof cn, 0, hipMemcpyHostToDevice, st));
+    // buffers
+    const int nz = (I.K + 1) / 2;
+    CT2_TRY(ensure_scratch((void**)&s.Dt, &s.Dt_bytes, dt_bytes, "multipole-mesh derivative table"));
+    CT2_TRY(ensure_scratch((void**)&s.Zm, &s.Z_bytes, sizeof(float2) * n3 * nz, "multipole grids"));
+    if (s.Zl) dev_free(s.Zl);
+    s.Zl = nullptr;
+    CT2_TRY(dev_alloc((void**)&s.Zl, sizeof(float2) * n3 * nz, "local-expansion grids"));
+    CT2_TRY(ensure_scratch((void**)&s.tabz, &s.tabz_bytes, sizeof(double2) * n3, "table transform"));
+    // the table: short-range samples on the host, one FP64 transform per component
+    std::vector<double> S;
+    short_samples(I, nc, S);
+    std::vector<double> z(2 * n3);
+    for (int k = 0; k < I.K; ++k) {
+        for (size_t q = 0; q < n3; ++q) {
+            z[2 * q] = S[(size_t)k * n3 + q];
+            z[2 * q + 1] = 0.0;
+        }
+        CT2_HIP(hipMemcpyAsync(s.tabz, z.data(), sizeof(double2) * n3, hipMemcpyHostToDevice, st));
+        CT2_TRY(fft3d(s.tabz, nc, 1, true));
+        mm_table_finish<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(s.tabz, nc, I.e[k][0], I.e[k][1], I.e[k][2],
+                                                                       s.Dt + (size_t)k * n3);
+        CT2_LAUNCH_CHECK();
+        CT2_HIP(hipStreamSynchronize(st));  // z is refilled next
+    }
+    s.nc = nc;
+    s.P = order;
+    s.K = I.K;
+    s.nz = nz;
+    s.slen = (int)slen;
+    s.nnear = nnear;
+    return CT2_OK;
+}
+
+extern "C" int ct2_mm_forces(void) {
+    CT2_TRY(mm_require());
+    CT2_TRY(mm_forces());
+    ctx().last_path = CT2_PATH_MM;
+    return CT2_OK;
+}
+
+extern "C" int ct2_kick_mm(const ct2_kick_params* p, unsigned flags, int* max_rung) {
+    CT2_TRY(mm_require());
+    CT2_TRY(check_kick_params(p));
+    if (flags & ~(unsigned)CT2_KICK_STORE) return set_error(CT2_ERR_INVALID, "unknown flags 0x%x", flags);
+    CT2_TRY(mm_forces());
+    ctx().last_path = CT2_PATH_MM;
+    return run_epilogue(p, max_rung);
+}
+
+extern "C" int ct2_mm_table_download(double* host) {
+    CT2_TRY(require_init());
+    if (!g_mm.nc) return set_error(CT2_ERR_STATE, "ct2_mm_init has not been called");
+    if (!host) return set_error(CT2_ERR_INVALID, "null output");
+    Context& c = ctx();
+    CT2_HIP(hipMemcpyAsync(host, g_mm.Dt, sizeof(double) * (size_t)g_mm.K * g_mm.nc * g_mm.nc * g_mm.nc,
+                           hipMemcpyDeviceToHost, c.stream));
+    CT2_HIP(hipStreamSynchronize(c.stream));
+    return CT2_OK;
+}

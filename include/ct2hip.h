@@ -247,6 +247,7 @@ int ct2_kick_pm_mlp(const ct2_kick_params* params, unsigned flags, int* max_rung
 #define CT2_PATH_PM 2      /* PM alone */
 #define CT2_PATH_PM_NEAR 3 /* PM + near field */
 #define CT2_PATH_PM_MLP 4  /* PM + learned correction */
+#define CT2_PATH_MM 5      /* multipole mesh + near-set direct pairs */
 int ct2_last_force_path(void);
 
 /* ------------------------------------------------------------ rungs */
@@ -298,6 +299,29 @@ int ct2_debug_fail_allocations(int after, int count);
  * x += fixed32(v * (dt/scale)) per axis, with fixed32(s) defined as
  * RNE(s * 2^32) mod 2^32 (0 for non-finite s).  Bit-exact. */
 int ct2_drift(float dt, float scale);
+
+/* ------------------------------------------------------------ multipole mesh */
+/* ct2's own far field (particle.cpp:116-271, kick.cpp:11-29, 105-125) with
+ * Cartesian Taylor expansions of total order `order` (1..8; ct2 uses 8) on
+ * nc^3 cells (nc a power of two in [8, 256]): multipoles of each cell
+ * (particle_compute_multipoles), one FP32 3-D FFT per pair of components, the
+ * product with the FP64 table of derivatives of the periodic Green's function
+ * (particle_greens: greens + greens_ewald, less the nearest image on the cells
+ * with |offset|^2 <= 10, which are summed directly), inverse FFTs
+ * (particle_compute_expansions), and at each particle the softened direct
+ * pairs with the near set's cells (kick_cell's neighbour loop, kick.cpp:36-67)
+ * plus the local expansion (L2P, kick.cpp:29).  ct2_mm_init builds the table
+ * (once per nc and order; FP64, K nc^3 values for K = (order+1)(order+2)
+ * (order+3)/6 components, at most 8 GiB). */
+int ct2_mm_init(int nc, int order);
+/* Force and potential of every particle into the stored force arrays. */
+int ct2_mm_forces(void);
+/* kick() with the multipole-mesh force: ct2_mm_forces then the epilogue of
+ * kick.cpp:75-99 on the current rung range.  flags: CT2_KICK_STORE. */
+int ct2_kick_mm(const ct2_kick_params* params, unsigned flags, int* max_rung);
+/* The transformed derivative table, [K][nc^3] doubles: Re (even degree) or
+ * Im (odd degree) of the DFT of D_n over the cell offsets, divided by nc^3. */
+int ct2_mm_table_download(double* host);
 
 /* ------------------------------------------------------------ FFT */
 /* In-place 3-D complex transform of an n^3 cube in device memory, natural
