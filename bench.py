@@ -117,6 +117,31 @@ def near_leg(api, ev: HipEvents, side: int, kick_ms: float, idx, J, reps: int, n
     return out
 
 
+def mm_leg(api, ev: HipEvents, n: int, kick_ms: float, idx, J, reps: int, order: int) -> dict:
+    """ct2's multipole mesh (near-set direct pairs + Cartesian expansions of
+    the given order through FFTs on about 64 particles per cell), timed and
+    judged like the PM legs; init_s is the one-time table build."""
+    nc = api.mm_cells_for(n)
+    t0 = time.perf_counter()
+    api.mm_init(nc, order)
+    init_s = time.perf_counter() - t0
+    from ct2_amd import _lib
+    lib = _lib.load()
+    _lib.check(lib.ct2_mm_forces())
+    ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_mm_forces()), reps)))
+    fx, fy, fz, pot = api.forces_download()
+    e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
+    d = pot[idx] - J[3]
+    return {"cells": nc, "order": order, "components": api.mm_ncomp(order), "ms": round(ms, 3),
+            "init_s": round(init_s, 2), "median_rel_err": float(np.median(e)),
+            "p90_rel_err": float(np.percentile(e, 90)), "p99_rel_err": float(np.percentile(e, 99)),
+            # the judge softens every pair's nearest image, the mesh only the direct pairs:
+            # a near-constant offset of the potential (GM sum eps^2 / 2r^3 over far pairs)
+            "pot_offset_gm": float(d.mean() * n), "pot_spread_rel": float(d.std() / np.abs(J[3]).mean()),
+            "speedup_vs_direct": round(kick_ms / ms, 1),
+            "target_median_1e-3_and_10x_met": bool(np.median(e) <= 1e-3 and kick_ms / ms >= 10.0)}
+
+
 def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: int, method: str) -> dict:
     """The stepping configuration: side^3 particles from a Gaussian random
     field, 512^3 grid, KDK leapfrog a = 0.02 -> 0.03 with FP64 factors, the
@@ -409,9 +434,11 @@ def main() -> int:
     ap.add_argument("--stepping-steps", type=int, default=10)
     ap.add_argument("--kick-only", action="store_true",
                     help="time the kick alone (for counter runs under rocprofv3 --pmc)")
-    ap.add_argument("--only", choices=["fft", "near"], default=None,
-                    help="time one leg alone (for counter runs under rocprofv3 --pmc): the FP32 FFTs, or the "
-                         "PM + near-field force (TSC, grid 2 side, split 1.5 cells, cutoff 5.5 split radii)")
+    ap.add_argument("--only", choices=["fft", "near", "mm"], default=None,
+                    help="time one leg alone (for counter runs under rocprofv3 --pmc): the FP32 FFTs, the "
+                         "PM + near-field force (TSC, grid 2 side, split 1.5 cells, cutoff 5.5 split radii), or "
+                         "the multipole mesh (about 64 particles per cell, order --mm-order)")
+    ap.add_argument("--mm-order", type=int, default=8, help="expansion order of --only mm")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -456,6 +483,17 @@ def main() -> int:
         ms = [float(v) for v in time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), args.reps)]
         line = {"leg": "pm_near", "n": n, "pm_grid": 2 * side, "rs_cells": 1.5, "rcut_rs": 5.5,
                 "cells_per_side": api.pm_near_cells(), "ms": ms}
+        api.finalize()
+        print(json.dumps(line), flush=True)
+        return 0
+    if args.only == "mm":
+        nc = api.mm_cells_for(n)
+        api.mm_init(nc, args.mm_order)
+        from ct2_amd import _lib
+        lib = _lib.load()
+        _lib.check(lib.ct2_mm_forces())
+        ms = [float(v) for v in time_calls(ev, lambda: _lib.check(lib.ct2_mm_forces()), args.reps)]
+        line = {"leg": "mm", "n": n, "cells": nc, "order": args.mm_order, "ms": ms}
         api.finalize()
         print(json.dumps(line), flush=True)
         return 0
@@ -521,6 +559,8 @@ def main() -> int:
     print("[bench] PM + near field ...", file=sys.stderr, flush=True)
     nearres = [near_leg(api, ev, side, kick_avg, idx, J, args.reps, ng, rs, rc, order)
                for ng, rs, rc, order in ((2 * side, 1.5, 5.5, 2), (2 * side, 2.0, 5.5, 2), (4 * side, 4.0, 5.5, 1))]
+    print("[bench] multipole mesh ...", file=sys.stderr, flush=True)
+    mmres = [mm_leg(api, ev, n, kick_avg, idx, J, args.reps, order) for order in (4, 6, 8)]
     print("[bench] learned-correction training ...", file=sys.stderr, flush=True)
     train = train_leg(api, ev, n)
     print("[bench] rung-subset kicks ...", file=sys.stderr, flush=True)
@@ -574,6 +614,7 @@ def main() -> int:
                   "frac_hbm": round(36 * n / drift_avg / 1e6 / HBM_PEAK_GBS, 4)},
         "pm_mlp_vs_direct": pmres,
         "pm_near_vs_direct": nearres,
+        "multipole_mesh_vs_direct": mmres,
         "rung_subset_kicks": subset,
         "host_io": host_io,
         "clustered_precision": precision,
