@@ -439,6 +439,7 @@ def main() -> int:
                          "PM + near-field force (TSC, grid 2 side, split 1.5 cells, cutoff 5.5 split radii), or "
                          "the multipole mesh (about 64 particles per cell, order --mm-order)")
     ap.add_argument("--mm-order", type=int, default=8, help="expansion order of --only mm")
+    ap.add_argument("--mm-cells", type=int, default=0, help="cell grid of --only mm (0: about 64 particles per cell)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -487,7 +488,7 @@ def main() -> int:
         print(json.dumps(line), flush=True)
         return 0
     if args.only == "mm":
-        nc = api.mm_cells_for(n)
+        nc = args.mm_cells or api.mm_cells_for(n)
         api.mm_init(nc, args.mm_order)
         from ct2_amd import _lib
         lib = _lib.load()

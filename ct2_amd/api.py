@@ -250,8 +250,10 @@ def mm_ncomp(order: int) -> int:
     return (order + 1) * (order + 2) * (order + 3) // 6
 
 
-def mm_cells_for(n: int, per_cell: int = 64) -> int:
-    """The power-of-two cell grid with about `per_cell` particles per cell (>= 8)."""
+def mm_cells_for(n: int, per_cell: int = 8) -> int:
+    """The power-of-two cell grid with about `per_cell` (at least) particles per
+    cell, at least 8 cells a side; at 8 per cell the near-set pairs and the
+    far field cost about the same."""
     nc = 8
     while (2 * nc) ** 3 * per_cell <= n and nc < 256:
         nc *= 2

@@ -243,6 +243,7 @@ __global__ __launch_bounds__((Lanes<true, T, N>::value)) void fft_rows(typename 
     load_twiddles<T, N, DIR>(tw, twg, Lanes<true, T, N>::value);
     const int p = threadIdx.x % CF::P;
     const int c = threadIdx.x / CF::P;
+    data += (size_t)blockIdx.y * ((size_t)N * N * N);  // cube blockIdx.y of a batch
     const long base = ((long)blockIdx.x * CF::ROWS + c) * N;
     if constexpr (nstages_of(N) == 1) __syncthreads();
     run_stages<T, N, true, DIR, 0, 1, U64>(data, sm, tw, base, 1, p, c, wsrc);
@@ -264,6 +265,7 @@ __global__ __launch_bounds__((Lanes<false, T, N>::value)) void fft_cols(typename
     load_twiddles<T, N, DIR>(tw, twg, Lanes<false, T, N>::value);
     const int c = threadIdx.x % CF::COLS;
     const int p = threadIdx.x / CF::COLS;
+    data += (size_t)blockIdx.y * ((size_t)N * N * N);  // cube blockIdx.y of a batch
     unsigned b = blockIdx.x;
 #if FFT_XCD_SWIZZLE
     // workgroups are dealt to the 8 XCDs in turn: give XCD x the contiguous
@@ -304,8 +306,10 @@ int twiddle_table(int N, const typename V2<T>::t** out) {
     return CT2_OK;
 }
 
+// batch cubes of N^3 one after the other (blockIdx.y); the uint64 source
+// (wsrc) only with batch 1
 template <typename T, int N, int DIR>
-int fft3d_launch(typename V2<T>::t* d, hipStream_t st, const unsigned long long* wsrc = nullptr) {
+int fft3d_launch(typename V2<T>::t* d, hipStream_t st, const unsigned long long* wsrc = nullptr, int batch = 1) {
     using CF = Cfg<T, N>;
     const typename V2<T>::t* tw = nullptr;
     CT2_TRY(twiddle_table<T>(N, &tw));
@@ -314,32 +318,34 @@ int fft3d_launch(typename V2<T>::t* d, hipStream_t st, const unsigned long long*
     if (wsrc)
         fft_rows<T, N, DIR, true><<<(unsigned)(n2 / CF::ROWS), Lanes<true, T, N>::value, 0, st>>>(d, tw, wsrc);
     else
-        fft_rows<T, N, DIR><<<(unsigned)(n2 / CF::ROWS), Lanes<true, T, N>::value, 0, st>>>(d, tw);
+        fft_rows<T, N, DIR><<<dim3((unsigned)(n2 / CF::ROWS), batch), Lanes<true, T, N>::value, 0, st>>>(d, tw);
     CT2_LAUNCH_CHECK();
     const int inner = N / Cfg<T, N, col_k<T, N>()>::COLS;
     // y: columns z, slabs x
-    fft_cols<T, N, DIR><<<(unsigned)(n * inner), Lanes<false, T, N>::value, 0, st>>>(d, tw, n, n2, inner, false);
+    fft_cols<T, N, DIR><<<dim3((unsigned)(n * inner), batch), Lanes<false, T, N>::value, 0, st>>>(d, tw, n, n2, inner,
+                                                                                                false);
     CT2_LAUNCH_CHECK();
     // x: columns z, slabs y
-    fft_cols<T, N, DIR><<<(unsigned)(n * inner), Lanes<false, T, N>::value, 0, st>>>(d, tw, n2, n, inner, true);
+    fft_cols<T, N, DIR><<<dim3((unsigned)(n * inner), batch), Lanes<false, T, N>::value, 0, st>>>(d, tw, n2, n, inner,
+                                                                                                true);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
 
 template <typename T, int DIR>
-int fft3d_dispatch(void* data, int n, hipStream_t st) {
+int fft3d_dispatch(void* data, int n, hipStream_t st, int batch = 1) {
     using T2 = typename V2<T>::t;
     T2* d = (T2*)data;
     switch (n) {
-        case 8: return fft3d_launch<T, 8, DIR>(d, st);
-        case 16: return fft3d_launch<T, 16, DIR>(d, st);
-        case 32: return fft3d_launch<T, 32, DIR>(d, st);
-        case 64: return fft3d_launch<T, 64, DIR>(d, st);
-        case 128: return fft3d_launch<T, 128, DIR>(d, st);
-        case 256: return fft3d_launch<T, 256, DIR>(d, st);
-        case 512: return fft3d_launch<T, 512, DIR>(d, st);
+        case 8: return fft3d_launch<T, 8, DIR>(d, st, nullptr, batch);
+        case 16: return fft3d_launch<T, 16, DIR>(d, st, nullptr, batch);
+        case 32: return fft3d_launch<T, 32, DIR>(d, st, nullptr, batch);
+        case 64: return fft3d_launch<T, 64, DIR>(d, st, nullptr, batch);
+        case 128: return fft3d_launch<T, 128, DIR>(d, st, nullptr, batch);
+        case 256: return fft3d_launch<T, 256, DIR>(d, st, nullptr, batch);
+        case 512: return fft3d_launch<T, 512, DIR>(d, st, nullptr, batch);
         case 1024:
-            if constexpr (sizeof(T) == 4) return fft3d_launch<T, 1024, DIR>(d, st);
+            if constexpr (sizeof(T) == 4) return fft3d_launch<T, 1024, DIR>(d, st, nullptr, batch);
             [[fallthrough]];
         default: return set_error(CT2_ERR_INVALID, "unsupported FFT size %d", n);
     }
@@ -367,16 +373,20 @@ int fft3d_density(const unsigned long long* W, void* out, int n) {
     }
 }
 
-int fft3d(void* data, int n, int dir, bool f64) {
+int fft3d_batch(void* data, int n, int dir, bool f64, int batch) {
     CT2_TRY(require_init());
     if (!data) return set_error(CT2_ERR_INVALID, "null FFT buffer");
     if (!is_pow2(n) || n < 8 || n > (f64 ? 512 : 1024))
         return set_error(CT2_ERR_INVALID, "FFT size %d is not a power of two in [8, %d]", n, f64 ? 512 : 1024);
     if (dir != 1 && dir != -1) return set_error(CT2_ERR_INVALID, "FFT direction must be +1 or -1");
+    if (batch < 1 || batch > 65535) return set_error(CT2_ERR_INVALID, "FFT batch %d outside [1, 65535]", batch);
     hipStream_t st = ctx().stream;
-    if (f64) return dir > 0 ? fft3d_dispatch<double, 1>(data, n, st) : fft3d_dispatch<double, -1>(data, n, st);
-    return dir > 0 ? fft3d_dispatch<float, 1>(data, n, st) : fft3d_dispatch<float, -1>(data, n, st);
+    if (f64)
+        return dir > 0 ? fft3d_dispatch<double, 1>(data, n, st, batch) : fft3d_dispatch<double, -1>(data, n, st, batch);
+    return dir > 0 ? fft3d_dispatch<float, 1>(data, n, st, batch) : fft3d_dispatch<float, -1>(data, n, st, batch);
 }
+
+int fft3d(void* data, int n, int dir, bool f64) { return fft3d_batch(data, n, dir, f64, 1); }
 
 void release_fft_plans() {
     for (auto& kv : g_tw) dev_free(kv.second);

@@ -61,6 +61,8 @@ int ensure_scratch(void** p, size_t* have, size_t bytes, const char* what);
 
 // In-place 3-D complex FFT on the library stream (fft.hip).
 int fft3d(void* data, int n, int dir, bool f64);
+// `batch` such cubes stored one after the other, in one set of launches.
+int fft3d_batch(void* data, int n, int dir, bool f64, int batch);
 int fft3d_density(const unsigned long long* W, void* out, int n);
 
 // PM force of the container into the context force arrays (pm.hip).

@@ -50,16 +50,17 @@ constexpr double MM_BETA = 0.5;    // Ewald split per cell
 constexpr int MM_QB = 16;          // wave vectors per MG2L workgroup
 constexpr int MM_SLOTS = 16;       // MG2L work lists per wave vector (256 threads)
 constexpr int MM_SCHED_MAX = 320;  // entries per list
-constexpr int MM_NEAR_MAX = 160;   // offsets of the near set (147 at |d|^2 <= 10)
+constexpr int MM_COLS_MAX = 48;    // (dx, dy) columns of the near set (37 at |d|^2 <= 10)
 constexpr int MM_WAVES = 4;        // waves per near-field workgroup
 
-// MG2L list entry: k | l << 8 | n << 16 | odd(n) << 24 | pad << 25 | flush << 26
-constexpr uint32_t E_ODD = 1u << 24, E_PAD = 1u << 25, E_FLUSH = 1u << 26;
+// MG2L list entry: l | n << 8 | j << 16 | odd(n) << 24 | pad << 25 | save << 26 | flush << 27.
+// The components 2j and 2j+1 share a list: at the last product of 2j (save)
+// the sum is set aside, at the last of 2j+1 (flush) L^_2j + i L^_2j+1 goes out.
+constexpr uint32_t E_ODD = 1u << 24, E_PAD = 1u << 25, E_SAVE = 1u << 26, E_FLUSH = 1u << 27;
 
 __constant__ int c_e[MM_KMAX][3];         // exponents of component c
 __constant__ float c_ifact[MM_KMAX];      // 1 / (a! b! c!)
-__constant__ uint32_t c_sched[MM_SLOTS][MM_SCHED_MAX];
-__constant__ int c_near[MM_NEAR_MAX][3];  // near-set cell offsets
+__constant__ int c_cols[MM_COLS_MAX][3];  // near set by column: dx, dy, half-height w in z
 
 struct MMIndex {
     int P = 0, K = 0;
@@ -99,7 +100,7 @@ struct MMIndex {
 
 struct MMState {
     int nc = 0, P = 0, K = 0, nz = 0;  // nz = complex grids (K/2 rounded up)
-    int slen = 0, nnear = 0;
+    int slen = 0, ncols = 0;
     double* Dt = nullptr;  // [K][nc^3]
     size_t Dt_bytes = 0;
     float2* Zm = nullptr;  // multipole grids [nz][nc^3], then their transforms
@@ -122,6 +123,8 @@ struct MMState {
     size_t scan_tmp_bytes = 0;
     double2* tabz = nullptr;  // one FP64 complex grid while the table is built
     size_t tabz_bytes = 0;
+    uint32_t* sched = nullptr;  // the MG2L lists [MM_SLOTS][slen]
+    size_t sched_bytes = 0;
 };
 MMState g_mm;
 
@@ -287,24 +290,49 @@ __global__ __launch_bounds__(256) void mm_bounds(const uint32_t* __restrict__ ke
     start[c] = lo;
 }
 
-__global__ __launch_bounds__(256) void mm_item_counts(const int* __restrict__ start, int n3, int* __restrict__ cnt) {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c < n3) cnt[c] = (start[c + 1] - start[c] + 63) >> 6;
-}
-
-// items[3 i + {0,1,2}] = (cell, first sorted slot, sinks <= 64); cnt holds the exclusive sum
-__global__ __launch_bounds__(256) void mm_items(const int* __restrict__ start, const int* __restrict__ first, int n3,
-                                                int* __restrict__ items, int* __restrict__ nitems) {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= n3) return;
-    const int s0 = start[c], s1 = start[c + 1];
-    int it = first[c];
-    for (int s = s0; s < s1; s += 64, ++it) {
-        items[3 * it] = c;
-        items[3 * it + 1] = s;
-        items[3 * it + 2] = min(64, s1 - s);
+// Work items of the near-set kernel: the sorted particles of each row of
+// cells (fixed x and y cell) cut into runs of at most 64 that span fewer than
+// zspan z cells.  One thread per row walks the row's cells; pass 0 counts the
+// row's items into cnt, pass 1 writes (row, first slot, sinks) at first[row].
+__global__ __launch_bounds__(256) void mm_row_items(const int* __restrict__ start, int lg, int zspan, int pass,
+                                                    int* __restrict__ cnt, const int* __restrict__ first,
+                                                    int* __restrict__ items, int* __restrict__ nitems) {
+    const int nc = 1 << lg, rows = nc * nc;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    int it = pass ? first[r] : 0;
+    int f = start[r * nc], c = 0, z0 = 0;
+    for (int z = 0; z < nc; ++z) {
+        int left = start[r * nc + z + 1] - start[r * nc + z];
+        while (left > 0) {
+            if (c == 64 || (c > 0 && z - z0 >= zspan)) {
+                if (pass) {
+                    items[3 * it] = r;
+                    items[3 * it + 1] = f;
+                    items[3 * it + 2] = c;
+                }
+                ++it;
+                f += c;
+                c = 0;
+            }
+            if (c == 0) z0 = z;
+            const int t = min(left, 64 - c);
+            c += t;
+            left -= t;
+        }
     }
-    if (c == n3 - 1) *nitems = it;
+    if (c > 0) {
+        if (pass) {
+            items[3 * it] = r;
+            items[3 * it + 1] = f;
+            items[3 * it + 2] = c;
+        }
+        ++it;
+    }
+    if (!pass)
+        cnt[r] = it;
+    else if (r == rows - 1)
+        *nitems = it;
 }
 
 // ------------------------------------------------------------ P2M
@@ -354,19 +382,21 @@ __global__ __launch_bounds__(256) void mm_p2m(const uint4* __restrict__ P, const
 // ------------------------------------------------------------ MG2L
 
 // MM_QB wave vectors per workgroup: the transformed multipoles unpacked from
-// the paired grids (Z(q) and Z(-q)) and the table rows in LDS; thread (slot,
-// qq) walks list `slot` of (k, l, k+l) products, all lists of equal length,
-// every k in one list, and writes L^_k at its last product; the local
-// expansions leave paired the same way for the inverse transforms.
-__global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, const double* __restrict__ Dt, int nc,
-                                               int K, int nz, int slen, float2* __restrict__ Zl) {
+// the paired grids (Z(q) and Z(-q)), the table rows and the product lists in
+// LDS; thread (slot, qq) walks list `slot` of products M^_l D^_n, all lists of
+// equal length, each pair of components (2j, 2j+1) in one list, and writes
+// L^_2j + i L^_2j+1 (paired like the multipoles, for the inverse transforms).
+__global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, const double* __restrict__ Dt,
+                                               const uint32_t* __restrict__ sched, int nc, int K, int nz, int slen,
+                                               float2* __restrict__ Zl) {
     __shared__ float2 Mh[MM_KMAX][MM_QB];
     __shared__ double Dq[MM_KMAX][MM_QB];
-    __shared__ float2 Lh[MM_KMAX][MM_QB];
+    __shared__ uint32_t Sl[MM_SLOTS * MM_SCHED_MAX];
     const size_t n3 = (size_t)nc * nc * nc;
     const size_t q0 = (size_t)blockIdx.x * MM_QB;
     const int t = threadIdx.x;
     const int m = nc - 1;
+    for (int e = t; e < MM_SLOTS * slen; e += 256) Sl[e] = sched[e];
     for (int e = t; e < nz * MM_QB; e += 256) {
         const int j = e / MM_QB, qq = e % MM_QB;
         const size_t q = q0 + qq;
@@ -382,12 +412,13 @@ __global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, co
     }
     __syncthreads();
     const int qq = t & (MM_QB - 1), slot = t / MM_QB;
-    double ar = 0.0, ai = 0.0;
+    const uint32_t* L = Sl + slot * slen;
+    double ar = 0.0, ai = 0.0, er = 0.0, ei = 0.0;
     for (int i = 0; i < slen; ++i) {
-        const uint32_t w = c_sched[slot][i];
+        const uint32_t w = L[i];
         if (w & E_PAD) continue;
-        const float2 mv = Mh[(w >> 8) & 255][qq];
-        const double d = Dq[(w >> 16) & 255][qq];
+        const float2 mv = Mh[w & 255][qq];
+        const double d = Dq[(w >> 8) & 255][qq];
         if (w & E_ODD) {  // M (i d)
             ar -= (double)mv.y * d;
             ai += (double)mv.x * d;
@@ -395,17 +426,15 @@ __global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, co
             ar += (double)mv.x * d;
             ai += (double)mv.y * d;
         }
-        if (w & E_FLUSH) {
-            Lh[w & 255][qq] = make_float2((float)ar, (float)ai);
+        if (w & E_SAVE) {
+            er = ar;
+            ei = ai;
             ar = ai = 0.0;
         }
-    }
-    __syncthreads();
-    for (int e = t; e < nz * MM_QB; e += 256) {
-        const int j = e / MM_QB, qq2 = e % MM_QB;
-        const float2 a = Lh[2 * j][qq2];
-        const float2 b = 2 * j + 1 < K ? Lh[2 * j + 1][qq2] : make_float2(0.0f, 0.0f);
-        Zl[(size_t)j * n3 + q0 + qq2] = make_float2(a.x - b.y, a.y + b.x);  // L_a + i L_b
+        if (w & E_FLUSH) {  // L_2j + i L_2j+1
+            Zl[(size_t)((w >> 16) & 255) * n3 + q0 + qq] = make_float2((float)(er - ai), (float)(ei + ar));
+            ar = ai = 0.0;
+        }
     }
 }
 
@@ -423,81 +452,122 @@ __device__ __forceinline__ void two_sum_f(float& s, float& c, float v) {
     s = t;
 }
 
-// One wave per work item (up to 64 sinks of one cell), items taken in turn
-// by the waves of a persistent grid.  Sources of each near-set cell stream
-// through the wave's LDS slot as FP32 offsets from the sink cell's centre
-// (the cell offsets are at most 3, so every such difference is the minimum
-// image); per cell their sum is folded in with TwoSum.  Then the local
-// expansion of the cell at the sink's offset.
-template <int KERNEL>
+// One wave per work item (up to 64 consecutive sinks of a row of cells,
+// spanning fewer than zspan z cells), items taken in turn by the waves of a
+// persistent grid.  For each (dx, dy) column of the near set (half-height w
+// in z) the sources of the column's cells z_first - w .. z_last + w stream
+// through the wave's LDS slot; a sink takes those whose cell is within w of
+// its own in z.  EXACT: separations from the fixed-point positions (any
+// grid); otherwise FP32 offsets from the centre of the item's first cell,
+// which are the minimum image when zspan + 3.5 < nc / 2 and nc >= 16 (a
+// near pair is less than 4 cells apart on every axis).  Each column's sum is
+// folded in with TwoSum; then the local expansion of the sink's cell.
+template <int KERNEL, bool EXACT>
 __global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __restrict__ P, const int* __restrict__ start,
                                                              const int* __restrict__ items, const int* __restrict__ nitems,
                                                              const float2* __restrict__ Zl, int lg, int Pord, int K,
-                                                             int nnear, NearK kc, double fnear, double pnear,
+                                                             int ncols, NearK kc, double fnear, double pnear,
                                                              double ffar, double pfar, float* __restrict__ fx,
                                                              float* __restrict__ fy, float* __restrict__ fz,
                                                              float* __restrict__ pot) {
-    __shared__ float4 S[MM_WAVES][64];
+    __shared__ uint4 S[MM_WAVES][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nc = 1 << lg, sh = 32 - lg, m = nc - 1;
     const size_t n3 = (size_t)nc * nc * nc;
+    const uint32_t half = 1u << (sh - 1);
     const int total = *nitems;
     for (int item = blockIdx.x * MM_WAVES + w; item < total; item += gridDim.x * MM_WAVES) {
-        const int cell = items[3 * item], first = items[3 * item + 1], cnt = items[3 * item + 2];
-        const int cx = cell >> (2 * lg), cy = (cell >> lg) & m, cz = cell & m;
-        const uint32_t half = 1u << (sh - 1);
-        const uint32_t rx = ((uint32_t)cx << sh) + half, ry = ((uint32_t)cy << sh) + half, rz = ((uint32_t)cz << sh) + half;
+        const int row = items[3 * item], first = items[3 * item + 1], cnt = items[3 * item + 2];
+        const int cx = row >> lg, cy = row & m;
         const bool live = lane < cnt;
-        const uint4 me = P[first + (live ? lane : 0)];
+        const uint4 me = P[first + (live ? lane : cnt - 1)];
+        const int zs = (int)(me.z >> sh);
+        const int zl = __builtin_amdgcn_readfirstlane(zs);
+        const int zh = __shfl(zs, cnt - 1);
+        const uint32_t rx = ((uint32_t)cx << sh) + half, ry = ((uint32_t)cy << sh) + half,
+                       rz = ((uint32_t)zl << sh) + half;
         const float ux = (float)(int32_t)(me.x - rx), uy = (float)(int32_t)(me.y - ry), uz = (float)(int32_t)(me.z - rz);
         float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f, cxs = 0.f, cys = 0.f, czs = 0.f, cps = 0.f;
-        for (int o = 0; o < nnear; ++o) {
-            const int sc = ((((cx + c_near[o][0]) & m) << lg | ((cy + c_near[o][1]) & m)) << lg) | ((cz + c_near[o][2]) & m);
-            const int j0 = start[sc], j1 = start[sc + 1];
+        for (int col = 0; col < ncols; ++col) {
+            const int dx = c_cols[col][0], dy = c_cols[col][1], hw = c_cols[col][2];
+            const int srow = (((cx + dx) & m) << lg) | ((cy + dy) & m);
+            const int span = zh - zl + 2 * hw + 1;
+            const int zsw = zs - hw;  // a source of cell zc counts iff (zc - zsw) mod nc <= 2 hw
+            int seg[2][2], nseg;
+            if (span >= nc) {
+                seg[0][0] = 0;
+                seg[0][1] = nc;
+                nseg = 1;
+            } else {
+                const int zlo = (zl - hw) & m, zhi = zlo + span;
+                seg[0][0] = zlo;
+                seg[0][1] = min(zhi, nc);
+                seg[1][0] = 0;
+                seg[1][1] = zhi - nc;
+                nseg = zhi > nc ? 2 : 1;
+            }
             float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
-            for (int jb = j0; jb < j1; jb += 64) {
-                const int nj = min(64, j1 - jb);
-                if (lane < nj) {
-                    const uint4 s = P[jb + lane];
-                    S[w][lane] = make_float4((float)(int32_t)(s.x - rx), (float)(int32_t)(s.y - ry),
-                                             (float)(int32_t)(s.z - rz), 0.0f);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                for (int j = 0; j < nj; ++j) {
-                    const float4 s = S[w][j];
-                    const float dx = s.x - ux, dy = s.y - uy, dz = s.z - uz;
-                    float k, p;
-                    if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
-                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, kc.eps2u)));
-                        const float wv = __builtin_amdgcn_rsqf(r2);
-                        k = (wv * wv) * wv;
-                        p = wv;
-                    } else {
-                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        const bool in = r2 < kc.h2u;
-                        const float wv = __builtin_amdgcn_rsqf(in ? kc.h2u : r2);
-                        k = in ? kc.hinv3u : (wv * wv) * wv;
-                        p = in ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : wv;
+            for (int g = 0; g < nseg; ++g) {
+                const int j0 = start[srow * nc + seg[g][0]], j1 = start[srow * nc + seg[g][1]];
+                for (int jb = j0; jb < j1; jb += 64) {
+                    const int nj = min(64, j1 - jb);
+                    if (lane < nj) {
+                        const uint4 q = P[jb + lane];
+                        const uint32_t zc = q.z >> sh;
+                        S[w][lane] = EXACT ? make_uint4(q.x, q.y, q.z, zc)
+                                           : make_uint4(__float_as_uint((float)(int32_t)(q.x - rx)),
+                                                        __float_as_uint((float)(int32_t)(q.y - ry)),
+                                                        __float_as_uint((float)(int32_t)(q.z - rz)), zc);
                     }
-                    tx = fmaf(dx, k, tx);
-                    ty = fmaf(dy, k, ty);
-                    tz = fmaf(dz, k, tz);
-                    tp += p;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    for (int j = 0; j < nj; ++j) {
+                        const uint4 q = S[w][j];
+                        if ((((int)q.w - zsw) & m) > 2 * hw) continue;
+                        float ddx, ddy, ddz;
+                        if constexpr (EXACT) {
+                            ddx = (float)(int32_t)(q.x - me.x);
+                            ddy = (float)(int32_t)(q.y - me.y);
+                            ddz = (float)(int32_t)(q.z - me.z);
+                        } else {
+                            ddx = __uint_as_float(q.x) - ux;
+                            ddy = __uint_as_float(q.y) - uy;
+                            ddz = __uint_as_float(q.z) - uz;
+                        }
+                        float k, p;
+                        if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+                            const float r2 = fmaf(ddz, ddz, fmaf(ddy, ddy, fmaf(ddx, ddx, kc.eps2u)));
+                            const float wv = __builtin_amdgcn_rsqf(r2);
+                            k = (wv * wv) * wv;
+                            p = wv;
+                        } else {
+                            const float r2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+                            const bool in = r2 < kc.h2u;
+                            const float wv = __builtin_amdgcn_rsqf(in ? kc.h2u : r2);
+                            k = in ? kc.hinv3u : (wv * wv) * wv;
+                            p = in ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : wv;
+                        }
+                        tx = fmaf(ddx, k, tx);
+                        ty = fmaf(ddy, k, ty);
+                        tz = fmaf(ddz, k, tz);
+                        tp += p;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
             two_sum_f(sx, cxs, tx);
             two_sum_f(sy, cys, ty);
             two_sum_f(sz, czs, tz);
             two_sum_f(sp, cps, tp);
         }
-        // the local expansion at a = offset in cells
+        // the local expansion of the sink's own cell at a = offset from its centre, in cells
+        const size_t cell = ((size_t)row << lg) | (size_t)zs;
         const float inv = ldexpf(1.0f, -sh);
-        const float ax = ux * inv, ay = uy * inv, az = uz * inv;
+        const float ax = (float)(int32_t)(me.x - rx) * inv, ay = (float)(int32_t)(me.y - ry) * inv,
+                    az = (float)(int32_t)(me.z - (((uint32_t)zs << sh) + half)) * inv;
         float pa[MM_PMAX + 1], pb[MM_PMAX + 1], pc[MM_PMAX + 1];
         pa[0] = pb[0] = pc[0] = 1.0f;
         for (int e = 1; e <= Pord; ++e) {
@@ -540,8 +610,10 @@ int mm_forces() {
     CT2_TRY(ensure_scratch((void**)&s.keys, &s.keys_bytes, sizeof(uint32_t) * 4 * (size_t)n, "multipole-mesh keys"));
     CT2_TRY(ensure_scratch((void**)&s.Ps, &s.Ps_bytes, sizeof(uint4) * (size_t)n, "multipole-mesh particles"));
     CT2_TRY(ensure_scratch((void**)&s.start, &s.start_bytes, sizeof(int) * ((size_t)n3 + 1), "cell starts"));
-    CT2_TRY(ensure_scratch((void**)&s.cnt, &s.cnt_bytes, sizeof(int) * 2 * (size_t)n3, "item counts"));
-    const size_t max_items = (size_t)n / 64 + (size_t)n3 + 1;
+    const int rows = nc * nc;
+    CT2_TRY(ensure_scratch((void**)&s.cnt, &s.cnt_bytes, sizeof(int) * 2 * (size_t)rows, "item counts"));
+    const int zspan = nc >= 32 ? 8 : nc >= 16 ? 4 : nc;  // the FP32 offsets stay minimum images
+    const size_t max_items = (size_t)n / 64 + (size_t)n3 / (size_t)std::min(zspan, nc) + (size_t)rows + 1;
     CT2_TRY(ensure_scratch((void**)&s.items, &s.items_bytes, sizeof(int) * (3 * max_items + 1), "work items"));
     s.nitems = s.items + 3 * max_items;
     uint32_t *kin = s.keys, *kout = s.keys + n, *vin = s.keys + 2L * n, *vout = s.keys + 3L * n;
@@ -556,22 +628,23 @@ int mm_forces() {
     CT2_LAUNCH_CHECK();
     mm_bounds<<<(unsigned)((n3 + 256) / 256), 256, 0, st>>>(kout, n, n3, s.start);
     CT2_LAUNCH_CHECK();
-    mm_item_counts<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(s.start, n3, s.cnt);
+    int* first = s.cnt + rows;
+    const unsigned rb = (unsigned)((rows + 255) / 256);
+    mm_row_items<<<rb, 256, 0, st>>>(s.start, lg, zspan, 0, s.cnt, first, s.items, s.nitems);
     CT2_LAUNCH_CHECK();
-    int* first = s.cnt + n3;
     tmp = 0;
-    CT2_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, s.cnt, first, n3, st));
+    CT2_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, s.cnt, first, rows, st));
     CT2_TRY(ensure_scratch(&s.scan_tmp, &s.scan_tmp_bytes, std::max<size_t>(tmp, 256), "item scan"));
-    CT2_HIP(hipcub::DeviceScan::ExclusiveSum(s.scan_tmp, tmp, s.cnt, first, n3, st));
-    mm_items<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(s.start, first, n3, s.items, s.nitems);
+    CT2_HIP(hipcub::DeviceScan::ExclusiveSum(s.scan_tmp, tmp, s.cnt, first, rows, st));
+    mm_row_items<<<rb, 256, 0, st>>>(s.start, lg, zspan, 1, s.cnt, first, s.items, s.nitems);
     CT2_LAUNCH_CHECK();
     // far field
     mm_p2m<<<(unsigned)n3, 256, 0, st>>>(s.Ps, s.start, lg, s.P, s.K, (size_t)n3, s.Zm);
     CT2_LAUNCH_CHECK();
-    for (int j = 0; j < s.nz; ++j) CT2_TRY(fft3d(s.Zm + (size_t)j * n3, nc, 1, false));
-    mm_mg2l<<<(unsigned)(n3 / MM_QB), 256, 0, st>>>(s.Zm, s.Dt, nc, s.K, s.nz, s.slen, s.Zl);
+    CT2_TRY(fft3d_batch(s.Zm, nc, 1, false, s.nz));
+    mm_mg2l<<<(unsigned)(n3 / MM_QB), 256, 0, st>>>(s.Zm, s.Dt, s.sched, nc, s.K, s.nz, s.slen, s.Zl);
     CT2_LAUNCH_CHECK();
-    for (int j = 0; j < s.nz; ++j) CT2_TRY(fft3d(s.Zl + (size_t)j * n3, nc, -1, false));
+    CT2_TRY(fft3d_batch(s.Zl, nc, -1, false, s.nz));
     // near pairs + L2P
     const double two32 = 4294967296.0, h = c.opts.hsoft, hu = h * two32, gm = c.opts.GM;
     NearK kc;
@@ -582,14 +655,19 @@ int mm_forces() {
     const double fnear = gm * two32 * two32, pnear = gm * two32;
     const double ffar = gm * (double)nc * nc, pfar = gm * (double)nc;
     const unsigned grid = 2048;
-    if (c.opts.kernel == CT2_KERNEL_PLUMMER)
-        mm_near_l2p<CT2_KERNEL_PLUMMER><<<grid, 64 * MM_WAVES, 0, st>>>(s.Ps, s.start, s.items, s.nitems, s.Zl, lg, s.P,
-                                                                       s.K, s.nnear, kc, fnear, pnear, ffar, pfar, c.fx,
-                                                                       c.fy, c.fz, c.pot);
+    const bool plummer = c.opts.kernel == CT2_KERNEL_PLUMMER, exact = nc < 16;
+#define CT2_MM_NEAR(KER, EX)                                                                                    \
+    mm_near_l2p<KER, EX><<<grid, 64 * MM_WAVES, 0, st>>>(s.Ps, s.start, s.items, s.nitems, s.Zl, lg, s.P, s.K, \
+                                                         s.ncols, kc, fnear, pnear, ffar, pfar, c.fx, c.fy, c.fz, c.pot)
+    if (plummer && exact)
+        CT2_MM_NEAR(CT2_KERNEL_PLUMMER, true);
+    else if (plummer)
+        CT2_MM_NEAR(CT2_KERNEL_PLUMMER, false);
+    else if (exact)
+        CT2_MM_NEAR(CT2_KERNEL_CT2, true);
     else
-        mm_near_l2p<CT2_KERNEL_CT2><<<grid, 64 * MM_WAVES, 0, st>>>(s.Ps, s.start, s.items, s.nitems, s.Zl, lg, s.P, s.K,
-                                                                   s.nnear, kc, fnear, pnear, ffar, pfar, c.fx, c.fy,
-                                                                   c.fz, c.pot);
+        CT2_MM_NEAR(CT2_KERNEL_CT2, false);
+#undef CT2_MM_NEAR
     CT2_LAUNCH_CHECK();
     c.forces_valid = true;
     return CT2_OK;
@@ -600,7 +678,7 @@ int mm_forces() {
 void release_mm() {
     MMState& s = g_mm;
     for (void* p : {(void*)s.Dt, (void*)s.Zm, (void*)s.Zl, (void*)s.keys, s.sort_tmp, (void*)s.Ps, (void*)s.start,
-                    (void*)s.items, (void*)s.cnt, s.scan_tmp, (void*)s.tabz})
+                    (void*)s.items, (void*)s.cnt, s.scan_tmp, (void*)s.tabz, (void*)s.sched})
         dev_free(p);
     s = MMState();
 }
@@ -624,52 +702,60 @@ extern "C" int ct2_mm_init(int nc, int order) {
     MMState& s = g_mm;
     hipStream_t st = c.stream;
     s.nc = 0;  // unusable until the table is complete
-    // the MG2L lists: products grouped by k, the k spread over MM_SLOTS lists
-    // by descending length (each to the shortest list), padded to equal length
-    std::vector<std::vector<uint32_t>> byk(I.K);
-    for (size_t p = 0; p < I.pa.size(); ++p) {
-        const int k = I.pa[p], l = I.pb[p], kl = I.pc[p];
-        byk[k].push_back((uint32_t)k | (uint32_t)l << 8 | (uint32_t)kl << 16 | ((I.deg(kl) & 1) ? E_ODD : 0u));
+    // the MG2L lists: products grouped by component pair (2j, 2j+1), the pairs
+    // spread over MM_SLOTS lists by descending length (each to the shortest
+    // list), padded to equal length
+    const int nz = (I.K + 1) / 2;
+    std::vector<std::vector<uint32_t>> byj(nz);
+    for (int k = 0; k < I.K; ++k) {
+        for (size_t p = 0; p < I.pa.size(); ++p) {
+            if (I.pa[p] != k) continue;
+            const int l = I.pb[p], kl = I.pc[p];
+            byj[k >> 1].push_back((uint32_t)l | (uint32_t)kl << 8 | (uint32_t)(k >> 1) << 16 |
+                                  ((I.deg(kl) & 1) ? E_ODD : 0u));
+        }
+        if (!(k & 1)) byj[k >> 1].back() |= E_SAVE;  // the end of the even component
     }
-    std::vector<int> korder(I.K);
-    for (int k = 0; k < I.K; ++k) korder[k] = k;
-    std::stable_sort(korder.begin(), korder.end(), [&](int a, int b) { return byk[a].size() > byk[b].size(); });
+    for (int j = 0; j < nz; ++j) byj[j].back() |= E_FLUSH;
+    std::vector<int> jorder(nz);
+    for (int j = 0; j < nz; ++j) jorder[j] = j;
+    std::stable_sort(jorder.begin(), jorder.end(), [&](int a, int b) { return byj[a].size() > byj[b].size(); });
     std::vector<std::vector<uint32_t>> lists(MM_SLOTS);
-    for (int k : korder) {
+    for (int j : jorder) {
         auto it = std::min_element(lists.begin(), lists.end(),
                                    [](const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) { return a.size() < b.size(); });
-        byk[k].back() |= E_FLUSH;
-        it->insert(it->end(), byk[k].begin(), byk[k].end());
+        it->insert(it->end(), byj[j].begin(), byj[j].end());
     }
     size_t slen = 0;
     for (auto& l : lists) slen = std::max(slen, l.size());
     if (slen > (size_t)MM_SCHED_MAX) return set_error(CT2_ERR_INVALID, "MG2L list of %zu entries exceeds %d", slen, MM_SCHED_MAX);
-    std::vector<uint32_t> sched((size_t)MM_SLOTS * MM_SCHED_MAX, E_PAD);
-    for (int sl = 0; sl < MM_SLOTS; ++sl) std::copy(lists[sl].begin(), lists[sl].end(), sched.begin() + (size_t)sl * MM_SCHED_MAX);
+    std::vector<uint32_t> sched((size_t)MM_SLOTS * slen, E_PAD);
+    for (int sl = 0; sl < MM_SLOTS; ++sl) std::copy(lists[sl].begin(), lists[sl].end(), sched.begin() + (size_t)sl * slen);
     int ce[MM_KMAX][3] = {};
     float cf[MM_KMAX] = {};
     for (int k = 0; k < I.K; ++k) {
         for (int q = 0; q < 3; ++q) ce[k][q] = I.e[k][q];
         cf[k] = (float)(1.0 / I.fact[k]);
     }
-    int cn[MM_NEAR_MAX][3] = {};
-    int nnear = 0;
+    // the near set |offset|^2 <= 10 as columns: (dx, dy) with dz in [-w, w]
+    int cn[MM_COLS_MAX][3] = {};
+    int ncols = 0;
     for (int a = -3; a <= 3; ++a)
         for (int b = -3; b <= 3; ++b)
-            for (int d = -3; d <= 3; ++d)
-                if (a * a + b * b + d * d <= MM_NEAR2) {
-                    cn[nnear][0] = a;
-                    cn[nnear][1] = b;
-                    cn[nnear][2] = d;
-                    ++nnear;
-                }
+            if (a * a + b * b <= MM_NEAR2) {
+                int hw = 0;
+                while (a * a + b * b + (hw + 1) * (hw + 1) <= MM_NEAR2) ++hw;
+                cn[ncols][0] = a;
+                cn[ncols][1] = b;
+                cn[ncols][2] = hw;
+                ++ncols;
+            }
     CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_e), ce, sizeof ce, 0, hipMemcpyHostToDevice, st));
     CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ifact), cf, sizeof cf, 0, hipMemcpyHostToDevice, st));
-    CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sched), sched.data(), sizeof(uint32_t) * sched.size(), 0,
-                                   hipMemcpyHostToDevice, st));
-    CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_near), cn, sizeof cn, 0, hipMemcpyHostToDevice, st));
+    CT2_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_cols), cn, sizeof cn, 0, hipMemcpyHostToDevice, st));
     // buffers
-    const int nz = (I.K + 1) / 2;
+    CT2_TRY(ensure_scratch((void**)&s.sched, &s.sched_bytes, sizeof(uint32_t) * sched.size(), "MG2L lists"));
+    CT2_HIP(hipMemcpyAsync(s.sched, sched.data(), sizeof(uint32_t) * sched.size(), hipMemcpyHostToDevice, st));
     CT2_TRY(ensure_scratch((void**)&s.Dt, &s.Dt_bytes, dt_bytes, "multipole-mesh derivative table"));
     CT2_TRY(ensure_scratch((void**)&s.Zm, &s.Z_bytes, sizeof(float2) * n3 * nz, "multipole grids"));
     if (s.Zl) dev_free(s.Zl);
@@ -697,7 +783,7 @@ extern "C" int ct2_mm_init(int nc, int order) {
     s.K = I.K;
     s.nz = nz;
     s.slen = (int)slen;
-    s.nnear = nnear;
+    s.ncols = ncols;
     return CT2_OK;
 }
 
