@@ -26,6 +26,10 @@
 #ifndef FFT_XCD_SWIZZLE
 #define FFT_XCD_SWIZZLE 1
 #endif
+#ifndef FFT_X_SLAB_FAST
+#define FFT_X_SLAB_FAST 1
+#endif
+
 
 namespace ct2 {
 namespace {
@@ -327,7 +331,7 @@ int fft3d_launch(typename V2<T>::t* d, hipStream_t st, const unsigned long long*
     CT2_LAUNCH_CHECK();
     // x: columns z, slabs y
     fft_cols<T, N, DIR><<<dim3((unsigned)(n * inner), batch), Lanes<false, T, N>::value, 0, st>>>(d, tw, n2, n, inner,
-                                                                                                true);
+                                                                                                FFT_X_SLAB_FAST);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
