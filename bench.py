@@ -106,8 +106,13 @@ def near_leg(api, ev: HipEvents, side: int, kick_ms: float, idx, J, reps: int, n
     ms = float(np.median(time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), reps)))
     fx, fy, fz, pot = api.forces_download()
     e = rel_err(np.stack([fx[idx], fy[idx], fz[idx]]), J[:3])
+    # the near field's work: pairs closer than rcut, N^2 (4/3) pi rcut^3 on a near-uniform set
+    rcut = rcut_rs * rs / ng
+    n = side ** 3
+    pairs = float(n) * n * 4.0 / 3.0 * np.pi * rcut ** 3
     out = {"assignment": {1: "cic", 2: "tsc"}[order], "pm_grid": ng, "rs_cells": rs, "rcut_rs": rcut_rs,
-           "cells_per_side": api.pm_near_cells(), "ms": round(ms, 3), "median_rel_err": float(np.median(e)),
+           "cells_per_side": api.pm_near_cells(), "ms": round(ms, 3), "near_pairs_within_rcut": pairs,
+           "near_pairs_per_s_incl_mesh": round(pairs / (ms * 1e-3), 1), "median_rel_err": float(np.median(e)),
            "p90_rel_err": float(np.percentile(e, 90)), "p99_rel_err": float(np.percentile(e, 99)),
            "pot_max_rel_err": float(np.max(np.abs(pot[idx] - J[3])) / np.abs(J[3]).max()),
            "speedup_vs_direct": round(kick_ms / ms, 1),
