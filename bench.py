@@ -124,7 +124,7 @@ def near_leg(api, ev: HipEvents, side: int, kick_ms: float, idx, J, reps: int, n
 
 def mm_leg(api, ev: HipEvents, n: int, kick_ms: float, idx, J, reps: int, order: int) -> dict:
     """ct2's multipole mesh (near-set direct pairs + Cartesian expansions of
-    the given order through FFTs on about 64 particles per cell), timed and
+    the given order through FFTs on about 8 particles per cell), timed and
     judged like the PM legs; init_s is the one-time table build."""
     nc = api.mm_cells_for(n)
     t0 = time.perf_counter()
@@ -154,7 +154,8 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
     "pm_mlp": PM (the assignment and split the model was trained on: TSC,
     1.0 cells) + the learned correction trained on this configuration's
     distribution (the force the configuration names);
-    "pm_near": TSC PM (split 2 cells) + the near field to 5.5 split radii.
+    "pm_near": TSC PM (split 2 cells) + the near field to 5.5 split radii;
+    "mm": ct2's multipole mesh (8 particles per cell, order 6).
     The direct kick at this N is not run (N^2 / rate at the measured rate
     would take minutes per step); its time is derived."""
     from ct2_amd import stepping
@@ -169,6 +170,8 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
         rec = stepping.run(side, 2 * side, method="pm_mlp", weights=trained[0],
                            rs_cells=float(meta.get("rs_cells", 1.25)), near=0.0, order=int(meta.get("order", 1)),
                            meta=meta, **kw)
+    elif method == "mm":
+        rec = stepping.run(side, 2 * side, method="mm", mm_order=6, **kw)
     else:
         rec = stepping.run(side, 2 * side, method="pm", rs_cells=2.0, near=5.5, order=2, **kw)
     n = side ** 3
@@ -176,6 +179,7 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
     med = [r["median_rel_err"] for r in rec["records"]]
     direct_ms = n * float(n) / rate * 1e3
     out = {"method": method, "n": n, "pm_grid": 2 * side, "assignment": rec["assignment"],
+           **({"mm_cells": rec["mm_cells"], "mm_order": rec["mm_order"]} if method == "mm" else {}),
            "rs_cells": rec["rs_cells"], "rcut_rs": rec["near_rcut_rs"], "steps": steps,
            "a": [rec["a_start"], rec["a_end"]], "force_kick_ms_median": round(float(np.median(ms)), 2),
            "drift_ms_median": round(float(np.median([r["drift_ms"] for r in rec["records"][:-1]])), 3),
@@ -439,12 +443,13 @@ def main() -> int:
     ap.add_argument("--stepping-steps", type=int, default=10)
     ap.add_argument("--kick-only", action="store_true",
                     help="time the kick alone (for counter runs under rocprofv3 --pmc)")
-    ap.add_argument("--only", choices=["fft", "near", "mm"], default=None,
+    ap.add_argument("--only", choices=["fft", "near", "mm", "mlp"], default=None,
                     help="time one leg alone (for counter runs under rocprofv3 --pmc): the FP32 FFTs, the "
                          "PM + near-field force (TSC, grid 2 side, split 1.5 cells, cutoff 5.5 split radii), or "
-                         "the multipole mesh (about 64 particles per cell, order --mm-order)")
+                         "the multipole mesh (about 8 particles per cell, order --mm-order), or PM + the "
+                         "learned correction (the committed model, grid --pm-ng)")
     ap.add_argument("--mm-order", type=int, default=8, help="expansion order of --only mm")
-    ap.add_argument("--mm-cells", type=int, default=0, help="cell grid of --only mm (0: about 64 particles per cell)")
+    ap.add_argument("--mm-cells", type=int, default=0, help="cell grid of --only mm (0: about 8 particles per cell)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -489,6 +494,26 @@ def main() -> int:
         ms = [float(v) for v in time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), args.reps)]
         line = {"leg": "pm_near", "n": n, "pm_grid": 2 * side, "rs_cells": 1.5, "rcut_rs": 5.5,
                 "cells_per_side": api.pm_near_cells(), "ms": ms}
+        api.finalize()
+        print(json.dumps(line), flush=True)
+        return 0
+    if args.only == "mlp":
+        # PM and PM + learned correction (the committed model's mesh) on the
+        # lattice, timed alone: the command the inference counter profile runs
+        from ct2_amd import _lib
+        from ct2_amd.train_mlp import DEFAULT_OUT, load_weights
+        lib = _lib.load()
+        w, meta = load_weights(DEFAULT_OUT)
+        api.pm_assignment(int(meta.get("order", 1)))
+        api.pm_init(args.pm_ng, float(meta.get("rs_cells", 1.25)))
+        api.mlp_init(1)
+        api.mlp_set_weights(w)
+        _lib.check(lib.ct2_pm_forces())
+        pm_ms = [float(v) for v in time_calls(ev, lambda: _lib.check(lib.ct2_pm_forces()), args.reps)]
+        _lib.check(lib.ct2_pm_mlp_forces())
+        ms = [float(v) for v in time_calls(ev, lambda: _lib.check(lib.ct2_pm_mlp_forces()), args.reps)]
+        line = {"leg": "pm_mlp", "n": n, "pm_grid": args.pm_ng, "pm_ms": pm_ms, "pm_mlp_ms": ms,
+                "mlp_tflops": round(189440 * n / (float(np.median(ms)) - float(np.median(pm_ms))) / 1e9, 2)}
         api.finalize()
         print(json.dumps(line), flush=True)
         return 0
@@ -578,7 +603,7 @@ def main() -> int:
     if args.stepping_side:
         print("[bench] stepping configuration ...", file=sys.stderr, flush=True)
         steps_res = [stepping_leg(api, ev, inter / (kick_avg * 1e-3), args.stepping_side, args.stepping_steps, 16384,
-                                  method) for method in ("pm_near", "pm_mlp")]
+                                  method) for method in ("pm_near", "pm_mlp", "mm")]
     pmres["direct_fp32_vs_fp64_min_image"] = {"median_rel_err": float(np.median(e_direct)),
                                               "p90": float(np.percentile(e_direct, 90)),
                                               "p99": float(np.percentile(e_direct, 99))}

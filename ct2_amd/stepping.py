@@ -13,8 +13,9 @@ with both factors integrated in FP64 (ct2_cosmos_kick_factor /
 ct2_cosmos_drift_factor).  The second half-kick of a step and the first of the
 next share one force evaluation.  Forces: PM + learned correction (or PM
 alone, or the direct sum), optionally with the direct near-field sum of
-pairs closer than `near` split radii (ct2's neighbour-cell structure); each
-step the periodic FP64 direct sum judges the force on sampled particles.
+pairs closer than `near` split radii (ct2's neighbour-cell structure), or
+ct2's multipole mesh (method mm); each step the periodic FP64 direct sum
+judges the force on sampled particles.
 
 Initial conditions: a side^3 lattice displaced by a Gaussian random field,
 density P(k) ~ k^-2 up to Nyquist, rms displacement `rms` cells per
@@ -39,7 +40,7 @@ def rel_err(F, J):
 
 
 def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, judge=16384, method="pm_mlp",
-        weights=None, timer=None, log=print, rs_cells=1.25, near=0.0, order=1, meta=None):
+        weights=None, timer=None, log=print, rs_cells=1.25, near=0.0, order=1, meta=None, mm_order=6, mm_cells=0):
     """Runs the loop; returns a record with per-step force/step times and the
     judged force errors.  `meta` (pm_mlp) is the trained model's metadata: the
     mesh it records is handed to the library, and the run stops if the kick
@@ -65,7 +66,10 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
         api.mlp_set_weights(weights)
         if meta is not None:
             api.mlp_set_mesh(*api.mlp_mesh_of(meta))
-    kick_fn = {"pm_mlp": lib.ct2_kick_pm_mlp, "pm": lib.ct2_kick_pm, "direct": lib.ct2_kick}[method]
+    if method == "mm":  # ct2's multipole mesh: direct pairs over the near cells + expansions of mm_order
+        mm_cells = mm_cells or api.mm_cells_for(n)
+        api.mm_init(mm_cells, mm_order)
+    kick_fn = {"pm_mlp": lib.ct2_kick_pm_mlp, "pm": lib.ct2_kick_pm, "direct": lib.ct2_kick, "mm": lib.ct2_kick_mm}[method]
 
     a = np.linspace(a_start, a_end, steps + 1)
     ah = 0.5 * (a[:-1] + a[1:])
@@ -115,7 +119,8 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
     if method in ("pm", "pm_mlp"):
         api.pm_near(0.0)
         api.pm_assignment(1)
-    return {"side": side, "n": n, "ng": ng, "method": method, "rs_cells": rs_cells, "near_rcut_rs": near,
+    out = {"mm_cells": mm_cells, "mm_order": mm_order} if method == "mm" else {}
+    return {**out, "side": side, "n": n, "ng": ng, "method": method, "rs_cells": rs_cells, "near_rcut_rs": near,
             "assignment": {1: "cic", 2: "tsc"}[order],
             "steps": steps, "a_start": a_start, "a_end": a_end,
             "records": records, "v_rms": float(np.sqrt(np.mean(p.vx.astype(np.float64) ** 2)))}
@@ -128,7 +133,8 @@ def main() -> int:
                     help="PM grid side (pm_mlp: the weights' grid-to-lattice ratio times --side; otherwise 2 side)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--judge", type=int, default=16384)
-    ap.add_argument("--method", default="pm_mlp", choices=["pm_mlp", "pm", "direct"])
+    ap.add_argument("--method", default="pm_mlp", choices=["pm_mlp", "pm", "direct", "mm"])
+    ap.add_argument("--mm-order", type=int, default=6, help="expansion order of the multipole mesh (method mm)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "ct2_amd", "weights", "mlp_grf_ng2x.npz"))
     ap.add_argument("--rs", type=float, default=None,
                     help="PM split radius in cells (pm_mlp: the weights' own; otherwise 1.25)")
@@ -172,7 +178,7 @@ def main() -> int:
     ev = HipEvents(api.stream())
     rec = run(args.side, args.ng, steps=args.steps, judge=args.judge, method=args.method, weights=w, timer=ev.time,
               log=lambda m: print(m, file=sys.stderr, flush=True), rs_cells=args.rs, near=args.near, order=args.assign,
-              meta=meta)
+              meta=meta, mm_order=args.mm_order)
     print(json.dumps(rec))
     api.finalize()
     return 0

@@ -91,7 +91,7 @@ def test_leapfrog_linear_growth(ct2):
     assert abs(growth - 1.5) > 0.004  # the filter's 0.5 % effect is resolved by the check
 
 
-@pytest.mark.parametrize("method", ["pm_mlp", "pm_near"])
+@pytest.mark.parametrize("method", ["pm_mlp", "pm_near", "mm"])
 def test_config5_steps(ct2, orc, method):
     """The stepping configuration at full size: 256^3 particles from a
     P(k) ~ k^-2 Gaussian random field (rms 0.3 cell), a 512^3 TSC grid, KDK
@@ -100,7 +100,8 @@ def test_config5_steps(ct2, orc, method):
     judged on 16,384 sampled particles against the periodic FP64 direct sum:
     PM + the committed learned correction (mlp_grf_ng2x.npz, its own mesh)
     at median <= 3e-3, PM + near field (rs 2 cells, cutoff 5.5 rs) at
-    median <= 1e-3; each drift is bit-exact against the oracle's on a sample."""
+    median <= 1e-3, ct2's multipole mesh (128^3 cells, order 6) at median
+    <= 3e-4; each drift is bit-exact against the oracle's on a sample."""
     import os
 
     from ct2_amd.train_mlp import load_weights
@@ -123,6 +124,9 @@ def test_config5_steps(ct2, orc, method):
         ct2.mlp_set_weights(w)
         ct2.mlp_set_mesh(*ct2.mlp_mesh_of(meta))
         kick, limit = ct2.kick_pm_mlp, 3e-3
+    elif method == "mm":
+        ct2.mm_init(ct2.mm_cells_for(n), 6)
+        kick, limit = ct2.kick_mm, 3e-4
     else:
         ct2.pm_assignment(2)
         ct2.pm_init(ng, 2.0)
@@ -135,7 +139,8 @@ def test_config5_steps(ct2, orc, method):
         pending = ct2.cosmos_kick_factor(cosmo, a[0], ah[0])
         for i in range(steps):
             kick(ct2.kick_params(ascending=True, top=True, rung=0, t0=float(2.0 * pending)), ct2.KICK_STORE)
-            assert ct2.last_force_path() == (ct2.PATH_PM_MLP if method == "pm_mlp" else ct2.PATH_PM_NEAR)
+            assert ct2.last_force_path() == {"pm_mlp": ct2.PATH_PM_MLP, "pm_near": ct2.PATH_PM_NEAR,
+                                             "mm": ct2.PATH_MM}[method]
             idx = np.sort(rng.choice(n, 16384, replace=False))
             fx, fy, fz, _ = ct2.forces_download()
             jx, jy, jz, _ = ct2.kick_sample_fp64(idx, periodic=True)
