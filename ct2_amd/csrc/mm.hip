@@ -452,16 +452,55 @@ __device__ __forceinline__ void two_sum_f(float& s, float& c, float v) {
     s = t;
 }
 
+// The pair term of the direct set: ddx, ddy, ddz in 2^-32 box units.
+template <int KERNEL>
+__device__ __forceinline__ void near_pair(float ddx, float ddy, float ddz, const NearK& kc, float& tx, float& ty,
+                                          float& tz, float& tp) {
+    float k, p;
+    if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
+        const float r2 = fmaf(ddz, ddz, fmaf(ddy, ddy, fmaf(ddx, ddx, kc.eps2u)));
+        const float wv = __builtin_amdgcn_rsqf(r2);
+        k = (wv * wv) * wv;
+        p = wv;
+    } else {
+        const float r2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+        const bool in = r2 < kc.h2u;
+        const float wv = __builtin_amdgcn_rsqf(in ? kc.h2u : r2);
+        k = in ? kc.hinv3u : (wv * wv) * wv;
+        p = in ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : wv;
+    }
+    tx = fmaf(ddx, k, tx);
+    ty = fmaf(ddy, k, ty);
+    tz = fmaf(ddz, k, tz);
+    tp += p;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int MM_STAGE = 256;  // sources per wave staged in LDS at a time
+#ifndef MM_NEAR_UNROLL
+#define MM_NEAR_UNROLL 4  // sources per step of a lane's run (LDS reads in flight)
+#endif
+
 // One wave per work item (up to 64 consecutive sinks of a row of cells,
 // spanning fewer than zspan z cells), items taken in turn by the waves of a
 // persistent grid.  For each (dx, dy) column of the near set (half-height w
-// in z) the sources of the column's cells z_first - w .. z_last + w stream
-// through the wave's LDS slot; a sink takes those whose cell is within w of
-// its own in z.  EXACT: separations from the fixed-point positions (any
-// grid); otherwise FP32 offsets from the centre of the item's first cell,
-// which are the minimum image when zspan + 3.5 < nc / 2 and nc >= 16 (a
-// near pair is less than 4 cells apart on every axis).  Each column's sum is
-// folded in with TwoSum; then the local expansion of the sink's cell.
+// in z) the sources of the column's cells z_first - w .. z_last + w (the
+// window) are staged in the wave's LDS slot in window order, up to MM_STAGE
+// at a time.  FP32 path (nc >= 16, where the window is shorter than the
+// column): a sink's own cells z - w .. z + w are one contiguous run of the
+// window, so each lane loops over exactly its run; the separations are FP32
+// offsets from the centre of the item's first cell, which are the minimum
+// image when zspan + 3.5 < nc / 2 (a near pair is less than 4 cells apart on
+// every axis).  EXACT path (8^3 cells, where the window can cover the whole
+// column and a sink's run wrap inside it): every lane walks the window and
+// takes the sources within w of its own cell, with separations from the
+// fixed-point positions.  Each column's sum is folded in with TwoSum; then
+// the local expansion of the sink's cell.
 template <int KERNEL, bool EXACT>
 __global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __restrict__ P, const int* __restrict__ start,
                                                              const int* __restrict__ items, const int* __restrict__ nitems,
@@ -470,7 +509,7 @@ __global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __rest
                                                              double ffar, double pfar, float* __restrict__ fx,
                                                              float* __restrict__ fy, float* __restrict__ fz,
                                                              float* __restrict__ pot) {
-    __shared__ uint4 S[MM_WAVES][64];
+    __shared__ uint4 S[MM_WAVES][MM_STAGE];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nc = 1 << lg, sh = 32 - lg, m = nc - 1;
     const size_t n3 = (size_t)nc * nc * nc;
@@ -491,72 +530,57 @@ __global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __rest
         for (int col = 0; col < ncols; ++col) {
             const int dx = c_cols[col][0], dy = c_cols[col][1], hw = c_cols[col][2];
             const int srow = (((cx + dx) & m) << lg) | ((cy + dy) & m);
-            const int span = zh - zl + 2 * hw + 1;
-            const int zsw = zs - hw;  // a source of cell zc counts iff (zc - zsw) mod nc <= 2 hw
-            int seg[2][2], nseg;
-            if (span >= nc) {
-                seg[0][0] = 0;
-                seg[0][1] = nc;
-                nseg = 1;
-            } else {
-                const int zlo = (zl - hw) & m, zhi = zlo + span;
-                seg[0][0] = zlo;
-                seg[0][1] = min(zhi, nc);
-                seg[1][0] = 0;
-                seg[1][1] = zhi - nc;
-                nseg = zhi > nc ? 2 : 1;
+            const int* st = start + (size_t)srow * nc;  // st[nc] is the end of the column
+            const int span = min(zh - zl + 2 * hw + 1, nc);
+            const int zlo = span == nc ? 0 : (zl - hw) & m;
+            // the window in order: cells zlo .. min(zlo + span, nc) - 1, then 0 .. zlo + span - nc - 1
+            const int s0 = st[zlo], w0 = st[min(zlo + span, nc)] - s0, s1 = st[0];
+            const int W = w0 + (zlo + span > nc ? st[zlo + span - nc] - s1 : 0);
+            int lo = 0, hi = 0;
+            if constexpr (!EXACT) {  // this sink's run: window cells zs - zl .. zs - zl + 2w
+                const int za = zlo + zs - zl, zb = za + 2 * hw + 1;
+                lo = za <= nc ? st[za] - s0 : w0 + st[za - nc] - s1;
+                hi = zb <= nc ? st[zb] - s0 : w0 + st[zb - nc] - s1;
             }
+            const int zsw = zs - hw;  // EXACT: a source of cell zc counts iff (zc - zsw) mod nc <= 2 hw
             float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
-            for (int g = 0; g < nseg; ++g) {
-                const int j0 = start[srow * nc + seg[g][0]], j1 = start[srow * nc + seg[g][1]];
-                for (int jb = j0; jb < j1; jb += 64) {
-                    const int nj = min(64, j1 - jb);
-                    if (lane < nj) {
-                        const uint4 q = P[jb + lane];
-                        const uint32_t zc = q.z >> sh;
-                        S[w][lane] = EXACT ? make_uint4(q.x, q.y, q.z, zc)
-                                           : make_uint4(__float_as_uint((float)(int32_t)(q.x - rx)),
-                                                        __float_as_uint((float)(int32_t)(q.y - ry)),
-                                                        __float_as_uint((float)(int32_t)(q.z - rz)), zc);
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int jb = 0; jb < W; jb += MM_STAGE) {
+                const int nj = min(MM_STAGE, W - jb);
+                for (int k = lane; k < nj; k += 64) {
+                    const int p = jb + k;
+                    const uint4 q = P[p < w0 ? s0 + p : s1 + (p - w0)];
+                    S[w][k] = EXACT ? make_uint4(q.x, q.y, q.z, q.z >> sh)
+                                    : make_uint4(__float_as_uint((float)(int32_t)(q.x - rx)),
+                                                 __float_as_uint((float)(int32_t)(q.y - ry)),
+                                                 __float_as_uint((float)(int32_t)(q.z - rz)), 0u);
+                }
+                wave_sync();
+                if constexpr (EXACT) {
                     for (int j = 0; j < nj; ++j) {
                         const uint4 q = S[w][j];
                         if ((((int)q.w - zsw) & m) > 2 * hw) continue;
-                        float ddx, ddy, ddz;
-                        if constexpr (EXACT) {
-                            ddx = (float)(int32_t)(q.x - me.x);
-                            ddy = (float)(int32_t)(q.y - me.y);
-                            ddz = (float)(int32_t)(q.z - me.z);
-                        } else {
-                            ddx = __uint_as_float(q.x) - ux;
-                            ddy = __uint_as_float(q.y) - uy;
-                            ddz = __uint_as_float(q.z) - uz;
-                        }
-                        float k, p;
-                        if constexpr (KERNEL == CT2_KERNEL_PLUMMER) {
-                            const float r2 = fmaf(ddz, ddz, fmaf(ddy, ddy, fmaf(ddx, ddx, kc.eps2u)));
-                            const float wv = __builtin_amdgcn_rsqf(r2);
-                            k = (wv * wv) * wv;
-                            p = wv;
-                        } else {
-                            const float r2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-                            const bool in = r2 < kc.h2u;
-                            const float wv = __builtin_amdgcn_rsqf(in ? kc.h2u : r2);
-                            k = in ? kc.hinv3u : (wv * wv) * wv;
-                            p = in ? fmaf(-0.5f * kc.hinv3u, r2, 1.5f * kc.hinvu) : wv;
-                        }
-                        tx = fmaf(ddx, k, tx);
-                        ty = fmaf(ddy, k, ty);
-                        tz = fmaf(ddz, k, tz);
-                        tp += p;
+                        near_pair<KERNEL>((float)(int32_t)(q.x - me.x), (float)(int32_t)(q.y - me.y),
+                                          (float)(int32_t)(q.z - me.z), kc, tx, ty, tz, tp);
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                } else {
+                    const int a = max(lo - jb, 0), b = min(hi - jb, nj);
+                    int j = a;
+                    for (; j + MM_NEAR_UNROLL - 1 < b; j += MM_NEAR_UNROLL) {
+                        uint4 q[MM_NEAR_UNROLL];
+#pragma unroll
+                        for (int u = 0; u < MM_NEAR_UNROLL; ++u) q[u] = S[w][j + u];
+#pragma unroll
+                        for (int u = 0; u < MM_NEAR_UNROLL; ++u)
+                            near_pair<KERNEL>(__uint_as_float(q[u].x) - ux, __uint_as_float(q[u].y) - uy,
+                                              __uint_as_float(q[u].z) - uz, kc, tx, ty, tz, tp);
+                    }
+                    for (; j < b; ++j) {
+                        const uint4 q = S[w][j];
+                        near_pair<KERNEL>(__uint_as_float(q.x) - ux, __uint_as_float(q.y) - uy,
+                                          __uint_as_float(q.z) - uz, kc, tx, ty, tz, tp);
+                    }
                 }
+                wave_sync();
             }
             two_sum_f(sx, cxs, tx);
             two_sum_f(sy, cys, ty);

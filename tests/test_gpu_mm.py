@@ -40,10 +40,13 @@ def test_table_matches_oracle(ct2, orc, nc, order):
     assert err.max() < 1e-11, err.max()
 
 
-@pytest.mark.parametrize("order", [1, 4, 8])
+@pytest.mark.parametrize("nc,order", [(8, 1), (8, 4), (8, 8), (16, 4), (16, 8), (32, 6)])
 @pytest.mark.parametrize("clustered", [False, True])
-def test_forces_vs_oracle(ct2, orc, order, clustered):
-    nc, n = 8, 3000
+def test_forces_vs_oracle(ct2, orc, nc, order, clustered):
+    """8^3 cells take the exact-separation path, 16^3 and 32^3 the FP32-offset
+    path with per-sink runs (clustered: columns of several hundred sources,
+    staged in more than one pass, and windows wrapping through z = 0)."""
+    n = 3000
     x, y, z = random_set(n, 21 + order, clustered)
     gm, soft = float(np.float32(1.0 / n)), float(np.float32(2e-3))
     ct2.particle_allocate(n)
@@ -54,7 +57,7 @@ def test_forces_vs_oracle(ct2, orc, order, clustered):
     assert ct2.last_force_path() == ct2.PATH_MM
     ox, oy, oz, op = orc.mm_forces(x, y, z, nc, order, gm, soft)
     e = rel_err(np.stack([fx, fy, fz]), np.stack([ox, oy, oz]))
-    print(f"order {order} clustered {clustered}: GPU vs oracle median {np.median(e):.2e} max {e.max():.2e}",
+    print(f"{nc}^3 cells order {order} clustered {clustered}: GPU vs oracle median {np.median(e):.2e} max {e.max():.2e}",
           flush=True)
     assert np.median(e) < 3e-6 and np.percentile(e, 99) < 5e-5, (np.median(e), e.max())
     assert np.max(np.abs(pot - op)) < 1e-5 * np.abs(op).max()  # the periodic potential crosses zero
@@ -79,6 +82,28 @@ def test_ct2_kernel_and_empty_cells(ct2, orc):
     e = rel_err(np.stack([fx, fy, fz]), np.stack([ox, oy, oz]))
     assert np.median(e) < 3e-6 and np.percentile(e, 99) < 5e-5, (np.median(e), e.max())
     assert np.max(np.abs(pot - op)) < 1e-5 * np.abs(op).max()  # the periodic potential crosses zero
+
+
+def test_dense_cluster_across_the_z_boundary(ct2, orc):
+    """A cluster of 2,500 particles (sigma 0.008 box) centred on a z cell face
+    at z = 0, on 16^3 cells: its columns hold windows of over a thousand
+    sources (staged in several passes) that wrap through z = 0."""
+    nc, n = 16, 4000
+    rng = np.random.default_rng(31)
+    p = rng.random((n, 3))
+    p[:2500] = np.array([0.40, 0.55, 0.0]) + 0.008 * rng.standard_normal((2500, 3))
+    x, y, z = [to_fixed(p[:, q]) for q in range(3)]
+    gm, soft = float(np.float32(1.0 / n)), float(np.float32(1e-3))
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z)
+    ct2.set_force_options(soft, gm, ct2.KERNEL_PLUMMER)
+    ct2.mm_init(nc, 4)
+    fx, fy, fz, pot = ct2.mm_forces()
+    ox, oy, oz, op = orc.mm_forces(x, y, z, nc, 4, gm, soft)
+    e = rel_err(np.stack([fx, fy, fz]), np.stack([ox, oy, oz]))
+    print(f"dense cluster: GPU vs oracle median {np.median(e):.2e} max {e.max():.2e}", flush=True)
+    assert np.median(e) < 3e-6 and np.percentile(e, 99) < 5e-5, (np.median(e), e.max())
+    assert np.max(np.abs(pot - op)) < 1e-5 * np.abs(op).max()
 
 
 def test_kick_mm_applies_the_epilogue_to_the_range(ct2, orc):
