@@ -49,14 +49,24 @@ constexpr int MM_NEAR2 = 10;       // |cell offset|^2 of the direct set (particl
 constexpr double MM_BETA = 0.5;    // Ewald split per cell
 constexpr int MM_QB = 16;          // wave vectors per MG2L workgroup
 constexpr int MM_SLOTS = 16;       // MG2L work lists per wave vector (256 threads)
-constexpr int MM_SCHED_MAX = 320;  // entries per list
+constexpr int MM_SCHED_MAX = 384;  // entries per list
+// list stride in words: rows of whole 16-byte groups, the lists of a wave 16 bytes apart modulo 128
+constexpr int mg2l_stride(int slen) { return (slen + 31) / 32 * 32 + 4; }
 constexpr int MM_COLS_MAX = 48;    // (dx, dy) columns of the near set (37 at |d|^2 <= 10)
 constexpr int MM_WAVES = 4;        // waves per near-field workgroup
 
-// MG2L list entry: l | n << 8 | j << 16 | odd(n) << 24 | pad << 25 | save << 26 | flush << 27.
-// The components 2j and 2j+1 share a list: at the last product of 2j (save)
-// the sum is set aside, at the last of 2j+1 (flush) L^_2j + i L^_2j+1 goes out.
-constexpr uint32_t E_ODD = 1u << 24, E_PAD = 1u << 25, E_SAVE = 1u << 26, E_FLUSH = 1u << 27;
+// MG2L list entry: flags | l << 7 | j << 15 | n << 22, so that l * 128 (the
+// byte offset of row l of the LDS multipoles) is a mask of the word and n * 64
+// (row n of the LDS table) a shift.  Entries come in groups of four products
+// M^_l D^_n of one output component and one parity of |n| (D^_n is real for
+// even |n|, i times real for odd); the group's last entry carries its flags.
+// The components 2j and 2j+1 share a list: at the last group of 2j (save) the
+// sum is set aside, at the last of 2j+1 (flush) L^_2j + i L^_2j+1 goes out.
+// Pads: l = 0 and n = K, a zero row.
+constexpr uint32_t G_ODD = 1u, G_SAVE = 2u, G_FLUSH = 4u;
+__host__ __device__ constexpr uint32_t mg2l_entry(int l, int n, int j, uint32_t flags) {
+    return flags | (uint32_t)l << 7 | (uint32_t)j << 15 | (uint32_t)n << 22;
+}
 
 __constant__ int c_e[MM_KMAX][3];         // exponents of component c
 __constant__ float c_ifact[MM_KMAX];      // 1 / (a! b! c!)
@@ -123,7 +133,7 @@ struct MMState {
     size_t scan_tmp_bytes = 0;
     double2* tabz = nullptr;  // one FP64 complex grid while the table is built
     size_t tabz_bytes = 0;
-    uint32_t* sched = nullptr;  // the MG2L lists [MM_SLOTS][slen]
+    uint32_t* sched = nullptr;  // the MG2L lists [MM_SLOTS][mg2l_stride(slen)]
     size_t sched_bytes = 0;
 };
 MMState g_mm;
@@ -337,66 +347,100 @@ __global__ __launch_bounds__(256) void mm_row_items(const int* __restrict__ star
 
 // ------------------------------------------------------------ P2M
 
-// One workgroup per cell: powers of -b of up to 256 particles in LDS, one
-// thread per component; FP64 sums.  Component c goes to grid c/2, real part
-// for even c, imaginary for odd (an odd K leaves the last imaginary part 0).
+// MM_P2M_CELLS consecutive cells per workgroup, MM_P2M_CELLS / 4 of them per
+// wave in turn: the powers of -b (offset from the cell centre, in cells) of up
+// to 64 particles in the wave's LDS slot, lane c (and c + 64, c + 128) sums
+// component c over them in FP64; the moments of the block's cells collect in
+// LDS and leave as rows of MM_P2M_CELLS consecutive cells per grid.  Component
+// c goes to grid c/2, real part for even c, imaginary for odd (an odd K leaves
+// the last imaginary part 0).
+constexpr int MM_P2M_CELLS = 32;
 __global__ __launch_bounds__(256) void mm_p2m(const uint4* __restrict__ P, const int* __restrict__ start, int lg, int Pord,
                                               int K, size_t n3, float2* __restrict__ Zm) {
-    __shared__ float pw[256][3][MM_PMAX + 1];
-    const int cell = blockIdx.x, t = threadIdx.x;
-    const int s0 = start[cell], s1 = start[cell + 1];
+    __shared__ float pw[4][64][3][MM_PMAX + 1];
+    __shared__ float Mo[MM_KMAX + 1][MM_P2M_CELLS];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const size_t cell0 = (size_t)blockIdx.x * MM_P2M_CELLS;
     const int sh = 32 - lg;
     const uint32_t mask = (1u << sh) - 1u, half = 1u << (sh - 1);
     const float inv = ldexpf(1.0f, -sh);
-    double acc = 0.0;
-    const int ea = t < K ? c_e[t][0] : 0, eb = t < K ? c_e[t][1] : 0, ec = t < K ? c_e[t][2] : 0;
-    for (int base = s0; base < s1; base += 256) {
-        __syncthreads();
-        if (base + t < s1) {
-            const uint4 p = P[base + t];
-            const uint32_t v[3] = {p.x, p.y, p.z};
+    int ex[3][3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const float mb = -(float)(int32_t)((v[q] & mask) - half) * inv;  // -b in cells
-                float w = 1.0f;
-                pw[t][q][0] = 1.0f;
-                for (int e = 1; e <= Pord; ++e) {
-                    w *= mb;
-                    pw[t][q][e] = w;
+    for (int r = 0; r < 3; ++r) {
+        const int c = lane + 64 * r;
+        ex[r][0] = c < K ? c_e[c][0] : 0;
+        ex[r][1] = c < K ? c_e[c][1] : 0;
+        ex[r][2] = c < K ? c_e[c][2] : 0;
+    }
+    for (int lc = w; lc < MM_P2M_CELLS; lc += 4) {
+        const size_t cell = cell0 + lc;
+        const int s0 = start[cell], s1 = start[cell + 1];
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int base = s0; base < s1; base += 64) {
+            if (base + lane < s1) {
+                const uint4 p = P[base + lane];
+                const uint32_t v[3] = {p.x, p.y, p.z};
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const float mb = -(float)(int32_t)((v[q] & mask) - half) * inv;  // -b in cells
+                    float pv = 1.0f;
+                    pw[w][lane][q][0] = 1.0f;
+                    for (int e = 1; e <= Pord; ++e) {
+                        pv *= mb;
+                        pw[w][lane][q][e] = pv;
+                    }
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int cnt = min(64, s1 - base);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (lane + 64 * r < K)
+                    for (int j = 0; j < cnt; ++j)
+                        acc[r] += (double)(pw[w][j][0][ex[r][0]] * pw[w][j][1][ex[r][1]] * pw[w][j][2][ex[r][2]]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        __syncthreads();
-        const int cnt = min(256, s1 - base);
-        if (t < K)
-            for (int j = 0; j < cnt; ++j) acc += (double)(pw[j][0][ea] * pw[j][1][eb] * pw[j][2][ec]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int c = lane + 64 * r;
+            if (c < K) Mo[c][lc] = (float)(acc[r] * (double)c_ifact[c]);
+            else if (c == K) Mo[c][lc] = 0.0f;  // the imaginary part of an unpaired last component
+        }
     }
-    if (t < K) {
-        float* z = (float*)(Zm + (size_t)(t >> 1) * n3 + cell);
-        z[t & 1] = (float)(acc * (double)c_ifact[t]);
-    } else if (t == K && (K & 1)) {
-        ((float*)(Zm + (size_t)(t >> 1) * n3 + cell))[1] = 0.0f;
+    __syncthreads();
+    // grid j, cells cell0 .. cell0 + MM_P2M_CELLS - 1: (M_2j, M_2j+1)
+    const int nz = (K + 1) / 2;
+    for (int e = t; e < nz * MM_P2M_CELLS; e += 256) {
+        const int jz = e / MM_P2M_CELLS, lc = e % MM_P2M_CELLS;
+        Zm[(size_t)jz * n3 + cell0 + lc] = make_float2(Mo[2 * jz][lc], Mo[2 * jz + 1][lc]);
     }
 }
 
 // ------------------------------------------------------------ MG2L
 
 // MM_QB wave vectors per workgroup: the transformed multipoles unpacked from
-// the paired grids (Z(q) and Z(-q)), the table rows and the product lists in
-// LDS; thread (slot, qq) walks list `slot` of products M^_l D^_n, all lists of
-// equal length, each pair of components (2j, 2j+1) in one list, and writes
-// L^_2j + i L^_2j+1 (paired like the multipoles, for the inverse transforms).
+// the paired grids (Z(q) and Z(-q)) and the table rows (rounded to FP32) in
+// LDS; thread (slot, qq) walks list `slot` (read through L1: every workgroup
+// reads the same lists) four products at a time, all lists of equal length,
+// each pair of components (2j, 2j+1) in one list, and writes L^_2j + i L^_2j+1
+// (paired like the multipoles, for the inverse transforms).  A group's four
+// products sum in FP32 (the multipoles are FP32 already), groups in FP64:
+// those with real D^_n into (ar, ai), those with imaginary D^_n (as M^ times
+// the real coefficient) into (br, bi); a component is (ar - bi) + i (ai + br).
+static_assert(MM_QB * sizeof(float2) == 128 && MM_QB * sizeof(float) == 64, "LDS rows of 128 and 64 bytes");
 __global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, const double* __restrict__ Dt,
                                                const uint32_t* __restrict__ sched, int nc, int K, int nz, int slen,
                                                float2* __restrict__ Zl) {
     __shared__ float2 Mh[MM_KMAX][MM_QB];
-    __shared__ double Dq[MM_KMAX][MM_QB];
-    __shared__ uint32_t Sl[MM_SLOTS * MM_SCHED_MAX];
+    __shared__ float Dq[MM_KMAX + 1][MM_QB];
     const size_t n3 = (size_t)nc * nc * nc;
     const size_t q0 = (size_t)blockIdx.x * MM_QB;
     const int t = threadIdx.x;
     const int m = nc - 1;
-    for (int e = t; e < MM_SLOTS * slen; e += 256) Sl[e] = sched[e];
     for (int e = t; e < nz * MM_QB; e += 256) {
         const int j = e / MM_QB, qq = e % MM_QB;
         const size_t q = q0 + qq;
@@ -408,32 +452,49 @@ __global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, co
     }
     for (int e = t; e < K * MM_QB; e += 256) {
         const int c = e / MM_QB, qq = e % MM_QB;
-        Dq[c][qq] = Dt[(size_t)c * n3 + q0 + qq];
+        Dq[c][qq] = (float)Dt[(size_t)c * n3 + q0 + qq];
     }
+    if (t < MM_QB) Dq[K][t] = 0.0f;  // the pads' row
     __syncthreads();
     const int qq = t & (MM_QB - 1), slot = t / MM_QB;
-    const uint32_t* L = Sl + slot * slen;
-    double ar = 0.0, ai = 0.0, er = 0.0, ei = 0.0;
-    for (int i = 0; i < slen; ++i) {
-        const uint32_t w = L[i];
-        if (w & E_PAD) continue;
-        const float2 mv = Mh[w & 255][qq];
-        const double d = Dq[(w >> 8) & 255][qq];
-        if (w & E_ODD) {  // M (i d)
-            ar -= (double)mv.y * d;
-            ai += (double)mv.x * d;
+    const uint32_t* L = sched + slot * mg2l_stride(slen);
+    const char* mb = (const char*)&Mh[0][qq];
+    const char* db = (const char*)&Dq[0][qq];
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0, er = 0.0, ei = 0.0;
+    for (int i = 0; i < slen; i += 4) {
+        const uint4 g = *(const uint4*)(L + i);
+        const uint32_t w[4] = {g.x, g.y, g.z, g.w};
+        float2 mv[4];
+        float d[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            mv[u] = *(const float2*)(mb + (w[u] & 0x7f80u));
+            d[u] = *(const float*)(db + ((w[u] >> 22) << 6));
+        }
+        float tr = 0.0f, ti = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            tr = fmaf(mv[u].x, d[u], tr);
+            ti = fmaf(mv[u].y, d[u], ti);
+        }
+        const uint32_t f = g.w;
+        if (f & G_ODD) {
+            br += (double)tr;
+            bi += (double)ti;
         } else {
-            ar += (double)mv.x * d;
-            ai += (double)mv.y * d;
+            ar += (double)tr;
+            ai += (double)ti;
         }
-        if (w & E_SAVE) {
-            er = ar;
-            ei = ai;
-            ar = ai = 0.0;
-        }
-        if (w & E_FLUSH) {  // L_2j + i L_2j+1
-            Zl[(size_t)((w >> 16) & 255) * n3 + q0 + qq] = make_float2((float)(er - ai), (float)(ei + ar));
-            ar = ai = 0.0;
+        if (f & (G_SAVE | G_FLUSH)) {
+            double lr = ar - bi, li = ai + br;
+            ar = ai = br = bi = 0.0;
+            if (f & G_SAVE) {
+                er = lr;
+                ei = li;
+                lr = li = 0.0;
+            }
+            if (f & G_FLUSH)  // L_2j + i L_2j+1
+                Zl[(size_t)((f >> 15) & 127) * n3 + q0 + qq] = make_float2((float)(er - li), (float)(ei + lr));
         }
     }
 }
@@ -663,7 +724,7 @@ int mm_forces() {
     mm_row_items<<<rb, 256, 0, st>>>(s.start, lg, zspan, 1, s.cnt, first, s.items, s.nitems);
     CT2_LAUNCH_CHECK();
     // far field
-    mm_p2m<<<(unsigned)n3, 256, 0, st>>>(s.Ps, s.start, lg, s.P, s.K, (size_t)n3, s.Zm);
+    mm_p2m<<<(unsigned)(n3 / MM_P2M_CELLS), 256, 0, st>>>(s.Ps, s.start, lg, s.P, s.K, (size_t)n3, s.Zm);
     CT2_LAUNCH_CHECK();
     CT2_TRY(fft3d_batch(s.Zm, nc, 1, false, s.nz));
     mm_mg2l<<<(unsigned)(n3 / MM_QB), 256, 0, st>>>(s.Zm, s.Dt, s.sched, nc, s.K, s.nz, s.slen, s.Zl);
@@ -726,21 +787,26 @@ extern "C" int ct2_mm_init(int nc, int order) {
     MMState& s = g_mm;
     hipStream_t st = c.stream;
     s.nc = 0;  // unusable until the table is complete
-    // the MG2L lists: products grouped by component pair (2j, 2j+1), the pairs
-    // spread over MM_SLOTS lists by descending length (each to the shortest
-    // list), padded to equal length
+    // the MG2L lists: per component, its products split by the parity of |n|
+    // into groups of four (padded), grouped by component pair (2j, 2j+1); the
+    // pairs spread over MM_SLOTS lists by descending length (each to the
+    // shortest list), padded with empty groups to equal length
     const int nz = (I.K + 1) / 2;
     std::vector<std::vector<uint32_t>> byj(nz);
     for (int k = 0; k < I.K; ++k) {
-        for (size_t p = 0; p < I.pa.size(); ++p) {
-            if (I.pa[p] != k) continue;
-            const int l = I.pb[p], kl = I.pc[p];
-            byj[k >> 1].push_back((uint32_t)l | (uint32_t)kl << 8 | (uint32_t)(k >> 1) << 16 |
-                                  ((I.deg(kl) & 1) ? E_ODD : 0u));
+        std::vector<uint32_t>& out = byj[k >> 1];
+        for (int odd = 0; odd < 2; ++odd) {
+            std::vector<uint32_t> grp;
+            for (size_t p = 0; p < I.pa.size(); ++p)
+                if (I.pa[p] == k && (I.deg(I.pc[p]) & 1) == odd) grp.push_back(mg2l_entry(I.pb[p], I.pc[p], k >> 1, 0));
+            if (grp.empty()) continue;
+            while (grp.size() % 4) grp.push_back(mg2l_entry(0, I.K, k >> 1, 0));
+            for (size_t g = 3; g < grp.size(); g += 4) grp[g] |= odd ? G_ODD : 0u;
+            out.insert(out.end(), grp.begin(), grp.end());
         }
-        if (!(k & 1)) byj[k >> 1].back() |= E_SAVE;  // the end of the even component
+        if (!(k & 1)) out.back() |= G_SAVE;  // the end of the even component
     }
-    for (int j = 0; j < nz; ++j) byj[j].back() |= E_FLUSH;
+    for (int j = 0; j < nz; ++j) byj[j].back() |= G_FLUSH;
     std::vector<int> jorder(nz);
     for (int j = 0; j < nz; ++j) jorder[j] = j;
     std::stable_sort(jorder.begin(), jorder.end(), [&](int a, int b) { return byj[a].size() > byj[b].size(); });
@@ -753,8 +819,9 @@ extern "C" int ct2_mm_init(int nc, int order) {
     size_t slen = 0;
     for (auto& l : lists) slen = std::max(slen, l.size());
     if (slen > (size_t)MM_SCHED_MAX) return set_error(CT2_ERR_INVALID, "MG2L list of %zu entries exceeds %d", slen, MM_SCHED_MAX);
-    std::vector<uint32_t> sched((size_t)MM_SLOTS * slen, E_PAD);
-    for (int sl = 0; sl < MM_SLOTS; ++sl) std::copy(lists[sl].begin(), lists[sl].end(), sched.begin() + (size_t)sl * slen);
+    const size_t sstride = (size_t)mg2l_stride((int)slen);
+    std::vector<uint32_t> sched((size_t)MM_SLOTS * sstride, mg2l_entry(0, I.K, 0, 0));
+    for (int sl = 0; sl < MM_SLOTS; ++sl) std::copy(lists[sl].begin(), lists[sl].end(), sched.begin() + (size_t)sl * sstride);
     int ce[MM_KMAX][3] = {};
     float cf[MM_KMAX] = {};
     for (int k = 0; k < I.K; ++k) {
