@@ -178,7 +178,7 @@ def stepping_leg(api, ev: HipEvents, rate: float, side: int, steps: int, judge: 
     ms = [r["force_kick_ms"] for r in rec["records"]][1:]  # the first call includes one-time set-up
     med = [r["median_rel_err"] for r in rec["records"]]
     direct_ms = n * float(n) / rate * 1e3
-    out = {"method": method, "n": n, "pm_grid": 2 * side, "assignment": rec["assignment"],
+    out = {"method": method, "n": n, "pm_grid": rec["ng"], "assignment": rec["assignment"],
            **({"mm_cells": rec["mm_cells"], "mm_order": rec["mm_order"]} if method == "mm" else {}),
            "rs_cells": rec["rs_cells"], "rcut_rs": rec["near_rcut_rs"], "steps": steps,
            "a": [rec["a_start"], rec["a_end"]], "force_kick_ms_median": round(float(np.median(ms)), 2),
@@ -443,11 +443,11 @@ def main() -> int:
     ap.add_argument("--stepping-steps", type=int, default=10)
     ap.add_argument("--kick-only", action="store_true",
                     help="time the kick alone (for counter runs under rocprofv3 --pmc)")
-    ap.add_argument("--only", choices=["fft", "near", "mm", "mlp"], default=None,
+    ap.add_argument("--only", choices=["fft", "near", "mm", "mlp", "train"], default=None,
                     help="time one leg alone (for counter runs under rocprofv3 --pmc): the FP32 FFTs, the "
                          "PM + near-field force (TSC, grid 2 side, split 1.5 cells, cutoff 5.5 split radii), or "
                          "the multipole mesh (about 8 particles per cell, order --mm-order), or PM + the "
-                         "learned correction (the committed model, grid --pm-ng)")
+                         "learned correction (the committed model, grid --pm-ng), or its training steps")
     ap.add_argument("--mm-order", type=int, default=8, help="expansion order of --only mm")
     ap.add_argument("--mm-cells", type=int, default=0, help="cell grid of --only mm (0: about 8 particles per cell)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -514,6 +514,13 @@ def main() -> int:
         ms = [float(v) for v in time_calls(ev, lambda: _lib.check(lib.ct2_pm_mlp_forces()), args.reps)]
         line = {"leg": "pm_mlp", "n": n, "pm_grid": args.pm_ng, "pm_ms": pm_ms, "pm_mlp_ms": ms,
                 "mlp_tflops": round(189440 * n / (float(np.median(ms)) - float(np.median(pm_ms))) / 1e9, 2)}
+        api.finalize()
+        print(json.dumps(line), flush=True)
+        return 0
+    if args.only == "train":
+        api.pm_assignment(2)
+        api.pm_init(args.pm_ng, 1.0)
+        line = {"leg": "mlp_training", "n": n, "pm_grid": args.pm_ng, "steps": train_leg(api, ev, n)}
         api.finalize()
         print(json.dumps(line), flush=True)
         return 0

@@ -119,9 +119,11 @@ def run(side=256, ng=512, a_start=0.02, a_end=0.03, steps=10, rms=0.3, seed=1, j
     if method in ("pm", "pm_mlp"):
         api.pm_near(0.0)
         api.pm_assignment(1)
+    mesh = method in ("pm", "pm_mlp")
     out = {"mm_cells": mm_cells, "mm_order": mm_order} if method == "mm" else {}
-    return {**out, "side": side, "n": n, "ng": ng, "method": method, "rs_cells": rs_cells, "near_rcut_rs": near,
-            "assignment": {1: "cic", 2: "tsc"}[order],
+    return {**out, "side": side, "n": n, "ng": ng if mesh else None, "method": method,
+            "rs_cells": rs_cells if mesh else None, "near_rcut_rs": near if mesh else None,
+            "assignment": {1: "cic", 2: "tsc"}[order] if mesh else None,
             "steps": steps, "a_start": a_start, "a_end": a_end,
             "records": records, "v_rms": float(np.sqrt(np.mean(p.vx.astype(np.float64) ** 2)))}
 
