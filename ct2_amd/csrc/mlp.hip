@@ -82,30 +82,65 @@ __device__ __forceinline__ float gelu_d(float z) {
 enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_DGELU = 3 };
 
 struct EpiArgs {
-    float* f32;      // fp32 output [M][ldc] (STORE, BIAS: result; BIAS_GELU: z; DGELU: dz)
+    float* f32;      // fp32 output [M][ldc] (STORE, BIAS: result; BIAS_GELU: z; DGELU: dz, optional)
     int ldc;
     bf16_t* h;       // bf16 output [M][ldc] (BIAS_GELU: gelu(z); DGELU: dz)
     bf16_t* ht;      // bf16 transposed output [N][ldt] (optional)
     int ldt;
     const float* bias;  // [N]
     const float* zin;   // DGELU: saved pre-activation [M][ldc]
+    float* cpart = nullptr;  // DGELU: column sums of dz over each 128-row block [M / 128][N] (optional)
 };
 
 __device__ __forceinline__ bf16x8 load8(const bf16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
 
+// Workgroup id -> (row block, column block, K slice) of a one-dimensional
+// grid of (M/128) (N/32) S workgroups.  Workgroups go to the 8 XCDs in turn
+// (id mod 8).  When the row blocks come in eights, every column block and K
+// slice of a row block runs on the same XCD, so its A rows are fetched into
+// that XCD's L2 once instead of once per column block; otherwise, when the K
+// slices come in eights, every tile of a slice runs on the same XCD, so the
+// slice's A and B rows are fetched there once.
+struct GemmBlock {
+    int rb, cb, z;
+};
+__device__ __forceinline__ GemmBlock gemm_block(int id, int nrb, int ncb, int S) {
+    GemmBlock g;
+    if (nrb % 8 == 0) {
+        const int xcd = id & 7, local = id >> 3, groups = nrb >> 3;
+        g.cb = local % ncb;
+        const int t = local / ncb;
+        g.rb = (t % groups) * 8 + xcd;
+        g.z = t / groups;
+    } else if (S % 8 == 0) {
+        const int xcd = id & 7, local = id >> 3, tiles = nrb * ncb;
+        const int t = local % tiles;
+        g.rb = t % nrb;
+        g.cb = t / nrb;
+        g.z = (local / tiles) * 8 + xcd;
+    } else {
+        g.rb = id % nrb;
+        const int t = id / nrb;
+        g.cb = t % ncb;
+        g.z = t / ncb;
+    }
+    return g;
+}
+
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt,
-                                                 int ldb, int M, int N, int K, EpiArgs ep) {
+                                                 int ldb, int M, int N, int K, int S, EpiArgs ep) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tm = (blockIdx.x * 4 + wave) * 32, tn = blockIdx.y * 32;
+    const GemmBlock gb = gemm_block((int)blockIdx.x, M / 128, N / 32, S);
+    const int tm = (gb.rb * 4 + wave) * 32, tn = gb.cb * 32;
     // a wave's output tile transposed, for the epilogues that also write C^T
     __shared__ __align__(16) bf16_t tt[(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) ? 4 : 1][32][32];
     if (tm >= M) return;
-    // split K: slice blockIdx.z of gridDim.z sums k in [kb, ke) (EPI_STORE only;
-    // the slice's partial product goes to plane z of ep.f32)
-    const int kc = ((K / 32 + gridDim.z - 1) / gridDim.z) * 32;
-    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
-    if constexpr (EPI == EPI_STORE) ep.f32 += (size_t)blockIdx.z * M * ep.ldc;
+    // split K: slice z of S sums k in [kb, ke) (EPI_STORE only; the slice's
+    // partial product goes to plane z of ep.f32)
+    const int kc = ((K / 32 + S - 1) / S) * 32;
+    const int kb = gb.z * kc, ke = min(K, kb + kc);
+    if constexpr (EPI == EPI_STORE) ep.f32 += (size_t)gb.z * M * ep.ldc;
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
     floatx4 acc[2][2] = {};
     const bf16_t* a0 = A + (size_t)(tm + r16) * lda + kq;
@@ -120,6 +155,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
         acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
     }
     // lane holds C[tm + 16 i + 4 (lane/16) + j][tn + 16 jj + lane%16]
+    float csum[2] = {0.0f, 0.0f};  // DGELU: this lane's sums of dz per column block jj
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -142,13 +178,33 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
                     tt[wave][col - tn][row - tm] = hv;
                 } else {  // EPI_DGELU
                     const float d = v * gelu_d(ep.zin[o]);
-                    ep.f32[o] = d;
+                    if (ep.f32) ep.f32[o] = d;
+                    csum[jj] += d;
                     const bf16_t hv = f2bf(d);
                     if (ep.h) ep.h[o] = hv;
                     tt[wave][col - tn][row - tm] = hv;
                 }
             }
         }
+    if constexpr (EPI == EPI_DGELU) {
+        // the bias gradient's share of this block: column sums over its 128
+        // rows (lane groups, then the four waves in order)
+        if (ep.cpart) {
+            __shared__ float cs[4][32];
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                float v = csum[jj];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (lane < 16) cs[wave][16 * jj + lane] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < 32) {
+                const int c = threadIdx.x;
+                ep.cpart[(size_t)gb.rb * N + tn + c] = ((cs[0][c] + cs[1][c]) + cs[2][c]) + cs[3][c];
+            }
+        }
+    }
     if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
         // the transposed copy: the wave's 32 x 32 tile through LDS, written as
         // 32 rows of 64 contiguous bytes (16 bytes per lane and store)
@@ -168,7 +224,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
 template <int EPI>
 int gemm(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K, const EpiArgs& ep) {
     if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
-    gemm_bf16<EPI><<<dim3(M / 128, N / 32, 1), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, ep);
+    gemm_bf16<EPI><<<(M / 128) * (N / 32), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
@@ -193,7 +249,7 @@ int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int 
     const int tiles = (M / 128) * (N / 32);
     const int S = std::max(1, std::min({smax, K / 256, (1024 + tiles - 1) / tiles}));
     EpiArgs ep{S > 1 ? scratch : out, N, nullptr, nullptr, 0, nullptr, nullptr};
-    gemm_bf16<EPI_STORE><<<dim3(M / 128, N / 32, S), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, ep);
+    gemm_bf16<EPI_STORE><<<(M / 128) * (N / 32) * S, 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
     CT2_LAUNCH_CHECK();
     if (S > 1) {
         splitk_reduce<<<(M * N + 255) / 256, 256, 0, st>>>(scratch, S, M * N, out);
@@ -655,7 +711,7 @@ struct Mlp {
     int cap = 0;
     bf16_t *X = nullptr, *XT = nullptr, *H1 = nullptr, *H1T = nullptr, *H2 = nullptr, *H2T = nullptr;
     bf16_t *dY = nullptr, *dYT = nullptr, *dZ2 = nullptr, *dZ2T = nullptr, *dZ1T = nullptr;
-    float *Z1 = nullptr, *Z2 = nullptr, *Yo = nullptr, *T = nullptr, *dYf = nullptr, *dZ2f = nullptr, *dZ1f = nullptr;
+    float *Z1 = nullptr, *Z2 = nullptr, *Yo = nullptr, *T = nullptr, *dYf = nullptr;
     float* lpart = nullptr;
     float* gsplit = nullptr;  // split-K planes of the weight-gradient products
     float* cspart = nullptr;  // per-block column sums of the bias gradients
@@ -672,13 +728,13 @@ void free_work() {
     for (void* p : {(void*)g_mlp.X, (void*)g_mlp.XT, (void*)g_mlp.H1, (void*)g_mlp.H1T, (void*)g_mlp.H2, (void*)g_mlp.H2T,
                     (void*)g_mlp.dY, (void*)g_mlp.dYT, (void*)g_mlp.dZ2, (void*)g_mlp.dZ2T, (void*)g_mlp.dZ1T,
                     (void*)g_mlp.Z1, (void*)g_mlp.Z2, (void*)g_mlp.Yo, (void*)g_mlp.T, (void*)g_mlp.dYf,
-                    (void*)g_mlp.dZ2f, (void*)g_mlp.dZ1f, (void*)g_mlp.lpart, (void*)g_mlp.gsplit,
+                    (void*)g_mlp.lpart, (void*)g_mlp.gsplit,
                     (void*)g_mlp.cspart, (void*)g_mlp.Xf, (void*)g_mlp.idx,
                     (void*)g_mlp.judge})
         dev_free(p);
     g_mlp.X = g_mlp.XT = g_mlp.H1 = g_mlp.H1T = g_mlp.H2 = g_mlp.H2T = nullptr;
     g_mlp.dY = g_mlp.dYT = g_mlp.dZ2 = g_mlp.dZ2T = g_mlp.dZ1T = nullptr;
-    g_mlp.Z1 = g_mlp.Z2 = g_mlp.Yo = g_mlp.T = g_mlp.dYf = g_mlp.dZ2f = g_mlp.dZ1f = g_mlp.lpart = g_mlp.Xf = nullptr;
+    g_mlp.Z1 = g_mlp.Z2 = g_mlp.Yo = g_mlp.T = g_mlp.dYf = g_mlp.lpart = g_mlp.Xf = nullptr;
     g_mlp.gsplit = g_mlp.cspart = nullptr;
     g_mlp.idx = nullptr;
     g_mlp.judge = nullptr;
@@ -715,8 +771,6 @@ int ensure_work(int rows, bool training) {
         ALLOC(g_mlp.Z2, R * HID * 4, "mlp Z2");
         ALLOC(g_mlp.T, R * 4 * 4, "mlp T");
         ALLOC(g_mlp.dYf, R * KOUT * 4, "mlp dYf");
-        ALLOC(g_mlp.dZ2f, R * HID * 4, "mlp dZ2f");
-        ALLOC(g_mlp.dZ1f, R * HID * 4, "mlp dZ1f");
         ALLOC(g_mlp.lpart, (R / 8 + 1) * 4, "mlp loss");
         ALLOC(g_mlp.gsplit, (size_t)SPLITK_MAX * HID * HID * 4, "mlp split-K planes");
         ALLOC(g_mlp.cspart, (R / COLSUM_ROWS + 1) * HID * 4, "mlp column sums");
@@ -763,17 +817,19 @@ int backward(hipStream_t st, int M) {
     CT2_TRY(gemm_splitk(st, g_mlp.H2T, M, g_mlp.dYT, M, HID, KOUT, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W3));
     CT2_TRY(colsum(st, g_mlp.dYf, M, KOUT, g_mlp.cspart, g + OFF_B3));
     // dZ2 = (dY W3^T) * gelu'(Z2)
-    EpiArgs d2{g_mlp.dZ2f, HID, g_mlp.dZ2, g_mlp.dZ2T, M, nullptr, g_mlp.Z2};
+    EpiArgs d2{nullptr, HID, g_mlp.dZ2, g_mlp.dZ2T, M, nullptr, g_mlp.Z2, g_mlp.cspart};
     CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dY, KOUT, g_mlp.W3n, KOUT, M, HID, KOUT, d2));
     // dW2 = H1^T dZ2, db2
     CT2_TRY(gemm_splitk(st, g_mlp.H1T, M, g_mlp.dZ2T, M, HID, HID, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W2));
-    CT2_TRY(colsum(st, g_mlp.dZ2f, M, HID, g_mlp.cspart, g + OFF_B2));
+    colsum_final<<<HID, 256, 0, st>>>(g_mlp.cspart, M / 128, HID, g + OFF_B2);
+    CT2_LAUNCH_CHECK();
     // dZ1 = (dZ2 W2^T) * gelu'(Z1)
-    EpiArgs d1{g_mlp.dZ1f, HID, nullptr, g_mlp.dZ1T, M, nullptr, g_mlp.Z1};
+    EpiArgs d1{nullptr, HID, nullptr, g_mlp.dZ1T, M, nullptr, g_mlp.Z1, g_mlp.cspart};
     CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dZ2, HID, g_mlp.W2n, HID, M, HID, HID, d1));
     // dW1 = X^T dZ1, db1
     CT2_TRY(gemm_splitk(st, g_mlp.XT, M, g_mlp.dZ1T, M, KIN, HID, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W1));
-    CT2_TRY(colsum(st, g_mlp.dZ1f, M, HID, g_mlp.cspart, g + OFF_B1));
+    colsum_final<<<HID, 256, 0, st>>>(g_mlp.cspart, M / 128, HID, g + OFF_B1);
+    CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
 

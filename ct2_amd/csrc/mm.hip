@@ -607,13 +607,20 @@ __global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __rest
             float tx = 0.f, ty = 0.f, tz = 0.f, tp = 0.f;
             for (int jb = 0; jb < W; jb += MM_STAGE) {
                 const int nj = min(MM_STAGE, W - jb);
+                // EXACT: 16-byte records (fixed-point position, cell z); FP32 path:
+                // planes of x, y and z offsets (4-byte entries: the runs of a wave's
+                // cells start at different banks)
+                float* const sf = (float*)&S[w][0];
                 for (int k = lane; k < nj; k += 64) {
                     const int p = jb + k;
                     const uint4 q = P[p < w0 ? s0 + p : s1 + (p - w0)];
-                    S[w][k] = EXACT ? make_uint4(q.x, q.y, q.z, q.z >> sh)
-                                    : make_uint4(__float_as_uint((float)(int32_t)(q.x - rx)),
-                                                 __float_as_uint((float)(int32_t)(q.y - ry)),
-                                                 __float_as_uint((float)(int32_t)(q.z - rz)), 0u);
+                    if constexpr (EXACT) {
+                        S[w][k] = make_uint4(q.x, q.y, q.z, q.z >> sh);
+                    } else {
+                        sf[k] = (float)(int32_t)(q.x - rx);
+                        sf[MM_STAGE + k] = (float)(int32_t)(q.y - ry);
+                        sf[2 * MM_STAGE + k] = (float)(int32_t)(q.z - rz);
+                    }
                 }
                 wave_sync();
                 if constexpr (EXACT) {
@@ -627,19 +634,20 @@ __global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __rest
                     const int a = max(lo - jb, 0), b = min(hi - jb, nj);
                     int j = a;
                     for (; j + MM_NEAR_UNROLL - 1 < b; j += MM_NEAR_UNROLL) {
-                        uint4 q[MM_NEAR_UNROLL];
+                        float qx[MM_NEAR_UNROLL], qy[MM_NEAR_UNROLL], qz[MM_NEAR_UNROLL];
 #pragma unroll
-                        for (int u = 0; u < MM_NEAR_UNROLL; ++u) q[u] = S[w][j + u];
+                        for (int u = 0; u < MM_NEAR_UNROLL; ++u) {
+                            qx[u] = sf[j + u];
+                            qy[u] = sf[MM_STAGE + j + u];
+                            qz[u] = sf[2 * MM_STAGE + j + u];
+                        }
 #pragma unroll
                         for (int u = 0; u < MM_NEAR_UNROLL; ++u)
-                            near_pair<KERNEL>(__uint_as_float(q[u].x) - ux, __uint_as_float(q[u].y) - uy,
-                                              __uint_as_float(q[u].z) - uz, kc, tx, ty, tz, tp);
+                            near_pair<KERNEL>(qx[u] - ux, qy[u] - uy, qz[u] - uz, kc, tx, ty, tz, tp);
                     }
-                    for (; j < b; ++j) {
-                        const uint4 q = S[w][j];
-                        near_pair<KERNEL>(__uint_as_float(q.x) - ux, __uint_as_float(q.y) - uy,
-                                          __uint_as_float(q.z) - uz, kc, tx, ty, tz, tp);
-                    }
+                    for (; j < b; ++j)
+                        near_pair<KERNEL>(sf[j] - ux, sf[MM_STAGE + j] - uy, sf[2 * MM_STAGE + j] - uz, kc, tx, ty,
+                                          tz, tp);
                 }
                 wave_sync();
             }
