@@ -598,7 +598,7 @@ def main() -> int:
     nearres = [near_leg(api, ev, side, kick_avg, idx, J, args.reps, ng, rs, rc, order)
                for ng, rs, rc, order in ((2 * side, 1.5, 5.5, 2), (2 * side, 2.0, 5.5, 2), (4 * side, 4.0, 5.5, 1))]
     print("[bench] multipole mesh ...", file=sys.stderr, flush=True)
-    mmres = [mm_leg(api, ev, n, kick_avg, idx, J, args.reps, order) for order in (4, 6, 8)]
+    mmres = [mm_leg(api, ev, n, kick_avg, idx, J, args.reps, order) for order in (4, 5, 6, 8)]
     print("[bench] learned-correction training ...", file=sys.stderr, flush=True)
     train = train_leg(api, ev, n)
     print("[bench] rung-subset kicks ...", file=sys.stderr, flush=True)

@@ -26,6 +26,7 @@ int g_ncell = 0;                   // the last particle_to_grid
 std::vector<uint64_t> g_starts;    // its cell starts, g_ncell^3 + 1
 int g_mesh_ng = 0;   // 0: direct-sum kick
 bool g_learned = false;  // with g_mesh_ng > 0: PM + learned correction instead of PM + near field
+int g_mm_cells = 0;      // > 0: the multipole mesh (takes precedence over the PM modes)
 std::FILE* g_trace = nullptr;
 
 // ct2 aborts on allocation failure (memory.cpp:7-10); the binding does the
@@ -64,6 +65,7 @@ void ct2_gpu_set_grid(int ppcell, int bwidth) {
 
 void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order) {
     ensure_init();
+    g_mm_cells = 0;
     g_mesh_ng = ng;
     g_learned = false;
     if (ng > 0) {
@@ -82,8 +84,15 @@ void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells
     check(ct2_mlp_init(1), "ct2_mlp_init");
     check(ct2_mlp_set_weights(weights, n), "ct2_mlp_set_weights");
     check(ct2_mlp_set_mesh(rs_cells, order, 0.0, particles_per_node), "ct2_mlp_set_mesh");
+    g_mm_cells = 0;
     g_mesh_ng = ng;
     g_learned = true;
+}
+
+void ct2_gpu_use_multipoles(int ncell, int order) {
+    ensure_init();
+    if (ncell > 0) check(ct2_mm_init(ncell, order), "ct2_mm_init");
+    g_mm_cells = ncell;
 }
 
 void ct2_gpu_trace(std::FILE* f) { g_trace = f; }
@@ -124,7 +133,9 @@ int kick(const kick_params& p) {
           "upload");
     const ct2_kick_params q{p.ascending, p.descending, p.top, p.first, p.rung, p.t0, p.eta, p.max_dt, p.scale};
     int max_rung = 0;
-    if (g_mesh_ng > 0 && g_learned)
+    if (g_mm_cells > 0)
+        check(ct2_kick_mm(&q, CT2_KICK_STORE, &max_rung), "ct2_kick_mm");
+    else if (g_mesh_ng > 0 && g_learned)
         check(ct2_kick_pm_mlp(&q, CT2_KICK_STORE, &max_rung), "ct2_kick_pm_mlp");
     else if (g_mesh_ng > 0)
         check(ct2_kick_pm(&q, CT2_KICK_STORE, &max_rung), "ct2_kick_pm");

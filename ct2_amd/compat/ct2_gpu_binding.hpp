@@ -82,6 +82,12 @@ void ct2_gpu_use_mesh(int ng, double rs_cells, double rcut_rs, int order);
 // weights were trained for (ct2_kick_pm_mlp).
 void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells, int order,
                          double particles_per_node = 0.0);
+// ... or to ct2's own structure (kick.cpp:105-125, particle.cpp:116-271):
+// direct pairs over the cells with |offset|^2 <= 10 and the multipole mesh of
+// the given order (<= 8, ct2's PORDER) on ncell^3 cells, ncell a power of two
+// (about 8 particles per cell is fastest); ncell = 0 switches back to the
+// direct sum.
+void ct2_gpu_use_multipoles(int ncell, int order);
 
 // kick_step (kick.hpp:23, main.cpp:8-92) over the functions above, for
 // callers that do not link ct2's main.cpp (which runs unchanged on them).

@@ -108,6 +108,34 @@ static int learned_mode(const char* path) {
     return 0;
 }
 
+// kick() switched to the multipole mesh (16^3 cells, order 6) on 4,096 LCG
+// particles with the binding's default kernel; prints max_rung, then vx vy vz
+// rung per particle.
+static int mm_mode() {
+    const int n = 4096;
+    particle_allocate(n);
+    ct2_gpu_set_options(0.01f, 1.0f / n);
+    unsigned s = 4242u;
+    for (int i = 0; i < n; i++) {
+        s = s * 1664525u + 1013904223u;
+        particle_x_ptr[i] = s;
+        s = s * 1664525u + 1013904223u;
+        particle_y_ptr[i] = s;
+        s = s * 1664525u + 1013904223u;
+        particle_z_ptr[i] = s;
+        particle_vx_ptr[i] = particle_vy_ptr[i] = particle_vz_ptr[i] = 0.0f;
+        particle_rung_ptr[i] = 0;
+    }
+    ct2_gpu_use_multipoles(16, 6);
+    kick_params kp{true, true, true, false, 0, 0.01f, 0.15f, 0.01f, 1.0f};
+    const int max_rung = kick(kp);
+    std::printf("max_rung %d\n", max_rung);
+    for (int i = 0; i < n; i++)
+        std::printf("%.9e %.9e %.9e %d\n", particle_vx_ptr[i], particle_vy_ptr[i], particle_vz_ptr[i],
+                    (int)particle_rung_ptr[i]);
+    return 0;
+}
+
 // particles_init, then particle_to_grid over the rung range [first, n) left by
 // particle_sort_by_rung(1) (rungs 0/1 alternate in blocks of 7); prints the
 // range, the grid's dimension, each cell's range, the neighbour count of a
@@ -146,6 +174,7 @@ int main(int argc, char** argv) {
     if (argc > 1 && std::strcmp(argv[1], "fft") == 0) return fft_mode();
     if (argc > 1 && std::strcmp(argv[1], "rungs") == 0) return rungs_mode();
     if (argc > 1 && std::strcmp(argv[1], "grid") == 0) return grid_mode();
+    if (argc > 1 && std::strcmp(argv[1], "mm") == 0) return mm_mode();
     const int n = argc > 1 ? std::atoi(argv[1]) : 512;
     particle_allocate(n);
     if (argc > 2)

@@ -156,3 +156,32 @@ def test_grid_accessors_through_ct2_signatures(ct2):
     starts = first + np.searchsorted(key, np.arange(nc ** 3), side="left")
     ends = first + np.searchsorted(key, np.arange(nc ** 3), side="right")
     assert np.array_equal(cells[:, 0], starts) and np.array_equal(cells[:, 1], ends)
+
+
+def test_multipole_mesh_through_kick(ct2):
+    """kick() switched to ct2's own structure (ct2_gpu_use_multipoles: direct
+    pairs over the cells with |offset|^2 <= 10 + the multipole mesh, 16^3
+    cells, order 6) gives, bit for bit, the velocities and rungs of the
+    library's own ct2_kick_mm on the same particles."""
+    r = subprocess.run([EXE, "mm"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    max_rung = int(lines[0].split()[1])
+    rows = [l.split() for l in lines[1:]]
+    got = np.array([[float(v) for v in row[:3]] for row in rows], np.float32).T
+    got_rung = np.array([int(row[3]) for row in rows])
+    n = got.shape[1]
+    x, y, z = lcg_positions(n, 4242)
+    zf = np.zeros(n, np.float32)
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z, zf, zf, zf, np.zeros(n, np.int8))
+    ct2.set_force_options(float(np.float32(0.01)), float(np.float32(1.0 / n)), 1)  # the binding's default kernel
+    ct2.mm_init(16, 6)
+    params = ct2.kick_params(ascending=True, descending=True, top=True, first=False, rung=0, t0=0.01, eta=0.15,
+                             max_dt=0.01, scale=1.0)
+    want_max = ct2.kick_mm(params)
+    p = ct2.particles_download()
+    assert max_rung == want_max
+    assert np.array_equal(got, np.stack([p.vx, p.vy, p.vz]))
+    assert np.array_equal(got_rung, p.rung.astype(int))
+    assert np.abs(got).max() > 0
