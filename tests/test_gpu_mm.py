@@ -106,6 +106,26 @@ def test_dense_cluster_across_the_z_boundary(ct2, orc):
     assert np.max(np.abs(pot - op)) < 1e-5 * np.abs(op).max()
 
 
+@pytest.mark.parametrize("nc,n", [(8, 1), (8, 5), (16, 2), (16, 37), (32, 64)])
+def test_few_particles(ct2, orc, nc, n):
+    """Ragged and tiny sets: one particle (its force is the periodic self term,
+    zero by symmetry, up to the expansion's truncation), a handful, and grids
+    with far more cells than particles."""
+    x, y, z = random_set(n, 100 + n)
+    gm, soft = float(np.float32(1.0 / max(n, 2))), float(np.float32(2e-3))
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z)
+    ct2.set_force_options(soft, gm, ct2.KERNEL_PLUMMER)
+    ct2.mm_init(nc, 4)
+    fx, fy, fz, pot = ct2.mm_forces()
+    ox, oy, oz, op = orc.mm_forces(x, y, z, nc, 4, gm, soft)
+    F, W = np.stack([fx, fy, fz]).astype(np.float64), np.stack([ox, oy, oz])
+    # one particle: its force is zero, measured against a pair one cell apart (GM nc^2)
+    scale = np.abs(W).max() if n > 1 else gm * nc * nc
+    assert np.abs(F - W).max() < 1e-5 * scale, (np.abs(F - W).max(), scale)
+    assert np.max(np.abs(pot - op)) < 1e-5 * max(np.abs(op).max(), gm)
+
+
 def test_kick_mm_applies_the_epilogue_to_the_range(ct2, orc):
     n = 4000
     x, y, z = random_set(n, 8)
