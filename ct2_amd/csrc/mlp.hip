@@ -531,13 +531,13 @@ __global__ void pack_infer(const float* __restrict__ w, bf16_t* __restrict__ W2p
     }
 }
 
-// bias + GELU of a 256-unit column block, packed as the next layer's B operand
-// (bias points at the lane's first unit, 4 * (lane / 16)): units 4 of step r
-// go to words 2 (r & 1), 2 (r & 1) + 1 of h[r / 2]
-__device__ __forceinline__ void bias_gelu_pack(const floatx4 (&acc)[16], const float* __restrict__ bias,
-                                               uint4 (&h)[8]) {
+// bias + GELU of 8 unit blocks (128 units) of a column block, packed as the
+// next layer's B operand (bias points at the lane's first unit, 4 * (lane /
+// 16)): units 4 of block r go to words 2 (r & 1), 2 (r & 1) + 1 of h[r / 2]
+__device__ __forceinline__ void bias_gelu_pack8(const floatx4 (&acc)[8], const float* __restrict__ bias,
+                                                uint4 (&h)[4]) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
+    for (int r = 0; r < 8; ++r) {
         const float4 b = *(const float4*)(bias + 16 * r);
         const uint32_t lo = pack_bf16x2(gelu_fast(acc[r][0] + b.x), gelu_fast(acc[r][1] + b.y));
         const uint32_t hi = pack_bf16x2(gelu_fast(acc[r][2] + b.z), gelu_fast(acc[r][3] + b.w));
@@ -607,18 +607,15 @@ __global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const uint32_t* __re
             oxy[c] = pack_bf16x2((float)(int32_t)(x - (ic[c] << s)) * inv, (float)(int32_t)(y - (jc[c] << s)) * inv);
             oz0[c] = pack_bf16x2((float)(int32_t)(z - (kc[c] << s)) * inv, 0.0f);
         }
-        floatx4 acc[INF_CB][16];
+        // every layer in two halves of 128 output units, so that the live
+        // accumulators are half a layer: the features of all four k-steps are
+        // gathered first (the operand of lane group g at k-step ks is features
+        // 32 ks + 8 g .. + 7: stencil nodes 8 ks + 2 g and + 1, 4 features
+        // each; at ks = 3 group 1 holds node 26 and the offsets, groups 2, 3
+        // zeros), layer 3 sums over the halves of layer 2 as they come
+        bf16x8 xf[INF_CB][KIN / 32];
 #pragma unroll
-        for (int c = 0; c < INF_CB; ++c)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
-        // layer 1: K = 128; each weight fragment serves every column block.  The
-        // operand of lane group g at k-step ks is features 32 ks + 8 g .. + 7:
-        // stencil nodes 8 ks + 2 g and + 1 (4 features each) from the node grid;
-        // at ks = 3, group 1 holds node 26 and the offsets, groups 2, 3 zeros.
-#pragma unroll
-        for (int ks = 0; ks < KIN / 32; ++ks) {
-            bf16x8 x[INF_CB];
+        for (int ks = 0; ks < KIN / 32; ++ks)
 #pragma unroll
             for (int c = 0; c < INF_CB; ++c) {
                 const int q = 8 * ks + 2 * g;
@@ -631,49 +628,70 @@ __global__ __launch_bounds__(INF_WAVES * 64) void mlp_infer(const uint32_t* __re
                     const uint2 n0 = node[stencil_node(26, ic[c], jc[c], kc[c], gbits, mask)];
                     v = make_uint4(n0.x, n0.y, oxy[c], oz0[c]);
                 }
-                x[c] = __builtin_bit_cast(bf16x8, v);
+                xf[c][ks] = __builtin_bit_cast(bf16x8, v);
             }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const bf16x8 a = load8(w1 + 16 * r * KIN + 32 * ks);
-#pragma unroll
-                for (int c = 0; c < INF_CB; ++c)
-                    acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c], acc[c][r], 0, 0, 0);
-            }
-            __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight: no spills
-        }
+        __builtin_amdgcn_sched_barrier(0);
         uint4 h[INF_CB][8];
 #pragma unroll
-        for (int c = 0; c < INF_CB; ++c) bias_gelu_pack(acc[c], bb1, h[c]);
-        // layer 2: K = 256, weights from LDS
+        for (int hf = 0; hf < 2; ++hf) {
+            floatx4 acc[INF_CB][8];
 #pragma unroll
-        for (int c = 0; c < INF_CB; ++c)
+            for (int c = 0; c < INF_CB; ++c)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+                for (int r = 0; r < 8; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < HID / 32; ++ks) {
+            for (int ks = 0; ks < KIN / 32; ++ks) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const bf16x8 a = __builtin_bit_cast(bf16x8, sW2[(16 * r + l16) * (HID / 8) + ((4 * ks + g) ^ l16)]);
+                for (int r = 0; r < 8; ++r) {
+                    const bf16x8 a = load8(w1 + 16 * (8 * hf + r) * KIN + 32 * ks);
 #pragma unroll
-                for (int c = 0; c < INF_CB; ++c)
-                    acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][ks]),
-                                                                        acc[c][r], 0, 0, 0);
+                    for (int c = 0; c < INF_CB; ++c)
+                        acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xf[c][ks], acc[c][r], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int c = 0; c < INF_CB; ++c) {
+                uint4 hh[4];
+                bias_gelu_pack8(acc[c], bb1 + 128 * hf, hh);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) h[c][4 * hf + k] = hh[k];
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-#pragma unroll
-        for (int c = 0; c < INF_CB; ++c) bias_gelu_pack(acc[c], bb2, h[c]);
-        // layer 3: outputs 0..15 (3 used)
         floatx4 y[INF_CB];
 #pragma unroll
         for (int c = 0; c < INF_CB; ++c) y[c] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < HID / 32; ++ks) {
-            const bf16x8 a = load8(w3 + 32 * ks);
+        for (int hf = 0; hf < 2; ++hf) {
+            floatx4 acc[INF_CB][8];
 #pragma unroll
             for (int c = 0; c < INF_CB; ++c)
-                y[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][ks]), y[c], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < HID / 32; ++ks) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const int row = 16 * (8 * hf + r) + l16;
+                    const bf16x8 a = __builtin_bit_cast(bf16x8, sW2[row * (HID / 8) + ((4 * ks + g) ^ l16)]);
+#pragma unroll
+                    for (int c = 0; c < INF_CB; ++c)
+                        acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, h[c][ks]),
+                                                                            acc[c][r], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int c = 0; c < INF_CB; ++c) {
+                uint4 h2[4];
+                bias_gelu_pack8(acc[c], bb2 + 128 * hf, h2);
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks)
+                    y[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load8(w3 + 32 * (4 * hf + ks)),
+                                                                   __builtin_bit_cast(bf16x8, h2[ks]), y[c], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
         // lanes of group 0 hold outputs 0..3 of their particles
 #pragma unroll
