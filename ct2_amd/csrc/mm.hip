@@ -124,10 +124,10 @@ struct MMState {
     size_t Ps_bytes = 0;
     int* start = nullptr;  // cell starts (nc^3 + 1)
     size_t start_bytes = 0;
-    int* items = nullptr;  // near-field work items (cell, first, count) and their count
+    int* items = nullptr;  // near-field work items (row of cells, first sink, sinks) and their count
     size_t items_bytes = 0;
     int* nitems = nullptr;
-    int* cnt = nullptr;  // items per cell, then their exclusive sum
+    int* cnt = nullptr;  // items per row, then the rows' first item (exclusive sum)
     size_t cnt_bytes = 0;
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
@@ -548,8 +548,8 @@ constexpr int MM_STAGE = 256;  // sources per wave staged in LDS at a time
 #endif
 
 // One wave per work item (up to 64 consecutive sinks of a row of cells,
-// spanning fewer than zspan z cells), items taken in turn by the waves of a
-// persistent grid.  For each (dx, dy) column of the near set (half-height w
+// spanning fewer than zspan z cells); a wave takes items in turn, about one
+// when the grid covers the item count.  For each (dx, dy) column of the near set (half-height w
 // in z) the sources of the column's cells z_first - w .. z_last + w (the
 // window) are staged in the wave's LDS slot in window order, up to MM_STAGE
 // at a time.  FP32 path (nc >= 16, where the window is shorter than the
@@ -747,7 +747,10 @@ int mm_forces() {
     kc.hinv3u = (float)(1.0 / (hu * hu * hu));
     const double fnear = gm * two32 * two32, pnear = gm * two32;
     const double ffar = gm * (double)nc * nc, pfar = gm * (double)nc;
-    const unsigned grid = 2048;
+    // about one work item per wave (max_items bounds the count the device
+    // computes; waves past it exit at once): measured faster than 2,048 to
+    // 8,192 persistent workgroups taking items in turn (unequal items)
+    const unsigned grid = (unsigned)std::min<size_t>(65535, (max_items + MM_WAVES - 1) / MM_WAVES);
     const bool plummer = c.opts.kernel == CT2_KERNEL_PLUMMER, exact = nc < 16;
 #define CT2_MM_NEAR(KER, EX)                                                                                    \
     mm_near_l2p<KER, EX><<<grid, 64 * MM_WAVES, 0, st>>>(s.Ps, s.start, s.items, s.nitems, s.Zl, lg, s.P, s.K, \
