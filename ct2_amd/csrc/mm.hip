@@ -113,6 +113,8 @@ struct MMState {
     int slen = 0, ncols = 0;
     double* Dt = nullptr;  // [K][nc^3]
     size_t Dt_bytes = 0;
+    float* Dt32 = nullptr;  // the same rounded to FP32, the rows MG2L reads
+    size_t Dt32_bytes = 0;
     float2* Zm = nullptr;  // multipole grids [nz][nc^3], then their transforms
     float2* Zl = nullptr;  // local-expansion grids
     size_t Z_bytes = 0;
@@ -234,9 +236,10 @@ void short_samples(const MMIndex& I, int nc, std::vector<double>& S) {
 }
 
 // Dt[c][q] = (parity part of the transformed short samples + the analytic
-// long-range transform) / nc^3 for one component c of exponents (a, b, d).
+// long-range transform) / nc^3 for one component c of exponents (a, b, d);
+// Dt32 the same rounded to FP32.
 __global__ __launch_bounds__(256) void mm_table_finish(const double2* __restrict__ Z, int nc, int a, int b, int d,
-                                                       double* __restrict__ Dt) {
+                                                       double* __restrict__ Dt, float* __restrict__ Dt32) {
     const size_t n3 = (size_t)nc * nc * nc;
     const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (q >= n3) return;
@@ -261,6 +264,7 @@ __global__ __launch_bounds__(256) void mm_table_finish(const double2* __restrict
     double v = (deg & 1) ? Z[q].y + s * lr : Z[q].x + s * lr;
     if (deg == 0 && q == 0) v -= M_PI / b2;
     Dt[q] = v / (double)n3;
+    Dt32[q] = (float)(v / (double)n3);
 }
 
 // ------------------------------------------------------------ binning
@@ -432,7 +436,7 @@ __global__ __launch_bounds__(256) void mm_p2m(const uint4* __restrict__ P, const
 // those with real D^_n into (ar, ai), those with imaginary D^_n (as M^ times
 // the real coefficient) into (br, bi); a component is (ar - bi) + i (ai + br).
 static_assert(MM_QB * sizeof(float2) == 128 && MM_QB * sizeof(float) == 64, "LDS rows of 128 and 64 bytes");
-__global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, const double* __restrict__ Dt,
+__global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, const float* __restrict__ Dt32,
                                                const uint32_t* __restrict__ sched, int nc, int K, int nz, int slen,
                                                float2* __restrict__ Zl) {
     __shared__ float2 Mh[MM_KMAX][MM_QB];
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, co
     }
     for (int e = t; e < K * MM_QB; e += 256) {
         const int c = e / MM_QB, qq = e % MM_QB;
-        Dq[c][qq] = (float)Dt[(size_t)c * n3 + q0 + qq];
+        Dq[c][qq] = Dt32[(size_t)c * n3 + q0 + qq];
     }
     if (t < MM_QB) Dq[K][t] = 0.0f;  // the pads' row
     __syncthreads();
@@ -735,7 +739,7 @@ int mm_forces() {
     mm_p2m<<<(unsigned)(n3 / MM_P2M_CELLS), 256, 0, st>>>(s.Ps, s.start, lg, s.P, s.K, (size_t)n3, s.Zm);
     CT2_LAUNCH_CHECK();
     CT2_TRY(fft3d_batch(s.Zm, nc, 1, false, s.nz));
-    mm_mg2l<<<(unsigned)(n3 / MM_QB), 256, 0, st>>>(s.Zm, s.Dt, s.sched, nc, s.K, s.nz, s.slen, s.Zl);
+    mm_mg2l<<<(unsigned)(n3 / MM_QB), 256, 0, st>>>(s.Zm, s.Dt32, s.sched, nc, s.K, s.nz, s.slen, s.Zl);
     CT2_LAUNCH_CHECK();
     CT2_TRY(fft3d_batch(s.Zl, nc, -1, false, s.nz));
     // near pairs + L2P
@@ -773,7 +777,7 @@ int mm_forces() {
 
 void release_mm() {
     MMState& s = g_mm;
-    for (void* p : {(void*)s.Dt, (void*)s.Zm, (void*)s.Zl, (void*)s.keys, s.sort_tmp, (void*)s.Ps, (void*)s.start,
+    for (void* p : {(void*)s.Dt, (void*)s.Dt32, (void*)s.Zm, (void*)s.Zl, (void*)s.keys, s.sort_tmp, (void*)s.Ps, (void*)s.start,
                     (void*)s.items, (void*)s.cnt, s.scan_tmp, (void*)s.tabz, (void*)s.sched})
         dev_free(p);
     s = MMState();
@@ -859,6 +863,7 @@ extern "C" int ct2_mm_init(int nc, int order) {
     CT2_TRY(ensure_scratch((void**)&s.sched, &s.sched_bytes, sizeof(uint32_t) * sched.size(), "MG2L lists"));
     CT2_HIP(hipMemcpyAsync(s.sched, sched.data(), sizeof(uint32_t) * sched.size(), hipMemcpyHostToDevice, st));
     CT2_TRY(ensure_scratch((void**)&s.Dt, &s.Dt_bytes, dt_bytes, "multipole-mesh derivative table"));
+    CT2_TRY(ensure_scratch((void**)&s.Dt32, &s.Dt32_bytes, dt_bytes / 2, "multipole-mesh FP32 table"));
     CT2_TRY(ensure_scratch((void**)&s.Zm, &s.Z_bytes, sizeof(float2) * n3 * nz, "multipole grids"));
     if (s.Zl) dev_free(s.Zl);
     s.Zl = nullptr;
@@ -876,7 +881,7 @@ extern "C" int ct2_mm_init(int nc, int order) {
         CT2_HIP(hipMemcpyAsync(s.tabz, z.data(), sizeof(double2) * n3, hipMemcpyHostToDevice, st));
         CT2_TRY(fft3d(s.tabz, nc, 1, true));
         mm_table_finish<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(s.tabz, nc, I.e[k][0], I.e[k][1], I.e[k][2],
-                                                                       s.Dt + (size_t)k * n3);
+                                                                       s.Dt + (size_t)k * n3, s.Dt32 + (size_t)k * n3);
         CT2_LAUNCH_CHECK();
         CT2_HIP(hipStreamSynchronize(st));  // z is refilled next
     }
