@@ -362,7 +362,7 @@ constexpr int MM_P2M_CELLS = 32;
 __global__ __launch_bounds__(256) void mm_p2m(const uint4* __restrict__ P, const int* __restrict__ start, int lg, int Pord,
                                               int K, size_t n3, float2* __restrict__ Zm) {
     __shared__ float pw[4][64][3][MM_PMAX + 1];
-    __shared__ float Mo[MM_KMAX + 1][MM_P2M_CELLS];
+    __shared__ float Mo[MM_KMAX + 1][MM_P2M_CELLS + 1];  // + 1: a cell's column spreads over the banks
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const size_t cell0 = (size_t)blockIdx.x * MM_P2M_CELLS;
     const int sh = 32 - lg;
