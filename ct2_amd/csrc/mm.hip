@@ -465,8 +465,10 @@ __global__ __launch_bounds__(256) void mm_mg2l(const float2* __restrict__ Zm, co
     const char* mb = (const char*)&Mh[0][qq];
     const char* db = (const char*)&Dq[0][qq];
     double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0, er = 0.0, ei = 0.0;
+    uint4 gn = *(const uint4*)L;  // the next group's entries, read one group ahead
     for (int i = 0; i < slen; i += 4) {
-        const uint4 g = *(const uint4*)(L + i);
+        const uint4 g = gn;
+        if (i + 4 < slen) gn = *(const uint4*)(L + i + 4);
         const uint32_t w[4] = {g.x, g.y, g.z, g.w};
         float2 mv[4];
         float d[4];
