@@ -55,23 +55,20 @@ __host__ __device__ inline float bf2f(bf16_t h) {
     return f;
 }
 
-__device__ __forceinline__ float gelu(float z) {
-    const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
-    return 0.5f * z * (1.0f + t);
-}
-// The same GELU for the fused inference kernel, branch-free and without the
-// library tanh or a division: 0.5 z (1 + tanh u) = z / (1 + e^{-2u}), with
-// u = c (z + 0.044715 z^3), from the hardware exp2 and reciprocal (each within
-// about 1 ulp); within ~1e-6 of gelu, far below the bf16 rounding that follows.
-__device__ __forceinline__ float gelu_fast(float z) {
+// GELU, tanh form, branch-free and without the library tanh or a division:
+// 0.5 z (1 + tanh u) = z s, s = 1 / (1 + e^{-2u}), u = c (z + 0.044715 z^3),
+// from the hardware exp2 and reciprocal (each within about 1 ulp); within
+// ~1e-6 of the tanh expression, far below the bf16 rounding that follows.
+__device__ __forceinline__ float gelu_sig(float z) {
     constexpr float K1 = -2.0f * GELU_C * 1.4426950408889634f;  // -2 c log2(e)
     constexpr float K3 = K1 * 0.044715f;
-    const float e = __builtin_amdgcn_exp2f(z * fmaf(z * z, K3, K1));
-    return z * __builtin_amdgcn_rcpf(1.0f + e);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(z * z, K3, K1)));
 }
+__device__ __forceinline__ float gelu_fast(float z) { return z * gelu_sig(z); }
+// its derivative: s + z s (1 - s) 2 u' (1 - s, not e s: finite where e^{-2u} overflows)
 __device__ __forceinline__ float gelu_d(float z) {
-    const float t = tanhf(GELU_C * (z + 0.044715f * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * GELU_C * (1.0f + 3.0f * 0.044715f * z * z);
+    const float s = gelu_sig(z);
+    return fmaf(z * s * (1.0f - s), 2.0f * GELU_C * fmaf(3.0f * 0.044715f, z * z, 1.0f), s);
 }
 
 // ------------------------------------------------------------------ GEMM
@@ -173,7 +170,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
                 } else if constexpr (EPI == EPI_BIAS_GELU) {
                     const float z = v + ep.bias[col];
                     if (ep.f32) ep.f32[o] = z;
-                    const bf16_t hv = f2bf(gelu(z));
+                    const bf16_t hv = f2bf(gelu_fast(z));
                     ep.h[o] = hv;
                     tt[wave][col - tn][row - tm] = hv;
                 } else {  // EPI_DGELU
