@@ -86,6 +86,7 @@ _SIGS = {
     "ct2_kick": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_kick_noalloc": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_mm_init": ([C.c_int, C.c_int], C.c_int),
+    "ct2_mm_cells_for": ([C.c_size_t, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "ct2_mm_forces": ([], C.c_int),
     "ct2_kick_mm": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_mm_table_download": ([_P], C.c_int),

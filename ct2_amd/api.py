@@ -252,12 +252,11 @@ def mm_ncomp(order: int) -> int:
 
 def mm_cells_for(n: int, per_cell: int = 8) -> int:
     """The power-of-two cell grid with about `per_cell` (at least) particles per
-    cell, at least 8 cells a side; at 8 per cell the near-set pairs and the
-    far field cost about the same."""
-    nc = 8
-    while (2 * nc) ** 3 * per_cell <= n and nc < 256:
-        nc *= 2
-    return nc
+    cell, at least 8 cells a side (ct2_mm_cells_for); at 8 per cell the
+    near-set pairs and the far field cost about the same."""
+    nc = C.c_int(0)
+    L.check(L.load().ct2_mm_cells_for(n, per_cell, C.byref(nc)))
+    return nc.value
 
 
 def mm_init(nc: int, order: int) -> None:

@@ -91,6 +91,7 @@ void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells
 
 void ct2_gpu_use_multipoles(int ncell, int order) {
     ensure_init();
+    if (ncell < 0) check(ct2_mm_cells_for(g_count, 8, &ncell), "ct2_mm_cells_for");
     if (ncell > 0) check(ct2_mm_init(ncell, order), "ct2_mm_init");
     g_mm_cells = ncell;
 }

@@ -85,7 +85,8 @@ void ct2_gpu_use_learned(const float* weights, size_t n, int ng, double rs_cells
 // ... or to ct2's own structure (kick.cpp:105-125, particle.cpp:116-271):
 // direct pairs over the cells with |offset|^2 <= 10 and the multipole mesh of
 // the given order (<= 8, ct2's PORDER) on ncell^3 cells, ncell a power of two
-// (about 8 particles per cell is fastest); ncell = 0 switches back to the
+// (about 8 particles per cell is fastest; ncell < 0 picks that grid for the
+// allocated particles, ct2_mm_cells_for); ncell = 0 switches back to the
 // direct sum.
 void ct2_gpu_use_multipoles(int ncell, int order);
 

@@ -896,6 +896,15 @@ extern "C" int ct2_mm_init(int nc, int order) {
     return CT2_OK;
 }
 
+extern "C" int ct2_mm_cells_for(size_t n, int particles_per_cell, int* nc) {
+    if (!nc) return set_error(CT2_ERR_INVALID, "null output");
+    if (particles_per_cell < 1) return set_error(CT2_ERR_INVALID, "particles per cell %d < 1", particles_per_cell);
+    size_t g = 8;
+    while (g < 256 && (2 * g) * (2 * g) * (2 * g) * (size_t)particles_per_cell <= n) g *= 2;
+    *nc = (int)g;
+    return CT2_OK;
+}
+
 extern "C" int ct2_mm_forces(void) {
     CT2_TRY(mm_require());
     CT2_TRY(mm_forces());

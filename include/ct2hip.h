@@ -314,6 +314,12 @@ int ct2_drift(float dt, float scale);
  * (once per nc and order; FP64, K nc^3 values for K = (order+1)(order+2)
  * (order+3)/6 components, at most 8 GiB). */
 int ct2_mm_init(int nc, int order);
+/* No counterpart in the reference (its cell grid follows ppcell in
+ * particle_to_grid, particle.cpp:59-114): *nc = the power-of-two cell grid in
+ * [8, 256] with at least `particles_per_cell` (>= 1) particles per cell on
+ * average for n particles, the largest such; 8 per cell balances the near-set
+ * pairs against the far field on this GPU. */
+int ct2_mm_cells_for(size_t n, int particles_per_cell, int* nc);
 /* Force and potential of every particle into the stored force arrays. */
 int ct2_mm_forces(void);
 /* kick() with the multipole-mesh force: ct2_mm_forces then the epilogue of

@@ -34,3 +34,17 @@ def test_option_validation_is_host_side():
     for o in bad:
         assert lib.ct2_set_force_options(C.byref(o)) == L.ERR_INVALID
     assert lib.ct2_set_force_options(C.byref(L.ForceOptions(0.02, 1.0, 1))) == L.OK
+
+
+def test_mm_cells_for_is_host_side():
+    """The multipole mesh's cell grid: the largest power of two in [8, 256]
+    with at least the given particles per cell on average."""
+    from ct2_amd import api
+    assert api.mm_cells_for(2 ** 21) == 64 and api.mm_cells_for(2 ** 24) == 128
+    assert api.mm_cells_for(2 ** 21, 64) == 32 and api.mm_cells_for(2 ** 21, 1) == 128
+    assert api.mm_cells_for(100) == 8 and api.mm_cells_for(0) == 8
+    assert api.mm_cells_for(2 ** 31) == 256
+    assert api.mm_cells_for(8 * 64 ** 3 - 1) == 32  # one short of 8 per cell on 64^3
+    nc = C.c_int(0)
+    assert L.load().ct2_mm_cells_for(1000, 0, C.byref(nc)) == L.ERR_INVALID
+    assert L.load().ct2_mm_cells_for(1000, 8, None) == L.ERR_INVALID
