@@ -549,6 +549,9 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 constexpr int MM_STAGE = 256;  // sources per wave staged in LDS at a time
+#ifndef MM_L2P_CHUNK
+#define MM_L2P_CHUNK 4  // local-expansion grids read ahead at a time by L2P
+#endif
 #ifndef MM_NEAR_UNROLL
 #define MM_NEAR_UNROLL 4  // sources per step of a lane's run (LDS reads in flight)
 #endif
@@ -675,14 +678,28 @@ __global__ __launch_bounds__(64 * MM_WAVES) void mm_near_l2p(const uint4* __rest
             pc[e] = pc[e - 1] * az;
         }
         float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-        for (int c = 0; c < K; ++c) {
-            const float2 z = Zl[(size_t)(c >> 1) * n3 + cell];
-            const float L = ((c & 1) ? z.y : z.x) * c_ifact[c];
-            const int a = c_e[c][0], b = c_e[c][1], d = c_e[c][2];
-            phi = fmaf(L, pa[a] * pb[b] * pc[d], phi);
-            if (a) gx = fmaf(L * (float)a, pa[a - 1] * pb[b] * pc[d], gx);
-            if (b) gy = fmaf(L * (float)b, pa[a] * pb[b - 1] * pc[d], gy);
-            if (d) gz = fmaf(L * (float)d, pa[a] * pb[b] * pc[d - 1], gz);
+        // the component pairs (grids) read MM_L2P_CHUNK at a time, all in flight
+        // together, instead of one load per component on the critical path
+        const int nzg = (K + 1) >> 1;
+        for (int j0 = 0; j0 < nzg; j0 += MM_L2P_CHUNK) {
+            float2 zc[MM_L2P_CHUNK];
+#pragma unroll
+            for (int j = 0; j < MM_L2P_CHUNK; ++j)
+                if (j0 + j < nzg) zc[j] = Zl[(size_t)(j0 + j) * n3 + cell];
+#pragma unroll
+            for (int j = 0; j < MM_L2P_CHUNK; ++j) {
+#pragma unroll
+                for (int hb = 0; hb < 2; ++hb) {
+                    const int c = 2 * (j0 + j) + hb;
+                    if (c >= K) continue;
+                    const float L = (hb ? zc[j].y : zc[j].x) * c_ifact[c];
+                    const int a = c_e[c][0], b = c_e[c][1], d = c_e[c][2];
+                    phi = fmaf(L, pa[a] * pb[b] * pc[d], phi);
+                    if (a) gx = fmaf(L * (float)a, pa[a - 1] * pb[b] * pc[d], gx);
+                    if (b) gy = fmaf(L * (float)b, pa[a] * pb[b - 1] * pc[d], gy);
+                    if (d) gz = fmaf(L * (float)d, pa[a] * pb[b] * pc[d - 1], gz);
+                }
+            }
         }
         if (live) {
             const uint32_t i = me.w;
