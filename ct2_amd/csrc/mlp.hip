@@ -79,7 +79,7 @@ __device__ __forceinline__ float gelu_d(float z) {
 enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_DGELU = 3 };
 
 struct EpiArgs {
-    float* f32;      // fp32 output [M][ldc] (STORE, BIAS: result; BIAS_GELU: z; DGELU: dz, optional)
+    float* f32;      // fp32 output [M][ldc] (STORE, BIAS: result; BIAS_GELU: z, optional; DGELU: unused)
     int ldc;
     bf16_t* h;       // bf16 output [M][ldc] (BIAS_GELU: gelu(z); DGELU: dz)
     bf16_t* ht;      // bf16 transposed output [N][ldt] (optional)
@@ -153,6 +153,64 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
     }
     // lane holds C[tm + 16 i + 4 (lane/16) + j][tn + 16 jj + lane%16]
     float csum[2] = {0.0f, 0.0f};  // DGELU: this lane's sums of dz per column block jj
+    if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
+        // the wave's 32 x 32 tile through LDS, so that the FP32 and bf16 row
+        // tiles leave (and DGELU's saved pre-activations arrive) as whole
+        // 128- and 64-byte rows, 16 bytes per lane and instruction
+        __shared__ __align__(16) float tz[4][32][36];
+        __shared__ __align__(16) bf16_t th[4][32][40];
+        if constexpr (EPI == EPI_DGELU) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = q * 64 + lane, r = e >> 3, part = e & 7;
+                *(float4*)&tz[wave][r][4 * part] = *(const float4*)(ep.zin + (size_t)(tm + r) * ep.ldc + tn + 4 * part);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                const int c = 16 * jj + r16;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int r = 16 * i + 4 * (lane >> 4) + j;
+                    const float v = acc[i][jj][j];
+                    if constexpr (EPI == EPI_BIAS_GELU) {
+                        const float z = v + ep.bias[tn + c];
+                        tz[wave][r][c] = z;
+                        const bf16_t hv = f2bf(gelu_fast(z));
+                        th[wave][r][c] = hv;
+                        tt[wave][c][r] = hv;
+                    } else {
+                        const float d = v * gelu_d(tz[wave][r][c]);
+                        csum[jj] += d;
+                        const bf16_t hv = f2bf(d);
+                        th[wave][r][c] = hv;
+                        tt[wave][c][r] = hv;
+                    }
+                }
+            }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (EPI == EPI_BIAS_GELU && ep.f32) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = q * 64 + lane, r = e >> 3, part = e & 7;
+                *(float4*)(ep.f32 + (size_t)(tm + r) * ep.ldc + tn + 4 * part) = *(const float4*)&tz[wave][r][4 * part];
+            }
+        }
+        if (ep.h) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int e = q * 64 + lane, r = e >> 2, part = e & 3;
+                *(uint4*)(ep.h + (size_t)(tm + r) * ep.ldc + tn + 8 * part) = *(const uint4*)&th[wave][r][8 * part];
+            }
+        }
+    } else {  // STORE, BIAS: 64-byte row pieces straight from the accumulators
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -163,26 +221,13 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
                 const int row = tm + 16 * i + 4 * (lane >> 4) + j;
                 const size_t o = (size_t)row * ep.ldc + col;
                 float v = acc[i][jj][j];
-                if constexpr (EPI == EPI_STORE) {
+                if constexpr (EPI == EPI_STORE)
                     ep.f32[o] = v;
-                } else if constexpr (EPI == EPI_BIAS) {
+                else
                     ep.f32[o] = v + ep.bias[col];
-                } else if constexpr (EPI == EPI_BIAS_GELU) {
-                    const float z = v + ep.bias[col];
-                    if (ep.f32) ep.f32[o] = z;
-                    const bf16_t hv = f2bf(gelu_fast(z));
-                    ep.h[o] = hv;
-                    tt[wave][col - tn][row - tm] = hv;
-                } else {  // EPI_DGELU
-                    const float d = v * gelu_d(ep.zin[o]);
-                    if (ep.f32) ep.f32[o] = d;
-                    csum[jj] += d;
-                    const bf16_t hv = f2bf(d);
-                    if (ep.h) ep.h[o] = hv;
-                    tt[wave][col - tn][row - tm] = hv;
-                }
             }
         }
+    }
     if constexpr (EPI == EPI_DGELU) {
         // the bias gradient's share of this block: column sums over its 128
         // rows (lane groups, then the four waves in order)
