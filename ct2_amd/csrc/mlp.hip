@@ -24,7 +24,8 @@ namespace ct2 {
 namespace {
 
 constexpr int KIN = 128, HID = 256, KOUT = 32, NOUT = 3, NFEAT = 111;
-constexpr int SPLITK_MAX = 64;  // K slices of the weight-gradient products at most
+constexpr int SPLITK_MAX = 256;  // K slices of the weight-gradient products at most
+constexpr int LOSS_CHUNK = 4096;  // training steps whose losses are copied to the host at once
 constexpr size_t OFF_W1 = 0, OFF_B1 = OFF_W1 + KIN * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
                  OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * KOUT, NPARAM = OFF_B3 + KOUT;
 constexpr float GELU_C = 0.7978845608028654f;
@@ -65,9 +66,9 @@ __device__ __forceinline__ float gelu_sig(float z) {
     return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(z * z, K3, K1)));
 }
 __device__ __forceinline__ float gelu_fast(float z) { return z * gelu_sig(z); }
-// its derivative: s + z s (1 - s) 2 u' (1 - s, not e s: finite where e^{-2u} overflows)
-__device__ __forceinline__ float gelu_d(float z) {
-    const float s = gelu_sig(z);
+// its derivative from s = gelu_sig(z): s + z s (1 - s) 2 u' (1 - s, not e s:
+// finite where e^{-2u} overflows)
+__device__ __forceinline__ float gelu_d(float z, float s) {
     return fmaf(z * s * (1.0f - s), 2.0f * GELU_C * fmaf(3.0f * 0.044715f, z * z, 1.0f), s);
 }
 
@@ -79,14 +80,15 @@ __device__ __forceinline__ float gelu_d(float z) {
 enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_DGELU = 3 };
 
 struct EpiArgs {
-    float* f32;      // fp32 output [M][ldc] (STORE, BIAS: result; BIAS_GELU: z, optional; DGELU: unused)
+    float* f32;      // fp32 output [M][ldc] (STORE, BIAS: result; unused otherwise)
     int ldc;
     bf16_t* h;       // bf16 output [M][ldc] (BIAS_GELU: gelu(z); DGELU: dz)
     bf16_t* ht;      // bf16 transposed output [N][ldt] (optional)
     int ldt;
     const float* bias;  // [N]
-    const float* zin;   // DGELU: saved pre-activation [M][ldc]
+    const bf16_t* gin;  // DGELU: the saved gelu'(z) [M][ldc]
     float* cpart = nullptr;  // DGELU: column sums of dz over each 128-row block [M / 128][N] (optional)
+    bf16_t* gd = nullptr;    // BIAS_GELU: gelu'(z) for the backward pass [M][ldc] (optional)
 };
 
 __device__ __forceinline__ bf16x8 load8(const bf16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
@@ -124,14 +126,104 @@ __device__ __forceinline__ GemmBlock gemm_block(int id, int nrb, int ncb, int S)
     return g;
 }
 
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A wave's LDS for the GELU / DGELU epilogue of one 32 x 32 tile.
+struct TileLds {
+    bf16_t tg[32][40];  // gelu'(z) rows (out of BIAS_GELU, into DGELU)
+    bf16_t th[32][40];  // bf16 rows
+    bf16_t tt[32][40];  // bf16 transposed (80-byte rows: the 16 columns a store instruction spans hit 16 banks)
+};
+
+// The GELU / DGELU epilogue of one wave's 32 x 32 tile at (tm, tn), whose
+// lane holds C[tm + 16 i + 4 (lane/16) + j][tn + 16 jj + lane%16] in
+// acc[i][jj][j].  The tile passes through LDS, so that its bf16 rows leave
+// (and DGELU's saved gelu'(z) arrive) as whole 64-byte rows, 16 bytes per
+// lane and instruction, and the transposed copy as 32 rows of 64 contiguous
+// bytes.  The forward pass keeps gelu'(z) in bf16 for the backward one (the
+// product dz = dh gelu'(z) is rounded to bf16 in any case) rather than z in
+// FP32: half the bytes written and read again.  DGELU adds this lane's dz of
+// column block jj to csum[jj].
+template <int EPI>
+__device__ __forceinline__ void gelu_tile(const EpiArgs& ep, int tm, int tn, const floatx4 (&acc)[2][2], int lane,
+                                          TileLds& L, float* csum) {
+    const int r16 = lane & 15;
+    if constexpr (EPI == EPI_DGELU) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = q * 64 + lane, r = e >> 2, part = e & 3;
+            *(uint4*)&L.tg[r][8 * part] = *(const uint4*)(ep.gin + (size_t)(tm + r) * ep.ldc + tn + 8 * part);
+        }
+        wave_sync();
+    }
+    // rows r, r + 1 of a column in pairs: one v_cvt_pk_bf16_f32 and one 4-byte
+    // write of the transposed tile per pair
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            const int c = 16 * jj + r16;
+#pragma unroll
+            for (int jp = 0; jp < 2; ++jp) {
+                const int r = 16 * i + 4 * (lane >> 4) + 2 * jp;
+                const float v0 = acc[i][jj][2 * jp], v1 = acc[i][jj][2 * jp + 1];
+                uint32_t hw;
+                if constexpr (EPI == EPI_BIAS_GELU) {
+                    const float bz = ep.bias[tn + c];
+                    const float z0 = v0 + bz, z1 = v1 + bz;
+                    const float s0 = gelu_sig(z0), s1 = gelu_sig(z1);
+                    hw = pack_bf16x2(z0 * s0, z1 * s1);
+                    if (ep.gd) {
+                        const uint32_t gw = pack_bf16x2(gelu_d(z0, s0), gelu_d(z1, s1));
+                        L.tg[r][c] = (bf16_t)(gw & 0xffffu);
+                        L.tg[r + 1][c] = (bf16_t)(gw >> 16);
+                    }
+                } else {
+                    const float d0 = v0 * bf2f(L.tg[r][c]), d1 = v1 * bf2f(L.tg[r + 1][c]);
+                    csum[jj] += d0;
+                    csum[jj] += d1;
+                    hw = pack_bf16x2(d0, d1);
+                }
+                L.th[r][c] = (bf16_t)(hw & 0xffffu);
+                L.th[r + 1][c] = (bf16_t)(hw >> 16);
+                *(uint32_t*)&L.tt[c][r] = hw;
+            }
+        }
+    wave_sync();
+    if (EPI == EPI_BIAS_GELU && ep.gd) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = q * 64 + lane, r = e >> 2, part = e & 3;
+            *(uint4*)(ep.gd + (size_t)(tm + r) * ep.ldc + tn + 8 * part) = *(const uint4*)&L.tg[r][8 * part];
+        }
+    }
+    if (ep.h) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = q * 64 + lane, r = e >> 2, part = e & 3;
+            *(uint4*)(ep.h + (size_t)(tm + r) * ep.ldc + tn + 8 * part) = *(const uint4*)&L.th[r][8 * part];
+        }
+    }
+    if (ep.ht) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = q * 64 + lane, c = e >> 2, part = e & 3;
+            *(uint4*)(ep.ht + (size_t)(tn + c) * ep.ldt + tm + 8 * part) = *(const uint4*)&L.tt[c][8 * part];
+        }
+    }
+    wave_sync();  // the next tile may reuse L
+}
+
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt,
                                                  int ldb, int M, int N, int K, int S, EpiArgs ep) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const GemmBlock gb = gemm_block((int)blockIdx.x, M / 128, N / 32, S);
     const int tm = (gb.rb * 4 + wave) * 32, tn = gb.cb * 32;
-    // a wave's output tile transposed, for the epilogues that also write C^T
-    __shared__ __align__(16) bf16_t tt[(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) ? 4 : 1][32][32];
     if (tm >= M) return;
     // split K: slice z of S sums k in [kb, ke) (EPI_STORE only; the slice's
     // partial product goes to plane z of ep.f32)
@@ -152,86 +244,13 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
         acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
     }
     // lane holds C[tm + 16 i + 4 (lane/16) + j][tn + 16 jj + lane%16]
-    float csum[2] = {0.0f, 0.0f};  // DGELU: this lane's sums of dz per column block jj
     if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
-        // the wave's 32 x 32 tile through LDS, so that the FP32 and bf16 row
-        // tiles leave (and DGELU's saved pre-activations arrive) as whole
-        // 128- and 64-byte rows, 16 bytes per lane and instruction
-        __shared__ __align__(16) float tz[4][32][36];
-        __shared__ __align__(16) bf16_t th[4][32][40];
-        if constexpr (EPI == EPI_DGELU) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int e = q * 64 + lane, r = e >> 3, part = e & 7;
-                *(float4*)&tz[wave][r][4 * part] = *(const float4*)(ep.zin + (size_t)(tm + r) * ep.ldc + tn + 4 * part);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int jj = 0; jj < 2; ++jj) {
-                const int c = 16 * jj + r16;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int r = 16 * i + 4 * (lane >> 4) + j;
-                    const float v = acc[i][jj][j];
-                    if constexpr (EPI == EPI_BIAS_GELU) {
-                        const float z = v + ep.bias[tn + c];
-                        tz[wave][r][c] = z;
-                        const bf16_t hv = f2bf(gelu_fast(z));
-                        th[wave][r][c] = hv;
-                        tt[wave][c][r] = hv;
-                    } else {
-                        const float d = v * gelu_d(tz[wave][r][c]);
-                        csum[jj] += d;
-                        const bf16_t hv = f2bf(d);
-                        th[wave][r][c] = hv;
-                        tt[wave][c][r] = hv;
-                    }
-                }
-            }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (EPI == EPI_BIAS_GELU && ep.f32) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int e = q * 64 + lane, r = e >> 3, part = e & 7;
-                *(float4*)(ep.f32 + (size_t)(tm + r) * ep.ldc + tn + 4 * part) = *(const float4*)&tz[wave][r][4 * part];
-            }
-        }
-        if (ep.h) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int e = q * 64 + lane, r = e >> 2, part = e & 3;
-                *(uint4*)(ep.h + (size_t)(tm + r) * ep.ldc + tn + 8 * part) = *(const uint4*)&th[wave][r][8 * part];
-            }
-        }
-    } else {  // STORE, BIAS: 64-byte row pieces straight from the accumulators
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int jj = 0; jj < 2; ++jj) {
-            const int col = tn + 16 * jj + r16;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int row = tm + 16 * i + 4 * (lane >> 4) + j;
-                const size_t o = (size_t)row * ep.ldc + col;
-                float v = acc[i][jj][j];
-                if constexpr (EPI == EPI_STORE)
-                    ep.f32[o] = v;
-                else
-                    ep.f32[o] = v + ep.bias[col];
-            }
-        }
-    }
-    if constexpr (EPI == EPI_DGELU) {
-        // the bias gradient's share of this block: column sums over its 128
-        // rows (lane groups, then the four waves in order)
-        if (ep.cpart) {
+        __shared__ TileLds lds[4];
+        float csum[2] = {0.0f, 0.0f};  // DGELU: this lane's sums of dz per column block jj
+        gelu_tile<EPI>(ep, tm, tn, acc, lane, lds[wave], csum);
+        if (EPI == EPI_DGELU && ep.cpart) {
+            // the bias gradient's share of this block: column sums over its
+            // 128 rows (lane groups, then the four waves in order)
             __shared__ float cs[4][32];
 #pragma unroll
             for (int jj = 0; jj < 2; ++jj) {
@@ -246,39 +265,179 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
                 ep.cpart[(size_t)gb.rb * N + tn + c] = ((cs[0][c] + cs[1][c]) + cs[2][c]) + cs[3][c];
             }
         }
-    }
-    if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
-        // the transposed copy: the wave's 32 x 32 tile through LDS, written as
-        // 32 rows of 64 contiguous bytes (16 bytes per lane and store)
-        if (ep.ht) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {  // STORE, BIAS: 64-byte row pieces straight from the accumulators
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int e = q * 64 + lane, c = e >> 2, part = e & 3;
-                *(uint4*)(ep.ht + (size_t)(tn + c) * ep.ldt + tm + 8 * part) = *(const uint4*)&tt[wave][c][8 * part];
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                const int col = tn + 16 * jj + r16;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = tm + 16 * i + 4 * (lane >> 4) + j;
+                    const size_t o = (size_t)row * ep.ldc + col;
+                    float v = acc[i][jj][j];
+                    if constexpr (EPI == EPI_STORE)
+                        ep.f32[o] = v;
+                    else
+                        ep.f32[o] = v + ep.bias[col];
+                }
+            }
+    }
+}
+
+// The products with M and N multiples of 128 (the hidden layers and their
+// weight gradients): one wave per 64 x 64 output tile (4 x 4 MFMA 16x16x32
+// tiles), 2 x 2 waves per 128 x 128 workgroup tile.  Each operand fragment a
+// wave loads feeds four MFMAs instead of two, which halves the L2 reads per
+// product of the 32 x 32 tiling.  GELU / DGELU epilogues run tile by tile as
+// there; EPI_STORE with S K slices writes slice z's partial product to plane
+// z of ep.f32.
+template <int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gemm_wide(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt,
+                                                 int ldb, int M, int N, int K, int S, EpiArgs ep) {
+    static_assert(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU || EPI == EPI_STORE, "no wide BIAS epilogue");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const GemmBlock gb = gemm_block((int)blockIdx.x, M / 128, N / 128, S);
+    const int wr = wave & 1, wc = wave >> 1;
+    const int tm = gb.rb * 128 + wr * 64, tn = gb.cb * 128 + wc * 64;
+    const int kc = ((K / 32 + S - 1) / S) * 32;
+    const int kb = gb.z * kc, ke = min(K, kb + kc);
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    floatx4 acc[4][4] = {};
+    const bf16_t* a = A + (size_t)(tm + r16) * lda + kq;
+    const bf16_t* b = Bt + (size_t)(tn + r16) * ldb + kq;
+    // Scheduling barriers keep the compiler from sinking each fragment load
+    // to just before its first MFMA, which serialises the memory latency of
+    // the four row fragments.  The long-K split products (STORE) also load
+    // the fragments of step k + 32 while the MFMAs of step k run (the last
+    // step loads its own again: in bounds, unused); the GELU products (K <=
+    // 256) keep the registers for occupancy instead.
+    bf16x8 fa[4], fb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        fa[i] = load8(a + (size_t)16 * i * lda + kb);
+        fb[i] = load8(b + (size_t)16 * i * ldb + kb);
+    }
+    for (int k = kb; k < ke; k += 32) {
+        const int kn = k + 32 < ke ? k + 32 : k;
+        bf16x8 na[4], nb[4];
+        if constexpr (EPI == EPI_STORE) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                na[i] = load8(a + (size_t)16 * i * lda + kn);
+                nb[i] = load8(b + (size_t)16 * i * ldb + kn);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (EPI == EPI_STORE) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                fa[i] = na[i];
+                fb[i] = nb[i];
+            }
+        } else if (k + 32 < ke) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                fa[i] = load8(a + (size_t)16 * i * lda + kn);
+                fb[i] = load8(b + (size_t)16 * i * ldb + kn);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if constexpr (EPI == EPI_STORE) {
+        float* out = ep.f32 + (size_t)gb.z * M * ep.ldc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    out[(size_t)(tm + 16 * i + 4 * (lane >> 4) + j) * ep.ldc + tn + 16 * jj + r16] = acc[i][jj][j];
+        return;
+    } else {
+    __shared__ TileLds lds[4];
+    float csum[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // DGELU: this lane's sums of dz per 16-column block
+#pragma unroll
+    for (int i2 = 0; i2 < 2; ++i2)
+#pragma unroll
+        for (int j2 = 0; j2 < 2; ++j2) {
+            const floatx4 sub[2][2] = {{acc[2 * i2][2 * j2], acc[2 * i2][2 * j2 + 1]},
+                                       {acc[2 * i2 + 1][2 * j2], acc[2 * i2 + 1][2 * j2 + 1]}};
+            gelu_tile<EPI>(ep, tm + 32 * i2, tn + 32 * j2, sub, lane, lds[wave], csum + 2 * j2);
+        }
+    if (EPI == EPI_DGELU && ep.cpart) {
+        // the bias gradient's share of this 128-row block: each wave's column
+        // sums over its 64 rows (lane groups), then the two waves of a column
+        // half in order
+        __shared__ float cs[4][64];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float v = csum[q];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (lane < 16) cs[wave][16 * q + lane] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 128) {
+            const int h = threadIdx.x >> 6, c = threadIdx.x & 63;
+            ep.cpart[(size_t)gb.rb * N + gb.cb * 128 + threadIdx.x] = cs[2 * h][c] + cs[2 * h + 1][c];
+        }
+    }
     }
 }
 
 template <int EPI>
 int gemm(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K, const EpiArgs& ep) {
     if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
+    if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
+        if (N % 128 == 0) {
+            gemm_wide<EPI><<<(M / 128) * (N / 128), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
+            CT2_LAUNCH_CHECK();
+            return CT2_OK;
+        }
+    }
     gemm_bf16<EPI><<<(M / 128) * (N / 32), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
 
-// out[i] = sum over z of part[z][i], z = 0 .. S-1 in order
+// out[i] = sum over z of part[z][i] (len % 4 == 0) in a fixed order: eight
+// thread groups each sum every eighth plane (z = g, g + 8, ...) for 128
+// consecutive entries (a float4 per thread), then group 0 adds the eight
+// partial sums in order
 __global__ __launch_bounds__(256) void splitk_reduce(const float* __restrict__ part, int S, int len,
                                                      float* __restrict__ out) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= len) return;
-    float s = 0.0f;
-    for (int z = 0; z < S; ++z) s += part[(size_t)z * len + i];
-    out[i] = s;
+    __shared__ float4 red[8][32];
+    const int g = threadIdx.x >> 5, e = threadIdx.x & 31;
+    const int i = (blockIdx.x * 32 + e) * 4;
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < len)
+        for (int z = g; z < S; z += 8) {
+            const float4 v = *(const float4*)(part + (size_t)z * len + i);
+            s.x += v.x;
+            s.y += v.y;
+            s.z += v.z;
+            s.w += v.w;
+        }
+    red[g][e] = s;
+    __syncthreads();
+    if (g == 0 && i < len) {
+        float4 t = red[0][e];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) {
+            t.x += red[q][e].x;
+            t.y += red[q][e].y;
+            t.z += red[q][e].z;
+            t.w += red[q][e].w;
+        }
+        *(float4*)(out + i) = t;
+    }
 }
 
 // C = A B with a long K (the weight gradients, K = batch rows): K is cut into S
@@ -288,13 +447,25 @@ __global__ __launch_bounds__(256) void splitk_reduce(const float* __restrict__ p
 int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K,
                 float* scratch, int smax, float* out) {
     if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
+    if (N % 128 == 0) {  // 128 x 128 workgroup tiles, at least 512 workgroups
+        const int tiles = (M / 128) * (N / 128);
+        const int S = std::max(1, std::min({smax, K / 256, (512 + tiles - 1) / tiles}));
+        EpiArgs ep{S > 1 ? scratch : out, N, nullptr, nullptr, 0, nullptr, nullptr};
+        gemm_wide<EPI_STORE><<<tiles * S, 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
+        CT2_LAUNCH_CHECK();
+        if (S > 1) {
+            splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
+            CT2_LAUNCH_CHECK();
+        }
+        return CT2_OK;
+    }
     const int tiles = (M / 128) * (N / 32);
     const int S = std::max(1, std::min({smax, K / 256, (1024 + tiles - 1) / tiles}));
     EpiArgs ep{S > 1 ? scratch : out, N, nullptr, nullptr, 0, nullptr, nullptr};
     gemm_bf16<EPI_STORE><<<(M / 128) * (N / 32) * S, 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
     CT2_LAUNCH_CHECK();
     if (S > 1) {
-        splitk_reduce<<<(M * N + 255) / 256, 256, 0, st>>>(scratch, S, M * N, out);
+        splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
         CT2_LAUNCH_CHECK();
     }
     return CT2_OK;
@@ -303,29 +474,39 @@ int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int 
 // ------------------------------------------------------------------ weights
 // bf16 copies in the layouts the GEMMs read: transposed [out][in] for the
 // forward products, natural [in][out] for the input-gradient products.
-__global__ void pack_weights(const float* __restrict__ w, bf16_t* __restrict__ W1t, bf16_t* __restrict__ W2t,
-                             bf16_t* __restrict__ W3t, bf16_t* __restrict__ W2n, bf16_t* __restrict__ W3n) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t < KIN * HID) {  // W1[k][n] -> W1t[n][k]
-        const int k = t / HID, n = t % HID;
-        W1t[n * KIN + k] = f2bf(w[OFF_W1 + t]);
-    }
-    if (t < HID * HID) {
-        const int k = t / HID, n = t % HID;
-        const bf16_t v = f2bf(w[OFF_W2 + t]);
-        W2t[n * HID + k] = v;
-        W2n[t] = v;
-    }
-    if (t < HID * KOUT) {
-        const int k = t / KOUT, n = t % KOUT;
-        const bf16_t v = f2bf(w[OFF_W3 + t]);
-        W3t[n * HID + k] = v;
-        W3n[t] = v;
+// Parameter i's bf16 copies in the GEMM layouts: W1t [HID][KIN], W2t / W2n
+// (transposed / as stored) and W3t / W3n; biases have none.
+struct PackedW {
+    bf16_t *W1t, *W2t, *W3t, *W2n, *W3n;
+};
+__device__ __forceinline__ void pack_param(int i, float wv, const PackedW& P) {
+    if (i < OFF_W1 + KIN * HID) {  // W1[k][n] -> W1t[n][k]
+        const int t = i - OFF_W1, k = t / HID, n = t % HID;
+        P.W1t[n * KIN + k] = f2bf(wv);
+    } else if (i >= OFF_W2 && i < OFF_W2 + HID * HID) {
+        const int t = i - OFF_W2, k = t / HID, n = t % HID;
+        const bf16_t v = f2bf(wv);
+        P.W2t[n * HID + k] = v;
+        P.W2n[t] = v;
+    } else if (i >= OFF_W3 && i < OFF_W3 + HID * KOUT) {
+        const int t = i - OFF_W3, k = t / KOUT, n = t % KOUT;
+        const bf16_t v = f2bf(wv);
+        P.W3t[n * HID + k] = v;
+        P.W3n[t] = v;
     }
 }
+static_assert(OFF_W1 == 0, "W1 leads the parameter vector");
 
+__global__ void pack_weights(const float* __restrict__ w, int n, PackedW P) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) pack_param(i, w[i], P);
+}
+
+// One Adam step of every parameter; the updated weights' bf16 copies are
+// written at once (no separate packing pass per step).
 __global__ void adam_step(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
-                          const float* __restrict__ g, int n, float lr, float b1, float b2, float c1, float c2, float eps) {
+                          const float* __restrict__ g, int n, float lr, float b1, float b2, float c1, float c2, float eps,
+                          PackedW P) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
@@ -333,7 +514,9 @@ __global__ void adam_step(float* __restrict__ w, float* __restrict__ m, float* _
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float wn = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wn;
+    pack_param(i, wn, P);
 }
 
 // ------------------------------------------------------------------ features
@@ -435,93 +618,127 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 // X[M][128], XT[128][M], T[M][4].
 // Block b draws rows [64 b, 64 b + 64) (M % 64 == 0); XT is written through an
 // LDS transpose, 64 contiguous rows (128 bytes) per feature.
-constexpr int BATCH_ROWS = 64, BATCH_THREADS = 4 * KIN;
+constexpr int BATCH_ROWS = 64, BATCH_THREADS = 256;
+// A training batch: row r takes sample mix64(key + r) mod size.  16 threads
+// per row move its 256 bytes of features as 16-byte pieces (and one the
+// 16-byte target); the block's 64 rows pass through LDS so that XT leaves as
+// 128 rows of 128 contiguous bytes.
 __global__ __launch_bounds__(BATCH_THREADS) void batch_kernel(const bf16_t* __restrict__ DX,
                                                               const float* __restrict__ DT, long size, uint64_t key,
                                                               int M, bf16_t* __restrict__ X, bf16_t* __restrict__ XT,
                                                               float* __restrict__ T) {
-    __shared__ bf16_t tile[KIN][BATCH_ROWS + 2];
-    const int k = threadIdx.x % KIN;  // one feature per lane, four rows at a time
-    const int rb = blockIdx.x * BATCH_ROWS;
-    for (int q = threadIdx.x / KIN; q < BATCH_ROWS; q += BATCH_THREADS / KIN) {
-        const int r = rb + q;
+    static_assert(KIN == 16 * 8 && BATCH_ROWS * KIN == BATCH_THREADS * 32, "16 pieces of 8 features per row");
+    __shared__ __align__(16) bf16_t tile[KIN][BATCH_ROWS + 8];
+    const int piece = threadIdx.x & 15, rb = blockIdx.x * BATCH_ROWS;
+#pragma unroll
+    for (int pass = 0; pass < BATCH_ROWS / 16; ++pass) {
+        const int q = pass * 16 + (threadIdx.x >> 4), r = rb + q;
         const long p = (long)(mix64(key + (uint64_t)r) % (uint64_t)size);
-        const bf16_t v = DX[(size_t)p * KIN + k];
-        X[(size_t)r * KIN + k] = v;
-        tile[k][q] = v;
-        if (k < 4) T[4 * (size_t)r + k] = DT[4 * (size_t)p + k];
+        const uint4 v = *(const uint4*)(DX + (size_t)p * KIN + 8 * piece);
+        *(uint4*)(X + (size_t)r * KIN + 8 * piece) = v;
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            tile[8 * piece + 2 * j][q] = (bf16_t)(w[j] & 0xffffu);
+            tile[8 * piece + 2 * j + 1][q] = (bf16_t)(w[j] >> 16);
+        }
+        if (piece == 0) *(float4*)(T + 4 * (size_t)r) = *(const float4*)(DT + 4 * (size_t)p);
     }
     __syncthreads();
     // feature f of the block's rows: 64 contiguous elements of XT row f
-    for (int e = threadIdx.x; e < KIN * BATCH_ROWS / 2; e += BATCH_THREADS) {
-        const int f = e / (BATCH_ROWS / 2), h = e % (BATCH_ROWS / 2);
-        const uint32_t w = (uint32_t)tile[f][2 * h] | ((uint32_t)tile[f][2 * h + 1] << 16);
-        *(uint32_t*)(XT + (size_t)f * M + rb + 2 * h) = w;
+#pragma unroll
+    for (int e = threadIdx.x; e < KIN * BATCH_ROWS / 8; e += BATCH_THREADS) {
+        const int f = e >> 3, part = e & 7;
+        *(uint4*)(XT + (size_t)f * M + rb + 8 * part) = *(const uint4*)&tile[f][8 * part];
     }
 }
 
-// dY = 2 w (Y - T) / M on the first 3 outputs (w = T[r][3]), 0 elsewhere;
-// per-block partials of sum w |Y - T|^2
-__global__ void loss_kernel(const float* __restrict__ Yo, const float* __restrict__ T, int M, float* __restrict__ dYf,
-                            bf16_t* __restrict__ dY, bf16_t* __restrict__ dYT, float* __restrict__ partial) {
+// dY = 2 w (Y - T) / M on the first 3 outputs (w = T[r][3]), 0 elsewhere,
+// LOSS_ROWS rows per block (M % LOSS_ROWS == 0): dY row-major and dYT
+// through LDS as KOUT rows of LOSS_ROWS contiguous entries; per-block sums
+// of w |Y - T|^2 (partial[b]) and of each column of dY over the block's rows
+// in order (cpart[b][o], the bias gradient's share)
+constexpr int LOSS_ROWS = 64;
+__global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ Yo, const float* __restrict__ T, int M,
+                                                   bf16_t* __restrict__ dY,
+                                                   bf16_t* __restrict__ dYT, float* __restrict__ partial,
+                                                   float* __restrict__ cpart) {
+    __shared__ float td[LOSS_ROWS][KOUT + 1];
+    __shared__ __align__(16) bf16_t tt[KOUT][LOSS_ROWS + 8];
     __shared__ float red[4];
-    const int r = blockIdx.x * 8 + (threadIdx.x >> 5), o = threadIdx.x & 31;
+    const int t = threadIdx.x, o = t & 31, rq = t >> 5;
+    const int rb = blockIdx.x * LOSS_ROWS;
     float e2 = 0.0f;
-    if (r < M) {
+#pragma unroll
+    for (int it = 0; it < LOSS_ROWS / 8; ++it) {
+        const int rl = it * 8 + rq, r = rb + rl;
         float d = 0.0f;
         if (o < NOUT) {
             const float wr = T[4 * (size_t)r + 3];
             const float e = Yo[(size_t)r * KOUT + o] - T[4 * (size_t)r + o];
-            e2 = wr * e * e;
+            e2 += wr * e * e;
             d = 2.0f * wr * e / (float)M;
         }
-        dYf[(size_t)r * KOUT + o] = d;
         const bf16_t h = f2bf(d);
         dY[(size_t)r * KOUT + o] = h;
-        dYT[(size_t)o * M + r] = h;
+        td[rl][o] = d;
+        tt[o][rl] = h;
     }
     for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e2;
+    if ((t & 63) == 0) red[t >> 6] = e2;
     __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (t == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (t < KOUT) {
+        float cs = 0.0f;
+        for (int rl = 0; rl < LOSS_ROWS; ++rl) cs += td[rl][t];
+        cpart[(size_t)blockIdx.x * KOUT + t] = cs;
+    }
+    static_assert(KOUT * LOSS_ROWS == 256 * 8, "one 16-byte piece of dYT per thread");
+    const int o2 = t >> 3, part = t & 7;
+    *(uint4*)(dYT + (size_t)o2 * M + rb + 8 * part) = *(const uint4*)&tt[o2][8 * part];
 }
 
-// db[n] = sum_r D[r][n] in two fixed-order stages (N <= 256): block b sums
-// rows [64 b, 64 b + 64) of each column with four interleaved accumulators
-// (colsum_part); then block n adds column n's block sums, 256 threads and an
-// LDS tree in a fixed pattern (colsum_final).  The result does not depend on
-// scheduling.
-constexpr int COLSUM_ROWS = 64;
-__global__ __launch_bounds__(256) void colsum_part(const float* __restrict__ D, int M, int N,
-                                                   float* __restrict__ part) {
-    const int n = threadIdx.x, b = blockIdx.x;
-    if (n >= N) return;
-    const int r0 = b * COLSUM_ROWS, r1 = min(M, r0 + COLSUM_ROWS);
-    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int r = r0; r < r1; ++r) s[(r - r0) & 3] += D[(size_t)r * N + n];
-    part[(size_t)b * N + n] = (s[0] + s[1]) + (s[2] + s[3]);
-}
-__global__ __launch_bounds__(256) void colsum_final(const float* __restrict__ part, int nb, int N,
-                                                    float* __restrict__ db) {
-    __shared__ float red[256];
-    const int n = blockIdx.x, t = threadIdx.x;
+// Per-block partials of a gradient vector: db[n] = sum over b < nb of
+// part[b][n].
+struct ColSums {
+    const float* part;
+    int nb, N;
+    float* db;
+};
+// The final reductions of the bias gradients and the loss, one launch:
+// block n < a.N sums column n of a, the next b.N blocks (b.N may be 0) the
+// columns of b, each with 256 strided sums and an LDS tree in a fixed
+// pattern; the last block, when lout is given, writes (sum of the lnb loss
+// partials) / M to *lout, in FP64.  The results do not depend on scheduling.
+__global__ __launch_bounds__(256) void colsums(ColSums a, ColSums b, const float* __restrict__ lp, int lnb, int M,
+                                               float* __restrict__ lout) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    int n = blockIdx.x;
+    if (n >= a.N + b.N) {
+        double s = 0.0;
+        for (int q = t; q < lnb; q += 256) s += lp[q];
+        red[t] = s;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (t < h) red[t] += red[t + h];
+            __syncthreads();
+        }
+        if (t == 0) *lout = (float)(red[0] / M);
+        return;
+    }
+    const ColSums c = n < a.N ? a : b;
+    if (n >= a.N) n -= a.N;
     float s = 0.0f;
-    for (int b = t; b < nb; b += 256) s += part[(size_t)b * N + n];
-    red[t] = s;
+    for (int q = t; q < c.nb; q += 256) s += c.part[(size_t)q * c.N + n];
+    float* fr = (float*)red;
+    fr[t] = s;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
-        if (t < h) red[t] += red[t + h];
+        if (t < h) fr[t] += fr[t + h];
         __syncthreads();
     }
-    if (t == 0) db[n] = red[0];
-}
-int colsum(hipStream_t st, const float* D, int M, int N, float* part, float* db) {
-    const int nb = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
-    colsum_part<<<nb, 256, 0, st>>>(D, M, N, part);
-    CT2_LAUNCH_CHECK();
-    colsum_final<<<N, 256, 0, st>>>(part, nb, N, db);
-    CT2_LAUNCH_CHECK();
-    return CT2_OK;
+    if (t == 0) c.db[n] = fr[0];
 }
 
 // F[p] += F_unit * Y[r][0..2] for p = r0 + r
@@ -771,8 +988,10 @@ struct Mlp {
     int cap = 0;
     bf16_t *X = nullptr, *XT = nullptr, *H1 = nullptr, *H1T = nullptr, *H2 = nullptr, *H2T = nullptr;
     bf16_t *dY = nullptr, *dYT = nullptr, *dZ2 = nullptr, *dZ2T = nullptr, *dZ1T = nullptr;
-    float *Z1 = nullptr, *Z2 = nullptr, *Yo = nullptr, *T = nullptr, *dYf = nullptr;
+    bf16_t *G1 = nullptr, *G2 = nullptr;  // gelu'(z) of the hidden layers, kept for the backward pass
+    float *Yo = nullptr, *T = nullptr;
     float* lpart = nullptr;
+    float* lsteps = nullptr;  // per-step losses of a training call, LOSS_CHUNK at a time
     float* gsplit = nullptr;  // split-K planes of the weight-gradient products
     float* cspart = nullptr;  // per-block column sums of the bias gradients
     float* Xf = nullptr;  // fp32 features for host download
@@ -787,14 +1006,15 @@ struct Mlp {
 void free_work() {
     for (void* p : {(void*)g_mlp.X, (void*)g_mlp.XT, (void*)g_mlp.H1, (void*)g_mlp.H1T, (void*)g_mlp.H2, (void*)g_mlp.H2T,
                     (void*)g_mlp.dY, (void*)g_mlp.dYT, (void*)g_mlp.dZ2, (void*)g_mlp.dZ2T, (void*)g_mlp.dZ1T,
-                    (void*)g_mlp.Z1, (void*)g_mlp.Z2, (void*)g_mlp.Yo, (void*)g_mlp.T, (void*)g_mlp.dYf,
-                    (void*)g_mlp.lpart, (void*)g_mlp.gsplit,
+                    (void*)g_mlp.G1, (void*)g_mlp.G2, (void*)g_mlp.Yo, (void*)g_mlp.T,
+                    (void*)g_mlp.lpart, (void*)g_mlp.lsteps, (void*)g_mlp.gsplit,
                     (void*)g_mlp.cspart, (void*)g_mlp.Xf, (void*)g_mlp.idx,
                     (void*)g_mlp.judge})
         dev_free(p);
     g_mlp.X = g_mlp.XT = g_mlp.H1 = g_mlp.H1T = g_mlp.H2 = g_mlp.H2T = nullptr;
     g_mlp.dY = g_mlp.dYT = g_mlp.dZ2 = g_mlp.dZ2T = g_mlp.dZ1T = nullptr;
-    g_mlp.Z1 = g_mlp.Z2 = g_mlp.Yo = g_mlp.T = g_mlp.dYf = g_mlp.lpart = g_mlp.Xf = nullptr;
+    g_mlp.G1 = g_mlp.G2 = nullptr;
+    g_mlp.Yo = g_mlp.T = g_mlp.lpart = g_mlp.lsteps = g_mlp.Xf = nullptr;
     g_mlp.gsplit = g_mlp.cspart = nullptr;
     g_mlp.idx = nullptr;
     g_mlp.judge = nullptr;
@@ -805,7 +1025,7 @@ void free_work() {
 int ensure_work(int rows, bool training) {
     constexpr int ROUND = INF_TILE > 256 ? INF_TILE : 256;  // whole inference tiles and GEMM tiles
     rows = (rows + ROUND - 1) / ROUND * ROUND;
-    const bool have_train = g_mlp.Z1 != nullptr;
+    const bool have_train = g_mlp.G1 != nullptr;
     if (g_mlp.cap >= rows && (!training || have_train)) return CT2_OK;
     const int cap = std::max(rows, g_mlp.cap);
     free_work();
@@ -827,13 +1047,13 @@ int ensure_work(int rows, bool training) {
         ALLOC(g_mlp.dZ2, R * HID * 2, "mlp dZ2");
         ALLOC(g_mlp.dZ2T, R * HID * 2, "mlp dZ2T");
         ALLOC(g_mlp.dZ1T, R * HID * 2, "mlp dZ1T");
-        ALLOC(g_mlp.Z1, R * HID * 4, "mlp Z1");
-        ALLOC(g_mlp.Z2, R * HID * 4, "mlp Z2");
+        ALLOC(g_mlp.G1, R * HID * 2, "mlp gelu' 1");
+        ALLOC(g_mlp.G2, R * HID * 2, "mlp gelu' 2");
         ALLOC(g_mlp.T, R * 4 * 4, "mlp T");
-        ALLOC(g_mlp.dYf, R * KOUT * 4, "mlp dYf");
         ALLOC(g_mlp.lpart, (R / 8 + 1) * 4, "mlp loss");
+        ALLOC(g_mlp.lsteps, LOSS_CHUNK * 4, "mlp step losses");
         ALLOC(g_mlp.gsplit, (size_t)SPLITK_MAX * HID * HID * 4, "mlp split-K planes");
-        ALLOC(g_mlp.cspart, (R / COLSUM_ROWS + 1) * HID * 4, "mlp column sums");
+        ALLOC(g_mlp.cspart, (R / LOSS_ROWS + 1) * HID * 4, "mlp column sums");
         ALLOC(g_mlp.Xf, R * KIN * 4, "mlp X fp32");
         ALLOC(g_mlp.judge, R * 4 * 8, "mlp judge");
     }
@@ -848,8 +1068,10 @@ int require_mlp() {
     return CT2_OK;
 }
 
+PackedW packed() { return PackedW{g_mlp.W1t, g_mlp.W2t, g_mlp.W3t, g_mlp.W2n, g_mlp.W3n}; }
+
 int repack(hipStream_t st) {
-    pack_weights<<<(HID * HID + 255) / 256, 256, 0, st>>>(g_mlp.w, g_mlp.W1t, g_mlp.W2t, g_mlp.W3t, g_mlp.W2n, g_mlp.W3n);
+    pack_weights<<<(int)((NPARAM + 255) / 256), 256, 0, st>>>(g_mlp.w, (int)NPARAM, packed());
     CT2_LAUNCH_CHECK();
     pack_infer<<<(HID * HID + 255) / 256, 256, 0, st>>>(g_mlp.w, g_mlp.W2p, g_mlp.W3p);
     CT2_LAUNCH_CHECK();
@@ -858,9 +1080,11 @@ int repack(hipStream_t st) {
 
 // Forward of M rows already in g_mlp.X; keeps Z/H for training when train.
 int forward(hipStream_t st, int M, bool train) {
-    EpiArgs e1{train ? g_mlp.Z1 : nullptr, HID, g_mlp.H1, train ? g_mlp.H1T : nullptr, M, g_mlp.w + OFF_B1, nullptr};
+    EpiArgs e1{nullptr, HID, g_mlp.H1, train ? g_mlp.H1T : nullptr, M, g_mlp.w + OFF_B1, nullptr, nullptr,
+               train ? g_mlp.G1 : nullptr};
     CT2_TRY(gemm<EPI_BIAS_GELU>(st, g_mlp.X, KIN, g_mlp.W1t, KIN, M, HID, KIN, e1));
-    EpiArgs e2{train ? g_mlp.Z2 : nullptr, HID, g_mlp.H2, train ? g_mlp.H2T : nullptr, M, g_mlp.w + OFF_B2, nullptr};
+    EpiArgs e2{nullptr, HID, g_mlp.H2, train ? g_mlp.H2T : nullptr, M, g_mlp.w + OFF_B2, nullptr, nullptr,
+               train ? g_mlp.G2 : nullptr};
     CT2_TRY(gemm<EPI_BIAS_GELU>(st, g_mlp.H1, HID, g_mlp.W2t, HID, M, HID, HID, e2));
     EpiArgs e3{g_mlp.Yo, KOUT, nullptr, nullptr, 0, g_mlp.w + OFF_B3, nullptr};
     CT2_TRY(gemm<EPI_BIAS>(st, g_mlp.H2, HID, g_mlp.W3t, HID, M, KOUT, HID, e3));
@@ -868,27 +1092,35 @@ int forward(hipStream_t st, int M, bool train) {
 }
 
 // Backward of the MSE loss for M rows with targets in g_mlp.T; gradient
-// into g_mlp.grad, per-block loss partials into g_mlp.lpart.
-int backward(hipStream_t st, int M) {
+// into g_mlp.grad, per-block loss partials into g_mlp.lpart and, when lout
+// is given, the loss itself into *lout (device memory).
+int backward(hipStream_t st, int M, float* lout) {
     float* g = g_mlp.grad;
-    loss_kernel<<<(M + 7) / 8, 256, 0, st>>>(g_mlp.Yo, g_mlp.T, M, g_mlp.dYf, g_mlp.dY, g_mlp.dYT, g_mlp.lpart);
+    loss_kernel<<<M / LOSS_ROWS, 256, 0, st>>>(g_mlp.Yo, g_mlp.T, M, g_mlp.dY, g_mlp.dYT, g_mlp.lpart,
+                                                g_mlp.cspart);
     CT2_LAUNCH_CHECK();
+    const ColSums none{nullptr, 0, 0, nullptr};
+    colsums<<<KOUT + (lout ? 1 : 0), 256, 0, st>>>(ColSums{g_mlp.cspart, M / LOSS_ROWS, KOUT, g + OFF_B3}, none,
+                                                   g_mlp.lpart, M / LOSS_ROWS, M, lout);
+    CT2_LAUNCH_CHECK();
+    // the hidden layers' bias partials: dZ2's in the first half of cspart, dZ1's in the second
+    float* cs2 = g_mlp.cspart;
+    float* cs1 = g_mlp.cspart + (size_t)(M / 128) * HID;
     // dW3 = H2^T dY, db3
     CT2_TRY(gemm_splitk(st, g_mlp.H2T, M, g_mlp.dYT, M, HID, KOUT, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W3));
-    CT2_TRY(colsum(st, g_mlp.dYf, M, KOUT, g_mlp.cspart, g + OFF_B3));
     // dZ2 = (dY W3^T) * gelu'(Z2)
-    EpiArgs d2{nullptr, HID, g_mlp.dZ2, g_mlp.dZ2T, M, nullptr, g_mlp.Z2, g_mlp.cspart};
+    EpiArgs d2{nullptr, HID, g_mlp.dZ2, g_mlp.dZ2T, M, nullptr, g_mlp.G2, cs2};
     CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dY, KOUT, g_mlp.W3n, KOUT, M, HID, KOUT, d2));
     // dW2 = H1^T dZ2, db2
     CT2_TRY(gemm_splitk(st, g_mlp.H1T, M, g_mlp.dZ2T, M, HID, HID, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W2));
-    colsum_final<<<HID, 256, 0, st>>>(g_mlp.cspart, M / 128, HID, g + OFF_B2);
-    CT2_LAUNCH_CHECK();
     // dZ1 = (dZ2 W2^T) * gelu'(Z1)
-    EpiArgs d1{nullptr, HID, nullptr, g_mlp.dZ1T, M, nullptr, g_mlp.Z1, g_mlp.cspart};
+    EpiArgs d1{nullptr, HID, nullptr, g_mlp.dZ1T, M, nullptr, g_mlp.G1, cs1};
     CT2_TRY(gemm<EPI_DGELU>(st, g_mlp.dZ2, HID, g_mlp.W2n, HID, M, HID, HID, d1));
     // dW1 = X^T dZ1, db1
     CT2_TRY(gemm_splitk(st, g_mlp.XT, M, g_mlp.dZ1T, M, KIN, HID, M, g_mlp.gsplit, SPLITK_MAX, g + OFF_W1));
-    colsum_final<<<HID, 256, 0, st>>>(g_mlp.cspart, M / 128, HID, g + OFF_B1);
+    // db2, db1
+    colsums<<<2 * HID, 256, 0, st>>>(ColSums{cs2, M / 128, HID, g + OFF_B2}, ColSums{cs1, M / 128, HID, g + OFF_B1},
+                                     nullptr, 0, M, nullptr);
     CT2_LAUNCH_CHECK();
     return CT2_OK;
 }
@@ -1084,8 +1316,8 @@ extern "C" int ct2_mlp_grad(const float* x, const float* t, size_t m, float* gra
     CT2_HIP(hipMemcpyAsync(g_mlp.XT, hxt.data(), hxt.size() * 2, hipMemcpyHostToDevice, c.stream));
     CT2_HIP(hipMemcpyAsync(g_mlp.T, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, c.stream));
     CT2_TRY(forward(c.stream, M, true));
-    CT2_TRY(backward(c.stream, M));
-    std::vector<float> lp((M + 7) / 8);
+    CT2_TRY(backward(c.stream, M, nullptr));
+    std::vector<float> lp(M / LOSS_ROWS);
     CT2_HIP(hipMemcpyAsync(grad, g_mlp.grad, NPARAM * 4, hipMemcpyDeviceToHost, c.stream));
     CT2_HIP(hipMemcpyAsync(lp.data(), g_mlp.lpart, lp.size() * 4, hipMemcpyDeviceToHost, c.stream));
     CT2_HIP(hipStreamSynchronize(c.stream));
@@ -1179,28 +1411,30 @@ extern "C" int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, floa
     Context& c = ctx();
     hipStream_t st = c.stream;
     const float b1 = 0.9f, b2 = 0.999f;
-    std::vector<float> lp((batch + 7) / 8);
+    // the step losses stay on the device until LOSS_CHUNK of them (or the
+    // last) are done: no host round trip between steps
     for (int s = 0; s < steps; ++s) {
         const uint64_t key = (seed * 0xD1B54A32D192ED03ull) ^ ((uint64_t)s * 0x632BE59BD9B4E019ull);
         batch_kernel<<<batch / BATCH_ROWS, BATCH_THREADS, 0, st>>>(g_mlp.DX, g_mlp.DT, g_mlp.dsize, key, batch,
                                                                    g_mlp.X, g_mlp.XT, g_mlp.T);
         CT2_LAUNCH_CHECK();
         CT2_TRY(forward(st, batch, true));
-        CT2_TRY(backward(st, batch));
+        const int slot = s % LOSS_CHUNK;
+        CT2_TRY(backward(st, batch, losses ? g_mlp.lsteps + slot : nullptr));
         ++g_mlp.adam_t;
         const float c1 = 1.0f - std::pow(b1, (float)g_mlp.adam_t), c2 = 1.0f - std::pow(b2, (float)g_mlp.adam_t);
         adam_step<<<(int)((NPARAM + 255) / 256), 256, 0, st>>>(g_mlp.w, g_mlp.m, g_mlp.v, g_mlp.grad, (int)NPARAM, lr, b1,
-                                                               b2, c1, c2, 1e-8f);
+                                                               b2, c1, c2, 1e-8f, packed());
         CT2_LAUNCH_CHECK();
-        CT2_TRY(repack(st));
         if (losses) {
-            CT2_HIP(hipMemcpyAsync(lp.data(), g_mlp.lpart, lp.size() * 4, hipMemcpyDeviceToHost, st));
-            CT2_HIP(hipStreamSynchronize(st));
-            double L = 0.0;
-            for (float v : lp) L += v;
-            losses[s] = (float)(L / batch);
+            if (slot == LOSS_CHUNK - 1 || s == steps - 1) {
+                CT2_HIP(hipMemcpyAsync(losses + (s - slot), g_mlp.lsteps, (size_t)(slot + 1) * 4,
+                                       hipMemcpyDeviceToHost, st));
+                CT2_HIP(hipStreamSynchronize(st));
+            }
         }
     }
+    CT2_TRY(repack(st));  // the inference layouts (the training ones are current)
     CT2_HIP(hipStreamSynchronize(st));
     g_mlp.trained = true;
     g_mlp.mesh_known = g_mlp.data_order > 0;
