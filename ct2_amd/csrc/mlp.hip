@@ -136,21 +136,21 @@ __device__ __forceinline__ void wave_sync() {
 struct TileLds {
     bf16_t tg[32][40];  // gelu'(z) rows (out of BIAS_GELU, into DGELU)
     bf16_t th[32][40];  // bf16 rows
-    bf16_t tt[32][40];  // bf16 transposed (80-byte rows: the 16 columns a store instruction spans hit 16 banks)
+    bf16_t tt[32][72];  // bf16 transposed, up to 64 rows (144-byte rows: the 16 columns a write spans hit 16 banks)
 };
 
 // The GELU / DGELU epilogue of one wave's 32 x 32 tile at (tm, tn), whose
 // lane holds C[tm + 16 i + 4 (lane/16) + j][tn + 16 jj + lane%16] in
 // acc[i][jj][j].  The tile passes through LDS, so that its bf16 rows leave
 // (and DGELU's saved gelu'(z) arrive) as whole 64-byte rows, 16 bytes per
-// lane and instruction, and the transposed copy as 32 rows of 64 contiguous
-// bytes.  The forward pass keeps gelu'(z) in bf16 for the backward one (the
+// lane and instruction; its transpose goes to columns roff .. roff + 31 of
+// L.tt, which store_ht writes out.  The forward pass keeps gelu'(z) in bf16 for the backward one (the
 // product dz = dh gelu'(z) is rounded to bf16 in any case) rather than z in
 // FP32: half the bytes written and read again.  DGELU adds this lane's dz of
 // column block jj to csum[jj].
 template <int EPI>
 __device__ __forceinline__ void gelu_tile(const EpiArgs& ep, int tm, int tn, const floatx4 (&acc)[2][2], int lane,
-                                          TileLds& L, float* csum) {
+                                          TileLds& L, float* csum, int roff) {
     const int r16 = lane & 15;
     if constexpr (EPI == EPI_DGELU) {
 #pragma unroll
@@ -190,7 +190,7 @@ __device__ __forceinline__ void gelu_tile(const EpiArgs& ep, int tm, int tn, con
                 }
                 L.th[r][c] = (bf16_t)(hw & 0xffffu);
                 L.th[r + 1][c] = (bf16_t)(hw >> 16);
-                *(uint32_t*)&L.tt[c][r] = hw;
+                *(uint32_t*)&L.tt[c][roff + r] = hw;
             }
         }
     wave_sync();
@@ -208,14 +208,22 @@ __device__ __forceinline__ void gelu_tile(const EpiArgs& ep, int tm, int tn, con
             *(uint4*)(ep.h + (size_t)(tm + r) * ep.ldc + tn + 8 * part) = *(const uint4*)&L.th[r][8 * part];
         }
     }
-    if (ep.ht) {
+    wave_sync();  // the next tile may reuse L.th, L.tg
+}
+
+// The transposed copy of rows tm .. tm + ROWS - 1, columns tn .. tn + 31
+// (ROWS = 32 or 64), from L.tt: 32 rows of 2 ROWS contiguous bytes, 16 per
+// lane and store.  A 64-row tile leaves as whole 128-byte lines (HT rows are
+// M entries apart, so 64-byte pieces cost a partial line each).
+template <int ROWS>
+__device__ __forceinline__ void store_ht(const EpiArgs& ep, int tm, int tn, int lane, TileLds& L) {
+    constexpr int PARTS = ROWS / 8;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int e = q * 64 + lane, c = e >> 2, part = e & 3;
-            *(uint4*)(ep.ht + (size_t)(tn + c) * ep.ldt + tm + 8 * part) = *(const uint4*)&L.tt[c][8 * part];
-        }
+    for (int q = 0; q < 32 * PARTS / 64; ++q) {
+        const int e = q * 64 + lane, c = e / PARTS, part = e % PARTS;
+        *(uint4*)(ep.ht + (size_t)(tn + c) * ep.ldt + tm + 8 * part) = *(const uint4*)&L.tt[c][8 * part];
     }
-    wave_sync();  // the next tile may reuse L
+    wave_sync();  // the next tile may reuse L.tt
 }
 
 template <int EPI>
@@ -247,7 +255,8 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
     if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
         __shared__ TileLds lds[4];
         float csum[2] = {0.0f, 0.0f};  // DGELU: this lane's sums of dz per column block jj
-        gelu_tile<EPI>(ep, tm, tn, acc, lane, lds[wave], csum);
+        gelu_tile<EPI>(ep, tm, tn, acc, lane, lds[wave], csum, 0);
+        if (ep.ht) store_ht<32>(ep, tm, tn, lane, lds[wave]);
         if (EPI == EPI_DGELU && ep.cpart) {
             // the bias gradient's share of this block: column sums over its
             // 128 rows (lane groups, then the four waves in order)
@@ -300,33 +309,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     const GemmBlock gb = gemm_block((int)blockIdx.x, M / 128, N / 128, S);
     const int wr = wave & 1, wc = wave >> 1;
     const int tm = gb.rb * 128 + wr * 64, tn = gb.cb * 128 + wc * 64;
-    const int kc = ((K / 32 + S - 1) / S) * 32;
+    // K slices of whole 64-k steps (K % 64 == 0 when S > 1)
+    const int kc = S == 1 ? K : ((K / 64 + S - 1) / S) * 64;
     const int kb = gb.z * kc, ke = min(K, kb + kc);
-    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    const int r16 = lane & 15, g4 = lane >> 4;
     floatx4 acc[4][4] = {};
-    const bf16_t* a = A + (size_t)(tm + r16) * lda + kq;
-    const bf16_t* b = Bt + (size_t)(tn + r16) * ldb + kq;
-    // Scheduling barriers keep the compiler from sinking each fragment load
-    // to just before its first MFMA, which serialises the memory latency of
-    // the four row fragments.  The long-K split products (STORE) also load
-    // the fragments of step k + 32 while the MFMAs of step k run (the last
-    // step loads its own again: in bounds, unused); the GELU products (K <=
-    // 256) keep the registers for occupancy instead.
-    bf16x8 fa[4], fb[4];
+    const bf16_t* a = A + (size_t)(tm + r16) * lda;
+    const bf16_t* b = Bt + (size_t)(tn + r16) * ldb;
+    // The split products (STORE, K slices of 256 or more) take 64-k steps:
+    // lane group g takes k + 16 g .. k + 16 g + 15 of its row (32 contiguous
+    // bytes, two loads), the first 8 for one MFMA and the last 8 for the next,
+    // so that the four groups consume whole 128-byte lines in one step (32-k
+    // steps use half a line per step, and the line is fetched again for the
+    // other half); A and B take the same assignment, so the sum over k is
+    // unchanged.  The GELU products (K <= 256) take 32-k steps, lane group g
+    // k + 8 g .. k + 8 g + 7, which keeps the registers for a fourth wave per
+    // SIMD (measured faster there).  Scheduling barriers keep the compiler
+    // from sinking each load to just before its first MFMA, which serialises
+    // the memory latency of the fragments.
+    int k = kb;
+    if constexpr (EPI == EPI_STORE) {
+        for (; k + 64 <= ke; k += 64) {
+            bf16x8 fa[2][4], fb[2][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        fa[i] = load8(a + (size_t)16 * i * lda + kb);
-        fb[i] = load8(b + (size_t)16 * i * ldb + kb);
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    fa[h][i] = load8(a + (size_t)16 * i * lda + k + 16 * g4 + 8 * h);
+                    fb[h][i] = load8(b + (size_t)16 * i * ldb + k + 16 * g4 + 8 * h);
+                }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[h][i], fb[h][j], acc[i][j], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
-    for (int k = kb; k < ke; k += 32) {
-        const int kn = k + 32 < ke ? k + 32 : k;
-        bf16x8 na[4], nb[4];
-        if constexpr (EPI == EPI_STORE) {
+    for (; k < ke; k += 32) {
+        bf16x8 fa[4], fb[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                na[i] = load8(a + (size_t)16 * i * lda + kn);
-                nb[i] = load8(b + (size_t)16 * i * ldb + kn);
-            }
+        for (int i = 0; i < 4; ++i) {
+            fa[i] = load8(a + (size_t)16 * i * lda + k + 8 * g4);
+            fb[i] = load8(b + (size_t)16 * i * ldb + k + 8 * g4);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -335,20 +363,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
             for (int j = 0; j < 4; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
         __builtin_amdgcn_sched_barrier(0);
-        if constexpr (EPI == EPI_STORE) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                fa[i] = na[i];
-                fb[i] = nb[i];
-            }
-        } else if (k + 32 < ke) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                fa[i] = load8(a + (size_t)16 * i * lda + kn);
-                fb[i] = load8(b + (size_t)16 * i * ldb + kn);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
     }
     if constexpr (EPI == EPI_STORE) {
         float* out = ep.f32 + (size_t)gb.z * M * ep.ldc;
@@ -364,13 +378,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     __shared__ TileLds lds[4];
     float csum[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // DGELU: this lane's sums of dz per 16-column block
 #pragma unroll
-    for (int i2 = 0; i2 < 2; ++i2)
+    for (int j2 = 0; j2 < 2; ++j2) {
 #pragma unroll
-        for (int j2 = 0; j2 < 2; ++j2) {
+        for (int i2 = 0; i2 < 2; ++i2) {
             const floatx4 sub[2][2] = {{acc[2 * i2][2 * j2], acc[2 * i2][2 * j2 + 1]},
                                        {acc[2 * i2 + 1][2 * j2], acc[2 * i2 + 1][2 * j2 + 1]}};
-            gelu_tile<EPI>(ep, tm + 32 * i2, tn + 32 * j2, sub, lane, lds[wave], csum + 2 * j2);
+            gelu_tile<EPI>(ep, tm + 32 * i2, tn + 32 * j2, sub, lane, lds[wave], csum + 2 * j2, 32 * i2);
         }
+        if (ep.ht) store_ht<64>(ep, tm, tn + 32 * j2, lane, lds[wave]);
+    }
     if (EPI == EPI_DGELU && ep.cpart) {
         // the bias gradient's share of this 128-row block: each wave's column
         // sums over its 64 rows (lane groups), then the two waves of a column
