@@ -32,12 +32,15 @@ def test_mlp_forward_vs_oracle(ct2, orc):
     assert np.median(np.abs(got - exact)) < 2e-2 * scale  # bf16 network vs FP64 network
 
 
-def test_mlp_gradients_vs_oracle(ct2, orc):
+@pytest.mark.parametrize("m", [256, 2048])
+def test_mlp_gradients_vs_oracle(ct2, orc, m):
+    """m = 2048 takes the split-K weight gradients (8 K slices and the
+    ordered reduction of their planes); 256 the single-slice products."""
     ct2.mlp_init(11)
     w = ct2.mlp_get_weights().astype(np.float64)
     rng = np.random.default_rng(2)
-    X = features(rng, 256)
-    T = np.concatenate([rng.standard_normal((256, 3)), rng.uniform(0.5, 2.0, (256, 1))], axis=1).astype(np.float32)
+    X = features(rng, m)
+    T = np.concatenate([rng.standard_normal((m, 3)), rng.uniform(0.5, 2.0, (m, 1))], axis=1).astype(np.float32)
     loss, g = ct2.mlp_grad(X, T)
     oloss, og = orc.mlp_grad(w, X, T, bf16=True)
     assert loss == pytest.approx(oloss, rel=1e-2)
