@@ -73,7 +73,9 @@ def test_mlp_features_from_pm_grids(ct2):
     funit = gm * ng * ng
     for r, p in enumerate(idx):
         c = [((int(v[p]) + (1 << (s - 1))) >> s) % ng for v in (x, y, z)]
-        off = [np.float32(np.int32(np.uint32(v[p]) - np.uint32(cc << s))) / np.float32(2 ** s) for v, cc in zip((x, y, z), c)]
+        # the wrapped 32-bit difference, as the kernel takes it
+        off = [np.float32(np.uint32((int(v[p]) - (cc << s)) & 0xFFFFFFFF).view(np.int32)) / np.float32(2 ** s)
+               for v, cc in zip((x, y, z), c)]
         assert np.allclose(X[r, 108:111], off, atol=1e-7)
         q = 0
         for a in (-1, 0, 1):
