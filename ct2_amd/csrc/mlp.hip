@@ -309,8 +309,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     const GemmBlock gb = gemm_block((int)blockIdx.x, M / 128, N / 128, S);
     const int wr = wave & 1, wc = wave >> 1;
     const int tm = gb.rb * 128 + wr * 64, tn = gb.cb * 128 + wc * 64;
-    // K slices of whole 64-k steps (K % 64 == 0 when S > 1)
-    const int kc = S == 1 ? K : ((K / 64 + S - 1) / S) * 64;
+    // K slices of whole 64-k steps (a K % 64 remainder of 32 falls to the
+    // last slice's 32-k step)
+    const int kc = S == 1 ? K : (((K + 63) / 64 + S - 1) / S) * 64;
     const int kb = gb.z * kc, ke = min(K, kb + kc);
     const int r16 = lane & 15, g4 = lane >> 4;
     floatx4 acc[4][4] = {};
@@ -1429,7 +1430,7 @@ extern "C" int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, floa
     const float b1 = 0.9f, b2 = 0.999f;
     // the step losses stay on the device until LOSS_CHUNK of them (or the
     // last) are done: no host round trip between steps
-    for (int s = 0; s < steps; ++s) {
+    auto step = [&](int s) -> int {
         const uint64_t key = (seed * 0xD1B54A32D192ED03ull) ^ ((uint64_t)s * 0x632BE59BD9B4E019ull);
         batch_kernel<<<batch / BATCH_ROWS, BATCH_THREADS, 0, st>>>(g_mlp.DX, g_mlp.DT, g_mlp.dsize, key, batch,
                                                                    g_mlp.X, g_mlp.XT, g_mlp.T);
@@ -1442,15 +1443,20 @@ extern "C" int ct2_mlp_train(int steps, int batch, float lr, uint64_t seed, floa
         adam_step<<<(int)((NPARAM + 255) / 256), 256, 0, st>>>(g_mlp.w, g_mlp.m, g_mlp.v, g_mlp.grad, (int)NPARAM, lr, b1,
                                                                b2, c1, c2, 1e-8f, packed());
         CT2_LAUNCH_CHECK();
-        if (losses) {
-            if (slot == LOSS_CHUNK - 1 || s == steps - 1) {
-                CT2_HIP(hipMemcpyAsync(losses + (s - slot), g_mlp.lsteps, (size_t)(slot + 1) * 4,
-                                       hipMemcpyDeviceToHost, st));
-                CT2_HIP(hipStreamSynchronize(st));
-            }
+        if (losses && (slot == LOSS_CHUNK - 1 || s == steps - 1)) {
+            CT2_HIP(hipMemcpyAsync(losses + (s - slot), g_mlp.lsteps, (size_t)(slot + 1) * 4, hipMemcpyDeviceToHost,
+                                   st));
+            CT2_HIP(hipStreamSynchronize(st));
         }
-    }
-    CT2_TRY(repack(st));  // the inference layouts (the training ones are current)
+        return CT2_OK;
+    };
+    int rc = CT2_OK;
+    for (int s = 0; s < steps && rc == CT2_OK; ++s) rc = step(s);
+    // the inference layouts follow the weights even when a step failed (Adam
+    // keeps the training layouts current itself)
+    const int rp = repack(st);
+    if (rc != CT2_OK) return rc;
+    CT2_TRY(rp);
     CT2_HIP(hipStreamSynchronize(st));
     g_mlp.trained = true;
     g_mlp.mesh_known = g_mlp.data_order > 0;
