@@ -131,6 +131,33 @@ def test_mlp_training_reduces_pm_error(ct2):
     assert np.median(e_ml) < 0.8 * np.median(e_pm)
 
 
+def test_mlp_training_losses_are_deterministic_across_copy_chunks(ct2):
+    """ct2_mlp_train keeps the step losses on the device and copies them in
+    chunks of 4,096 steps: 4,100 steps fill every entry, and two runs from the
+    same initial weights and seed give the same weights and losses bit for
+    bit (fixed-order reductions throughout, no atomics on floats)."""
+    side, ng = 16, 32
+    n = side ** 3
+    ct2.set_force_options(1.0 / (50 * side), float(np.float32(1.0 / n)), 0)
+    ct2.pm_init(ng, 1.25)
+    x, y, z = ct2.ic_lattice(side, 0.2, 4)
+    ct2.particle_allocate(n)
+    ct2.particles_upload(x, y, z)
+    ct2.mlp_init(7)
+    ct2.mlp_data_reset(1024)
+    ct2.mlp_data_append(np.random.default_rng(4).choice(n, 1024, replace=False))
+    w_init = ct2.mlp_get_weights()
+    runs = []
+    for _ in range(2):
+        ct2.mlp_set_weights(w_init)  # also clears Adam's moments and step count
+        losses = ct2.mlp_train(4100, 128, 1e-3, 3)
+        runs.append((losses, ct2.mlp_get_weights()))
+    (l0, w0), (l1, w1) = runs
+    assert np.isfinite(l0).all() and (l0 > 0).all()
+    assert np.array_equal(l0, l1) and np.array_equal(w0, w1)
+    assert np.mean(l0[-100:]) < np.mean(l0[:20])
+
+
 def test_fused_inference_matches_gemm_forward(ct2, orc):
     """ct2_pm_mlp_forces runs the three layers in one kernel (transposed
     products, activations kept in registers, k-permuted weights).  Its
