@@ -409,10 +409,154 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     }
 }
 
+// The wide products with M % 256 == 0, their B operand staged in LDS: 8
+// waves per workgroup (4 along M x 2 along N, 64 x 64 each), a 256 x 128
+// workgroup tile.  B (the weights, or dZ^T in the weight gradients) passes
+// through LDS in 64-k chunks, double buffered: the workgroup loads a chunk
+// once (128 columns x 128 bytes, each thread 32 contiguous bytes) while the
+// previous chunk's MFMAs run, where in gemm_wide every pair of waves sharing
+// columns loads it from L2.  A fragments come from memory as there.  Within a
+// chunk, lane group g takes k 16 g + 8 h .. + 7 in half h (the two halves of
+// a 32-byte piece of its row, both in flight in the same chunk), and B the
+// same k; a 32-k chunk (K % 64 remainder) takes 8 g .. 8 g + 7.  The staging
+// buffers and the epilogue tiles share the LDS.
+constexpr int LB_ROW = 72;  // bf16 per staged B column: 64 k + 8 (144-byte rows: 16 columns on distinct banks)
+union GemmLds {
+    bf16_t b[2][128][LB_ROW];
+    TileLds t[8];
+};
+template <int EPI>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void gemm_lds(
+    const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt, int ldb, int M, int N, int K, int S,
+    EpiArgs ep) {
+    static_assert(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU || EPI == EPI_STORE, "no staged BIAS epilogue");
+    __shared__ GemmLds L;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const GemmBlock gb = gemm_block((int)blockIdx.x, M / 256, N / 128, S);
+    const int wr = wave & 3, wc = wave >> 2;
+    const int tm = gb.rb * 256 + wr * 64, tn = gb.cb * 128 + wc * 64;
+    const int kc = S == 1 ? K : (((K + 63) / 64 + S - 1) / S) * 64;
+    const int kb = gb.z * kc, ke = min(K, kb + kc);
+    const int r16 = lane & 15, g4 = lane >> 4;
+    floatx4 acc[4][4] = {};
+    const bf16_t* a = A + (size_t)(tm + r16) * lda;
+    // staging: thread t moves column t / 4, k 16 (t % 4) .. + 15 of a chunk
+    const int scol = threadIdx.x >> 2, spart = threadIdx.x & 3;
+    const bf16_t* bsrc = Bt + (size_t)(gb.cb * 128 + scol) * ldb + 16 * spart;
+    const int nch = ke > kb ? (ke - kb + 63) / 64 : 0;
+    uint4 st0 = make_uint4(0u, 0u, 0u, 0u), st1 = st0;
+    auto fetch = [&](int c) {
+        const int k0 = kb + 64 * c;
+        if (16 * spart < ke - k0) {
+            st0 = *(const uint4*)(bsrc + k0);
+            st1 = *(const uint4*)(bsrc + k0 + 8);
+        }
+    };
+    auto stash = [&](int c) {
+        if (16 * spart < ke - (kb + 64 * c)) {
+            *(uint4*)&L.b[c & 1][scol][16 * spart] = st0;
+            *(uint4*)&L.b[c & 1][scol][16 * spart + 8] = st1;
+        }
+    };
+    if (nch > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int k0 = kb + 64 * c;
+        if (c + 1 < nch) fetch(c + 1);
+        const bf16_t(*Bl)[LB_ROW] = L.b[c & 1];
+        if (ke - k0 >= 64) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                bf16x8 fa[4], fb[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    fa[i] = load8(a + (size_t)16 * i * lda + k0 + 16 * g4 + 8 * h);
+                    fb[i] = *(const bf16x8*)&Bl[wc * 64 + 16 * i + r16][16 * g4 + 8 * h];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {  // a 32-k chunk
+            bf16x8 fa[4], fb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                fa[i] = load8(a + (size_t)16 * i * lda + k0 + 8 * g4);
+                fb[i] = *(const bf16x8*)&Bl[wc * 64 + 16 * i + r16][8 * g4];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (c + 1 < nch) stash(c + 1);  // the other buffer: its readers passed the last barrier
+        __syncthreads();
+    }
+    if constexpr (EPI == EPI_STORE) {
+        float* out = ep.f32 + (size_t)gb.z * M * ep.ldc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    out[(size_t)(tm + 16 * i + 4 * (lane >> 4) + j) * ep.ldc + tn + 16 * jj + r16] = acc[i][jj][j];
+    } else {
+        // the staging buffers are free (the loop ended on a barrier)
+        float csum[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // DGELU: this lane's sums of dz per 16-column block
+#pragma unroll
+        for (int j2 = 0; j2 < 2; ++j2) {
+#pragma unroll
+            for (int i2 = 0; i2 < 2; ++i2) {
+                const floatx4 sub[2][2] = {{acc[2 * i2][2 * j2], acc[2 * i2][2 * j2 + 1]},
+                                           {acc[2 * i2 + 1][2 * j2], acc[2 * i2 + 1][2 * j2 + 1]}};
+                gelu_tile<EPI>(ep, tm + 32 * i2, tn + 32 * j2, sub, lane, L.t[wave], csum + 2 * j2, 32 * i2);
+            }
+            if (ep.ht) store_ht<64>(ep, tm, tn + 32 * j2, lane, L.t[wave]);
+        }
+        if (EPI == EPI_DGELU && ep.cpart) {
+            // the bias gradient's share of each 128-row block (waves wr = 0, 1
+            // and 2, 3): each wave's column sums over its 64 rows, then the
+            // two waves of a block and column half in order
+            __shared__ float cs[8][64];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float v = csum[q];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (lane < 16) cs[wave][16 * q + lane] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < 256) {
+                const int hb = threadIdx.x >> 7, cc = threadIdx.x & 127, w0 = 2 * hb + 4 * (cc >> 6), c = cc & 63;
+                ep.cpart[(size_t)(2 * gb.rb + hb) * N + gb.cb * 128 + cc] = cs[w0][c] + cs[w0 + 1][c];
+            }
+        }
+    }
+}
+
 template <int EPI>
 int gemm(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K, const EpiArgs& ep) {
     if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
     if constexpr (EPI == EPI_BIAS_GELU || EPI == EPI_DGELU) {
+        // the staged tiles where they make at least one workgroup per CU
+        // (measured slower with fewer: batch 16,384 took 0.138 instead of
+        // 0.128 ms per step)
+        if (N % 128 == 0 && M % 256 == 0 && (M / 256) * (N / 128) >= 256) {
+            gemm_lds<EPI><<<(M / 256) * (N / 128), 512, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
+            CT2_LAUNCH_CHECK();
+            return CT2_OK;
+        }
         if (N % 128 == 0) {
             gemm_wide<EPI><<<(M / 128) * (N / 128), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
             CT2_LAUNCH_CHECK();
@@ -464,6 +608,20 @@ __global__ __launch_bounds__(256) void splitk_reduce(const float* __restrict__ p
 int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K,
                 float* scratch, int smax, float* out) {
     if (M % 128 || N % 32 || K % 32) return set_error(CT2_ERR_INVALID, "gemm shape %dx%dx%d not tiled", M, N, K);
+    // 256 x 128 staged tiles, about one workgroup per CU, where K is long
+    // enough for that many slices
+    if (N % 128 == 0 && M % 256 == 0 && (M / 256) * (N / 128) * std::min(smax, K / 256) >= 256) {
+        const int tiles = (M / 256) * (N / 128);
+        const int S = std::max(1, std::min({smax, K / 256, (256 + tiles - 1) / tiles}));
+        EpiArgs ep{S > 1 ? scratch : out, N, nullptr, nullptr, 0, nullptr, nullptr};
+        gemm_lds<EPI_STORE><<<tiles * S, 512, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
+        CT2_LAUNCH_CHECK();
+        if (S > 1) {
+            splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
+            CT2_LAUNCH_CHECK();
+        }
+        return CT2_OK;
+    }
     if (N % 128 == 0) {  // 128 x 128 workgroup tiles, at least 512 workgroups
         const int tiles = (M / 128) * (N / 128);
         const int S = std::max(1, std::min({smax, K / 256, (512 + tiles - 1) / tiles}));

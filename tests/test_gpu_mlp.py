@@ -32,10 +32,12 @@ def test_mlp_forward_vs_oracle(ct2, orc):
     assert np.median(np.abs(got - exact)) < 2e-2 * scale  # bf16 network vs FP64 network
 
 
-@pytest.mark.parametrize("m", [256, 2048])
+@pytest.mark.parametrize("m", [256, 2048, 32768])
 def test_mlp_gradients_vs_oracle(ct2, orc, m):
     """m = 2048 takes the split-K weight gradients (8 K slices and the
-    ordered reduction of their planes); 256 the single-slice products."""
+    ordered reduction of their planes); 256 the single-slice products;
+    32,768 the 256 x 128 LDS-staged tiles (hidden-layer products and the
+    layer-2 weight gradient, 128 K slices)."""
     ct2.mlp_init(11)
     w = ct2.mlp_get_weights().astype(np.float64)
     rng = np.random.default_rng(2)
