@@ -420,13 +420,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
 // a 32-byte piece of its row, both in flight in the same chunk), and B the
 // same k; a 32-k chunk (K % 64 remainder) takes 8 g .. 8 g + 7.  The staging
 // buffers and the epilogue tiles share the LDS.
-constexpr int LB_ROW = 72;  // bf16 per staged B column: 64 k + 8 (144-byte rows: 16 columns on distinct banks)
+typedef __attribute__((address_space(3))) void lds_void;
 union GemmLds {
-    bf16_t b[2][128][LB_ROW];
+    bf16_t b[2][128][64];  // staged B chunks: column c's 16-byte piece p at slot p ^ (c & 7)
     TileLds t[8];
 };
 template <int EPI>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void gemm_lds(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void gemm_lds(
     const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt, int ldb, int M, int N, int K, int S,
     EpiArgs ep) {
     static_assert(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU || EPI == EPI_STORE, "no staged BIAS epilogue");
@@ -440,57 +440,43 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void g
     const int r16 = lane & 15, g4 = lane >> 4;
     floatx4 acc[4][4] = {};
     const bf16_t* a = A + (size_t)(tm + r16) * lda;
-    // staging: thread t moves column t / 4, k 16 (t % 4) .. + 15 of a chunk
-    const int scol = threadIdx.x >> 2, spart = threadIdx.x & 3;
-    const bf16_t* bsrc = Bt + (size_t)(gb.cb * 128 + scol) * ldb + 16 * spart;
-    const int nch = ke > kb ? (ke - kb + 63) / 64 : 0;
-    uint4 st0 = make_uint4(0u, 0u, 0u, 0u), st1 = st0;
-    auto fetch = [&](int c) {
-        const int k0 = kb + 64 * c;
-        if (16 * spart < ke - k0) {
-            st0 = *(const uint4*)(bsrc + k0);
-            st1 = *(const uint4*)(bsrc + k0 + 8);
-        }
+    // staging: wave w moves column blocks 2 w and 2 w + 1 (8 columns each) of a
+    // chunk straight from memory into LDS (global_load_lds_dwordx4: lane l's 16
+    // bytes land at the block's base + 16 l, i.e. column l / 8, slot l % 8,
+    // which holds piece (l % 8) ^ (l / 8) of the column)
+    const int sp = (lane & 7) ^ (lane >> 3);
+    const bf16_t* bsrc = Bt + (size_t)(gb.cb * 128 + 16 * wave + (lane >> 3)) * ldb;
+    // whole 64-k chunks, then at most one 32-k chunk (K % 64 remainder),
+    // outside the loop so that the accumulators flow through one path
+    const int nfull = ke > kb ? (ke - kb) / 64 : 0, nch = ke > kb ? (ke - kb + 63) / 64 : 0;
+    auto stage = [&](int c) {
+        const int k0 = kb + 64 * c, p = ke - k0 >= 64 ? sp : (sp & 3);  // a 32-k chunk reads pieces 0..3 only
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            __builtin_amdgcn_global_load_lds((const void*)(bsrc + (size_t)8 * q * ldb + k0 + 8 * p),
+                                             (lds_void*)&L.b[c & 1][16 * wave + 8 * q][0], 16, 0, 0);
     };
-    auto stash = [&](int c) {
-        if (16 * spart < ke - (kb + 64 * c)) {
-            *(uint4*)&L.b[c & 1][scol][16 * spart] = st0;
-            *(uint4*)&L.b[c & 1][scol][16 * spart + 8] = st1;
-        }
-    };
-    if (nch > 0) {
-        fetch(0);
-        stash(0);
-    }
+    if (nch > 0) stage(0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int c = 0; c < nch; ++c) {
+    for (int c = 0; c < nfull; ++c) {
         const int k0 = kb + 64 * c;
-        if (c + 1 < nch) fetch(c + 1);
-        const bf16_t(*Bl)[LB_ROW] = L.b[c & 1];
-        if (ke - k0 >= 64) {
+        const bf16_t(*Bl)[64] = L.b[c & 1];
+        // each half's B fragments before its A loads; the next chunk's loads
+        // into the other buffer start once this chunk's last LDS reads are
+        // issued, before the second half's MFMAs
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                bf16x8 fa[4], fb[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    fa[i] = load8(a + (size_t)16 * i * lda + k0 + 16 * g4 + 8 * h);
-                    fb[i] = *(const bf16x8*)&Bl[wc * 64 + 16 * i + r16][16 * g4 + 8 * h];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
-                __builtin_amdgcn_sched_barrier(0);
-            }
-        } else {  // a 32-k chunk
+        for (int h = 0; h < 2; ++h) {
             bf16x8 fa[4], fb[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                fa[i] = load8(a + (size_t)16 * i * lda + k0 + 8 * g4);
-                fb[i] = *(const bf16x8*)&Bl[wc * 64 + 16 * i + r16][8 * g4];
+            for (int j = 0; j < 4; ++j) {
+                const int col = wc * 64 + 16 * j + r16;
+                fb[j] = *(const bf16x8*)&Bl[col][8 * ((2 * g4 + h) ^ (col & 7))];
             }
+            __builtin_amdgcn_sched_barrier(0);
+            if (h == 1 && c + 1 < nch) stage(c + 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) fa[i] = load8(a + (size_t)16 * i * lda + k0 + 16 * g4 + 8 * h);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -499,8 +485,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void g
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (c + 1 < nch) stash(c + 1);  // the other buffer: its readers passed the last barrier
+        __builtin_amdgcn_s_waitcnt(0);  // the next chunk has landed in LDS
         __syncthreads();
+    }
+    if (nch > nfull) {  // the 32-k chunk: k 8 g .. 8 g + 7
+        const int k0 = kb + 64 * nfull;
+        const bf16_t(*Bl)[64] = L.b[nfull & 1];
+        bf16x8 fa[4], fb[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = wc * 64 + 16 * j + r16;
+            fb[j] = *(const bf16x8*)&Bl[col][8 * (g4 ^ (col & 7))];
+            fa[j] = load8(a + (size_t)16 * j * lda + k0 + 8 * g4);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        __syncthreads();  // the epilogue reuses the staging buffers
     }
     if constexpr (EPI == EPI_STORE) {
         float* out = ep.f32 + (size_t)gb.z * M * ep.ldc;
