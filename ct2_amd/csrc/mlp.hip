@@ -413,13 +413,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
 // waves per workgroup (4 along M x 2 along N, 64 x 64 each), a 256 x 128
 // workgroup tile.  B (the weights, or dZ^T in the weight gradients) passes
 // through LDS in 64-k chunks, double buffered: the workgroup loads a chunk
-// once (128 columns x 128 bytes, each thread 32 contiguous bytes) while the
-// previous chunk's MFMAs run, where in gemm_wide every pair of waves sharing
-// columns loads it from L2.  A fragments come from memory as there.  Within a
-// chunk, lane group g takes k 16 g + 8 h .. + 7 in half h (the two halves of
-// a 32-byte piece of its row, both in flight in the same chunk), and B the
-// same k; a 32-k chunk (K % 64 remainder) takes 8 g .. 8 g + 7.  The staging
-// buffers and the epilogue tiles share the LDS.
+// once (128 columns x 128 bytes, by direct memory-to-LDS loads that use no
+// registers) while the current chunk's MFMAs run, where in gemm_wide every
+// pair of waves sharing columns loads it from L2.  A column's eight 16-byte
+// pieces sit in its 128-byte LDS row at slots p ^ (c & 7), so that the 16
+// columns a fragment read spans fall on different banks.  A fragments come
+// from memory as there.  Within a chunk, lane group g takes k 16 g + 8 h ..
+// + 7 in half h (the two halves of a 32-byte piece of its row, both in flight
+// in the same chunk), and B the same k; a 32-k chunk (K % 64 remainder) takes
+// 8 g .. 8 g + 7.  The staging buffers and the epilogue tiles share the LDS
+// (78 KB: two workgroups per CU at 128 registers).
 typedef __attribute__((address_space(3))) void lds_void;
 union GemmLds {
     bf16_t b[2][128][64];  // staged B chunks: column c's 16-byte piece p at slot p ^ (c & 7)
