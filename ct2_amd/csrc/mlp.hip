@@ -296,168 +296,62 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16_t* __restrict__ A, i
 
 // The products with M and N multiples of 128 (the hidden layers and their
 // weight gradients): one wave per 64 x 64 output tile (4 x 4 MFMA 16x16x32
-// tiles), 2 x 2 waves per 128 x 128 workgroup tile.  Each operand fragment a
-// wave loads feeds four MFMAs instead of two, which halves the L2 reads per
-// product of the 32 x 32 tiling.  GELU / DGELU epilogues run tile by tile as
-// there; EPI_STORE with S K slices writes slice z's partial product to plane
-// z of ep.f32.
-template <int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gemm_wide(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt,
-                                                 int ldb, int M, int N, int K, int S, EpiArgs ep) {
-    static_assert(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU || EPI == EPI_STORE, "no wide BIAS epilogue");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const GemmBlock gb = gemm_block((int)blockIdx.x, M / 128, N / 128, S);
-    const int wr = wave & 1, wc = wave >> 1;
-    const int tm = gb.rb * 128 + wr * 64, tn = gb.cb * 128 + wc * 64;
-    // K slices of whole 64-k steps (a K % 64 remainder of 32 falls to the
-    // last slice's 32-k step)
-    const int kc = S == 1 ? K : (((K + 63) / 64 + S - 1) / S) * 64;
-    const int kb = gb.z * kc, ke = min(K, kb + kc);
-    const int r16 = lane & 15, g4 = lane >> 4;
-    floatx4 acc[4][4] = {};
-    const bf16_t* a = A + (size_t)(tm + r16) * lda;
-    const bf16_t* b = Bt + (size_t)(tn + r16) * ldb;
-    // The split products (STORE, K slices of 256 or more) take 64-k steps:
-    // lane group g takes k + 16 g .. k + 16 g + 15 of its row (32 contiguous
-    // bytes, two loads), the first 8 for one MFMA and the last 8 for the next,
-    // so that the four groups consume whole 128-byte lines in one step (32-k
-    // steps use half a line per step, and the line is fetched again for the
-    // other half); A and B take the same assignment, so the sum over k is
-    // unchanged.  The GELU products (K <= 256) take 32-k steps, lane group g
-    // k + 8 g .. k + 8 g + 7, which keeps the registers for a fourth wave per
-    // SIMD (measured faster there).  Scheduling barriers keep the compiler
-    // from sinking each load to just before its first MFMA, which serialises
-    // the memory latency of the fragments.
-    int k = kb;
-    if constexpr (EPI == EPI_STORE) {
-        for (; k + 64 <= ke; k += 64) {
-            bf16x8 fa[2][4], fb[2][4];
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    fa[h][i] = load8(a + (size_t)16 * i * lda + k + 16 * g4 + 8 * h);
-                    fb[h][i] = load8(b + (size_t)16 * i * ldb + k + 16 * g4 + 8 * h);
-                }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[h][i], fb[h][j], acc[i][j], 0, 0, 0);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-    for (; k < ke; k += 32) {
-        bf16x8 fa[4], fb[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            fa[i] = load8(a + (size_t)16 * i * lda + k + 8 * g4);
-            fb[i] = load8(b + (size_t)16 * i * ldb + k + 8 * g4);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if constexpr (EPI == EPI_STORE) {
-        float* out = ep.f32 + (size_t)gb.z * M * ep.ldc;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    out[(size_t)(tm + 16 * i + 4 * (lane >> 4) + j) * ep.ldc + tn + 16 * jj + r16] = acc[i][jj][j];
-        return;
-    } else {
-    __shared__ TileLds lds[4];
-    float csum[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // DGELU: this lane's sums of dz per 16-column block
-#pragma unroll
-    for (int j2 = 0; j2 < 2; ++j2) {
-#pragma unroll
-        for (int i2 = 0; i2 < 2; ++i2) {
-            const floatx4 sub[2][2] = {{acc[2 * i2][2 * j2], acc[2 * i2][2 * j2 + 1]},
-                                       {acc[2 * i2 + 1][2 * j2], acc[2 * i2 + 1][2 * j2 + 1]}};
-            gelu_tile<EPI>(ep, tm + 32 * i2, tn + 32 * j2, sub, lane, lds[wave], csum + 2 * j2, 32 * i2);
-        }
-        if (ep.ht) store_ht<64>(ep, tm, tn + 32 * j2, lane, lds[wave]);
-    }
-    if (EPI == EPI_DGELU && ep.cpart) {
-        // the bias gradient's share of this 128-row block: each wave's column
-        // sums over its 64 rows (lane groups), then the two waves of a column
-        // half in order
-        __shared__ float cs[4][64];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float v = csum[q];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            if (lane < 16) cs[wave][16 * q + lane] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x < 128) {
-            const int h = threadIdx.x >> 6, c = threadIdx.x & 63;
-            ep.cpart[(size_t)gb.rb * N + gb.cb * 128 + threadIdx.x] = cs[2 * h][c] + cs[2 * h + 1][c];
-        }
-    }
-    }
-}
-
-// The wide products with M % 256 == 0, their B operand staged in LDS: 8
-// waves per workgroup (4 along M x 2 along N, 64 x 64 each), a 256 x 128
-// workgroup tile.  B (the weights, or dZ^T in the weight gradients) passes
+// tiles, so that each operand fragment feeds four MFMAs), RW x 2 waves per
+// workgroup tile of 64 RW x 128 (RW = 4 where that still makes a workgroup
+// per CU, 2 otherwise).  B (the weights, or dZ^T in the weight gradients) passes
 // through LDS in 64-k chunks, double buffered: the workgroup loads a chunk
 // once (128 columns x 128 bytes, by direct memory-to-LDS loads that use no
-// registers) while the current chunk's MFMAs run, where in gemm_wide every
-// pair of waves sharing columns loads it from L2.  A column's eight 16-byte
+// registers) while the current chunk's MFMAs run, instead of every pair of
+// waves sharing its columns loading it from L2.  A column's eight 16-byte
 // pieces sit in its 128-byte LDS row at slots p ^ (c & 7), so that the 16
 // columns a fragment read spans fall on different banks.  A fragments come
-// from memory as there.  Within a chunk, lane group g takes k 16 g + 8 h ..
-// + 7 in half h (the two halves of a 32-byte piece of its row, both in flight
-// in the same chunk), and B the same k; a 32-k chunk (K % 64 remainder) takes
-// 8 g .. 8 g + 7.  The staging buffers and the epilogue tiles share the LDS
-// (78 KB: two workgroups per CU at 128 registers).
+// straight from memory: within a chunk, lane group g takes k 16 g + 8 h .. + 7
+// in half h (the two halves of a 32-byte piece of its row, both in flight in
+// the same chunk, so whole 128-byte lines are consumed at once), and B the
+// same k; a 32-k chunk (K % 64 remainder) takes 8 g .. 8 g + 7.  Scheduling
+// barriers keep the compiler from sinking each load to just before its first
+// MFMA.  The staging buffers and the epilogue tiles share the LDS (39 or 78
+// KB: four or two workgroups per CU at 128 registers).  EPI_STORE with S K
+// slices writes slice z's partial product to plane z of ep.f32.
 typedef __attribute__((address_space(3))) void lds_void;
+template <int NW>
 union GemmLds {
     bf16_t b[2][128][64];  // staged B chunks: column c's 16-byte piece p at slot p ^ (c & 7)
-    TileLds t[8];
+    TileLds t[NW];
 };
-template <int EPI>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void gemm_lds(
+// RW waves along M (4: 256-row tiles of 8 waves; 2: 128-row tiles of 4 waves)
+template <int EPI, int RW>
+__global__ __launch_bounds__(128 * RW) __attribute__((amdgpu_waves_per_eu(4))) void gemm_lds(
     const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ Bt, int ldb, int M, int N, int K, int S,
     EpiArgs ep) {
     static_assert(EPI == EPI_BIAS_GELU || EPI == EPI_DGELU || EPI == EPI_STORE, "no staged BIAS epilogue");
-    __shared__ GemmLds L;
+    static_assert(RW == 2 || RW == 4, "128- or 256-row tiles");
+    constexpr int NB = 8 / RW;  // 8-column blocks of a chunk each wave stages
+    __shared__ GemmLds<2 * RW> L;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const GemmBlock gb = gemm_block((int)blockIdx.x, M / 256, N / 128, S);
-    const int wr = wave & 3, wc = wave >> 2;
-    const int tm = gb.rb * 256 + wr * 64, tn = gb.cb * 128 + wc * 64;
+    const GemmBlock gb = gemm_block((int)blockIdx.x, M / (64 * RW), N / 128, S);
+    const int wr = wave % RW, wc = wave / RW;
+    const int tm = gb.rb * 64 * RW + wr * 64, tn = gb.cb * 128 + wc * 64;
     const int kc = S == 1 ? K : (((K + 63) / 64 + S - 1) / S) * 64;
     const int kb = gb.z * kc, ke = min(K, kb + kc);
     const int r16 = lane & 15, g4 = lane >> 4;
     floatx4 acc[4][4] = {};
     const bf16_t* a = A + (size_t)(tm + r16) * lda;
-    // staging: wave w moves column blocks 2 w and 2 w + 1 (8 columns each) of a
-    // chunk straight from memory into LDS (global_load_lds_dwordx4: lane l's 16
-    // bytes land at the block's base + 16 l, i.e. column l / 8, slot l % 8,
-    // which holds piece (l % 8) ^ (l / 8) of the column)
+    // staging: wave w moves column blocks NB w .. NB w + NB - 1 (8 columns
+    // each) of a chunk straight from memory into LDS (global_load_lds_dwordx4:
+    // lane l's 16 bytes land at the block's base + 16 l, i.e. column l / 8,
+    // slot l % 8, which holds piece (l % 8) ^ (l / 8) of the column)
     const int sp = (lane & 7) ^ (lane >> 3);
-    const bf16_t* bsrc = Bt + (size_t)(gb.cb * 128 + 16 * wave + (lane >> 3)) * ldb;
+    const bf16_t* bsrc = Bt + (size_t)(gb.cb * 128 + 8 * NB * wave + (lane >> 3)) * ldb;
     // whole 64-k chunks, then at most one 32-k chunk (K % 64 remainder),
     // outside the loop so that the accumulators flow through one path
     const int nfull = ke > kb ? (ke - kb) / 64 : 0, nch = ke > kb ? (ke - kb + 63) / 64 : 0;
     auto stage = [&](int c) {
         const int k0 = kb + 64 * c, p = ke - k0 >= 64 ? sp : (sp & 3);  // a 32-k chunk reads pieces 0..3 only
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < NB; ++q)
             __builtin_amdgcn_global_load_lds((const void*)(bsrc + (size_t)8 * q * ldb + k0 + 8 * p),
-                                             (lds_void*)&L.b[c & 1][16 * wave + 8 * q][0], 16, 0, 0);
+                                             (lds_void*)&L.b[c & 1][8 * NB * wave + 8 * q][0], 16, 0, 0);
     };
     if (nch > 0) stage(0);
     __builtin_amdgcn_s_waitcnt(0);
@@ -532,10 +426,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void g
             if (ep.ht) store_ht<64>(ep, tm, tn + 32 * j2, lane, L.t[wave]);
         }
         if (EPI == EPI_DGELU && ep.cpart) {
-            // the bias gradient's share of each 128-row block (waves wr = 0, 1
-            // and 2, 3): each wave's column sums over its 64 rows, then the
+            // the bias gradient's share of each 128-row block (waves wr = 2 hb,
+            // 2 hb + 1): each wave's column sums over its 64 rows, then the
             // two waves of a block and column half in order
-            __shared__ float cs[8][64];
+            __shared__ float cs[2 * RW][64];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float v = csum[q];
@@ -544,9 +438,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void g
                 if (lane < 16) cs[wave][16 * q + lane] = v;
             }
             __syncthreads();
-            if (threadIdx.x < 256) {
-                const int hb = threadIdx.x >> 7, cc = threadIdx.x & 127, w0 = 2 * hb + 4 * (cc >> 6), c = cc & 63;
-                ep.cpart[(size_t)(2 * gb.rb + hb) * N + gb.cb * 128 + cc] = cs[w0][c] + cs[w0 + 1][c];
+            if (threadIdx.x < 64 * RW) {
+                const int hb = threadIdx.x >> 7, cc = threadIdx.x & 127, w0 = 2 * hb + RW * (cc >> 6), c = cc & 63;
+                ep.cpart[(size_t)(gb.rb * (RW / 2) + hb) * N + gb.cb * 128 + cc] = cs[w0][c] + cs[w0 + 1][c];
             }
         }
     }
@@ -560,12 +454,12 @@ int gemm(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, in
         // (measured slower with fewer: batch 16,384 took 0.138 instead of
         // 0.128 ms per step)
         if (N % 128 == 0 && M % 256 == 0 && (M / 256) * (N / 128) >= 256) {
-            gemm_lds<EPI><<<(M / 256) * (N / 128), 512, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
+            gemm_lds<EPI, 4><<<(M / 256) * (N / 128), 512, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
             CT2_LAUNCH_CHECK();
             return CT2_OK;
         }
         if (N % 128 == 0) {
-            gemm_wide<EPI><<<(M / 128) * (N / 128), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
+            gemm_lds<EPI, 2><<<(M / 128) * (N / 128), 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, 1, ep);
             CT2_LAUNCH_CHECK();
             return CT2_OK;
         }
@@ -621,7 +515,7 @@ int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int 
         const int tiles = (M / 256) * (N / 128);
         const int S = std::max(1, std::min({smax, K / 256, (256 + tiles - 1) / tiles}));
         EpiArgs ep{S > 1 ? scratch : out, N, nullptr, nullptr, 0, nullptr, nullptr};
-        gemm_lds<EPI_STORE><<<tiles * S, 512, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
+        gemm_lds<EPI_STORE, 4><<<tiles * S, 512, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
         CT2_LAUNCH_CHECK();
         if (S > 1) {
             splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
@@ -633,7 +527,7 @@ int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int 
         const int tiles = (M / 128) * (N / 128);
         const int S = std::max(1, std::min({smax, K / 256, (512 + tiles - 1) / tiles}));
         EpiArgs ep{S > 1 ? scratch : out, N, nullptr, nullptr, 0, nullptr, nullptr};
-        gemm_wide<EPI_STORE><<<tiles * S, 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
+        gemm_lds<EPI_STORE, 2><<<tiles * S, 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
         CT2_LAUNCH_CHECK();
         if (S > 1) {
             splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
