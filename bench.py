@@ -561,7 +561,8 @@ def main() -> int:
         return 0
 
     # the host boundary: the container upload (25 B per particle, pageable host
-    # arrays) and the force download (16 B per particle), synchronous, wall clock
+    # arrays) and the force download (16 B per particle, into new arrays),
+    # synchronous, wall clock
     zf, zr = np.zeros(n, np.float32), np.zeros(n, np.int8)
     t_up = []
     for _ in range(3):
@@ -573,8 +574,28 @@ def main() -> int:
         t1 = time.perf_counter()
         api.forces_download()
         t_down.append((time.perf_counter() - t1) * 1e3)
+    # the same copies from and into page-locked arrays (ct2_host_alloc, as the
+    # ct2 binding keeps its host arrays)
+    pin = [api.pinned_empty(n, dt) for dt in (np.uint32, np.uint32, np.uint32, np.float32, np.float32, np.float32,
+                                             np.int8)]
+    for dst, src in zip(pin, (x, y, z, zf, zf, zf, zr)):
+        dst[:] = src
+    t_upp = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        api.particles_upload(*pin)
+        t_upp.append((time.perf_counter() - t1) * 1e3)
+    fout = [api.pinned_empty(n, np.float32) for _ in range(4)]
+    t_downp = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        api.forces_download(fout)
+        t_downp.append((time.perf_counter() - t1) * 1e3)
+    del pin, fout
     host_io = {"upload_ms": round(float(np.median(t_up)), 3), "upload_bytes": 25 * n,
-               "download_forces_ms": round(float(np.median(t_down)), 3), "download_bytes": 16 * n}
+               "download_forces_ms": round(float(np.median(t_down)), 3), "download_bytes": 16 * n,
+               "upload_pinned_ms": round(float(np.median(t_upp)), 3),
+               "download_forces_pinned_ms": round(float(np.median(t_downp)), 3)}
 
     # drift: 36 B/particle
     drift_ms = time_calls(ev, lambda: api.drift(1e-9, 1.0), 20)

@@ -83,6 +83,8 @@ _SIGS = {
     "ct2_particles_upload": ([_P] * 7, C.c_int),
     "ct2_particles_download": ([_P] * 7, C.c_int),
     "ct2_particles_device_ptrs": ([C.POINTER(ParticlePtrs)], C.c_int),
+    "ct2_host_alloc": ([C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
+    "ct2_host_free": ([C.c_void_p], C.c_int),
     "ct2_kick": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_kick_noalloc": ([C.POINTER(KickParams), C.c_uint, C.POINTER(C.c_int)], C.c_int),
     "ct2_mm_init": ([C.c_int, C.c_int], C.c_int),

@@ -66,6 +66,23 @@ def particle_count() -> int:
     return int(L.load().ct2_particle_count())
 
 
+def pinned_empty(n: int, dtype) -> np.ndarray:
+    """An uninitialised numpy array of n elements in page-locked host memory
+    (ct2_host_alloc), freed with ct2_host_free when the array is collected."""
+    import weakref
+
+    dt = np.dtype(dtype)
+    nbytes = int(n) * dt.itemsize
+    ptr = C.c_void_p()
+    L.check(L.load().ct2_host_alloc(nbytes, C.byref(ptr)))
+    if not ptr.value:
+        return np.empty(n, dt)
+    buf = (C.c_char * nbytes).from_address(ptr.value)
+    a = np.frombuffer(buf, dtype=dt, count=n)
+    weakref.finalize(buf, L.load().ct2_host_free, ptr.value)
+    return a
+
+
 def particles_upload(x=None, y=None, z=None, vx=None, vy=None, vz=None, rung=None) -> None:
     n = particle_count()
     arrs = []
@@ -129,9 +146,14 @@ def kick_noalloc(params: KickParams | None = None, flags: int = 0) -> int:
 NOALLOC_MAX_SINKS = 1 << 18
 
 
-def forces_download():
+def forces_download(out=None):
+    """The stored force and potential, into `out` (four float32 arrays of
+    the particle count, e.g. from pinned_empty) or new arrays."""
     n = particle_count()
-    out = [np.empty(n, np.float32) for _ in range(4)]
+    if out is None:
+        out = [np.empty(n, np.float32) for _ in range(4)]
+    elif len(out) != 4 or any(a.dtype != np.float32 or a.size != n or not a.flags.c_contiguous for a in out):
+        raise ValueError("forces_download needs four contiguous float32 arrays of the particle count")
     L.check(L.load().ct2_forces_download(*[_p(a) for a in out]))
     return tuple(out)
 

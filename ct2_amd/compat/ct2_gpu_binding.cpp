@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -19,7 +20,7 @@ char* particle_rung_ptr = nullptr;
 
 namespace {
 size_t g_count = 0;
-std::vector<char> g_store;
+char* g_store = nullptr;  // the host arrays, one page-locked block (ct2_host_alloc)
 float g_eta = 0.15f;
 int g_ppcell = 256, g_bwidth = 3;  // options.cpp:61,54
 int g_ncell = 0;                   // the last particle_to_grid
@@ -101,9 +102,16 @@ void ct2_gpu_trace(std::FILE* f) { g_trace = f; }
 void particle_allocate(size_t count) {
     ensure_init();
     g_count = count;
-    // one block, sizeof(T) * count per array (particle.cpp:275-283 intent)
-    g_store.assign(count * (3 * sizeof(ct2_pos_fixed) + 3 * sizeof(float) + 1), 0);
-    char* p = g_store.data();
+    // one block, sizeof(T) * count per array (particle.cpp:275-283 intent),
+    // page-locked so that the per-call copies run at the host link's rate
+    const size_t bytes = count * (3 * sizeof(ct2_pos_fixed) + 3 * sizeof(float) + 1);
+    check(ct2_host_free(g_store), "ct2_host_free");
+    g_store = nullptr;
+    void* block = nullptr;
+    check(ct2_host_alloc(bytes, &block), "ct2_host_alloc");
+    g_store = (char*)block;
+    if (bytes) std::memset(g_store, 0, bytes);
+    char* p = g_store;
     particle_x_ptr = (ct2_pos_fixed*)p;
     particle_y_ptr = particle_x_ptr + count;
     particle_z_ptr = particle_y_ptr + count;

@@ -217,6 +217,24 @@ int ct2_particles_download(uint32_t* x, uint32_t* y, uint32_t* z, float* vx, flo
     return CT2_OK;
 }
 
+int ct2_host_alloc(size_t bytes, void** out) {
+    if (!out) return set_error(CT2_ERR_INVALID, "null output");
+    *out = nullptr;
+    if (bytes == 0) return CT2_OK;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return set_error(CT2_ERR_NOMEM, "pinned host allocation of %zu bytes failed", bytes);
+    }
+    *out = p;
+    return CT2_OK;
+}
+
+int ct2_host_free(void* p) {
+    if (p) CT2_HIP(hipHostFree(p));
+    return CT2_OK;
+}
+
 int ct2_particles_device_ptrs(ct2_particle_ptrs* out) {
     CT2_TRY(require_particles());
     if (!out) return set_error(CT2_ERR_INVALID, "null output");

@@ -75,6 +75,14 @@ int ct2_particles_upload(const uint32_t* x, const uint32_t* y, const uint32_t* z
 int ct2_particles_download(uint32_t* x, uint32_t* y, uint32_t* z, float* vx, float* vy,
                            float* vz, int8_t* rung);
 
+/* Page-locked host memory for the host side of those copies (no counterpart
+ * in the reference, whose arrays are ordinary heap blocks): copies from and
+ * into it run at the host link's rate, without the driver's staging of
+ * pageable memory.  ct2_host_alloc sets *out (NULL for 0 bytes) and returns
+ * CT2_ERR_NOMEM when the allocation fails; ct2_host_free takes NULL. */
+int ct2_host_alloc(size_t bytes, void** out);
+int ct2_host_free(void* p);
+
 /* Device pointers of the container (particle_x_ptr ... particle_rung_ptr,
  * particle.hpp:22-28) for callers with their own HIP code. */
 typedef struct {

@@ -48,3 +48,13 @@ def test_mm_cells_for_is_host_side():
     nc = C.c_int(0)
     assert L.load().ct2_mm_cells_for(1000, 0, C.byref(nc)) == L.ERR_INVALID
     assert L.load().ct2_mm_cells_for(1000, 8, None) == L.ERR_INVALID
+
+
+def test_host_alloc_argument_checks_are_host_side():
+    """ct2_host_alloc: a null output is rejected and 0 bytes give NULL
+    without touching the runtime; ct2_host_free takes NULL."""
+    lib = L.load()
+    assert lib.ct2_host_alloc(16, None) == L.ERR_INVALID
+    p = C.c_void_p(1234)
+    assert lib.ct2_host_alloc(0, C.byref(p)) == L.OK and not p.value
+    assert lib.ct2_host_free(None) == L.OK

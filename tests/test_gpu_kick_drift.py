@@ -51,6 +51,31 @@ def lattice(ct2, n, sigma, seed):
     return ct2.ic_lattice(n, sigma, seed)
 
 
+def test_pinned_host_arrays_round_trip(ct2):
+    """The container and the stored forces through page-locked host arrays
+    (ct2_host_alloc): an upload and download give the data back bit for bit,
+    and the force download into them equals the one into new arrays."""
+    n = 4099
+    rng = np.random.default_rng(17)
+    src = [rng.integers(0, 2 ** 32, n, dtype=np.uint32) for _ in range(3)]
+    src += [rng.standard_normal(n).astype(np.float32) for _ in range(3)]
+    src.append(rng.integers(0, 8, n).astype(np.int8))
+    pin = [ct2.pinned_empty(n, a.dtype) for a in src]
+    for d, a in zip(pin, src):
+        d[:] = a
+    ct2.particle_allocate(n)
+    ct2.particles_upload(*pin)
+    back = ct2.particles_download()
+    for a, b in zip(src, (back.x, back.y, back.z, back.vx, back.vy, back.vz, back.rung)):
+        assert np.array_equal(a, b)
+    ct2.set_force_options(1e-3, float(np.float32(1.0 / n)), ct2.KERNEL_PLUMMER)
+    ct2.kick(ct2.kick_params(rung=0, t0=0.0), ct2.KICK_STORE)
+    want = ct2.forces_download()
+    got = ct2.forces_download([ct2.pinned_empty(n, np.float32) for _ in range(4)])
+    for a, b in zip(want, got):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("kernel", [0, 1])
 @pytest.mark.parametrize("side", [16, 20])
 def test_kick_all_sinks_vs_fp64(ct2, orc, kernel, side):
