@@ -469,18 +469,22 @@ int gemm(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int ldb, in
     return CT2_OK;
 }
 
-// out[i] = sum over z of part[z][i] (len % 4 == 0) in a fixed order: eight
-// thread groups each sum every eighth plane (z = g, g + 8, ...) for 128
-// consecutive entries (a float4 per thread), then group 0 adds the eight
-// partial sums in order
+// out[i] = sum over z of part[z][i] (len % 4 == 0) in a fixed order: 32
+// thread groups each sum every 32nd plane (z = g, g + 32, ...) for 32
+// consecutive entries (a float4 per thread), then group 0 adds the 32 partial
+// sums in order.  Thirty-two groups of 8 threads rather than 8 of 32: the
+// outputs of the weight gradients are small (8,192 to 65,536 entries), and the
+// planes, not the entries, give the parallelism (256 slices of the layer-3
+// gradient took 10 us with 8 groups).
+constexpr int SK_GROUPS = 32, SK_ENTRIES = 256 / SK_GROUPS;
 __global__ __launch_bounds__(256) void splitk_reduce(const float* __restrict__ part, int S, int len,
                                                      float* __restrict__ out) {
-    __shared__ float4 red[8][32];
-    const int g = threadIdx.x >> 5, e = threadIdx.x & 31;
-    const int i = (blockIdx.x * 32 + e) * 4;
+    __shared__ float4 red[SK_GROUPS][SK_ENTRIES];
+    const int g = threadIdx.x / SK_ENTRIES, e = threadIdx.x % SK_ENTRIES;
+    const int i = (blockIdx.x * SK_ENTRIES + e) * 4;
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < len)
-        for (int z = g; z < S; z += 8) {
+        for (int z = g; z < S; z += SK_GROUPS) {
             const float4 v = *(const float4*)(part + (size_t)z * len + i);
             s.x += v.x;
             s.y += v.y;
@@ -492,7 +496,7 @@ __global__ __launch_bounds__(256) void splitk_reduce(const float* __restrict__ p
     if (g == 0 && i < len) {
         float4 t = red[0][e];
 #pragma unroll
-        for (int q = 1; q < 8; ++q) {
+        for (int q = 1; q < SK_GROUPS; ++q) {
             t.x += red[q][e].x;
             t.y += red[q][e].y;
             t.z += red[q][e].z;
@@ -518,7 +522,7 @@ int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int 
         gemm_lds<EPI_STORE, 4><<<tiles * S, 512, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
         CT2_LAUNCH_CHECK();
         if (S > 1) {
-            splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
+            splitk_reduce<<<(M * N + 4 * SK_ENTRIES - 1) / (4 * SK_ENTRIES), 256, 0, st>>>(scratch, S, M * N, out);
             CT2_LAUNCH_CHECK();
         }
         return CT2_OK;
@@ -530,7 +534,7 @@ int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int 
         gemm_lds<EPI_STORE, 2><<<tiles * S, 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
         CT2_LAUNCH_CHECK();
         if (S > 1) {
-            splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
+            splitk_reduce<<<(M * N + 4 * SK_ENTRIES - 1) / (4 * SK_ENTRIES), 256, 0, st>>>(scratch, S, M * N, out);
             CT2_LAUNCH_CHECK();
         }
         return CT2_OK;
@@ -541,7 +545,7 @@ int gemm_splitk(hipStream_t st, const bf16_t* A, int lda, const bf16_t* Bt, int 
     gemm_bf16<EPI_STORE><<<(M / 128) * (N / 32) * S, 256, 0, st>>>(A, lda, Bt, ldb, M, N, K, S, ep);
     CT2_LAUNCH_CHECK();
     if (S > 1) {
-        splitk_reduce<<<(M * N + 127) / 128, 256, 0, st>>>(scratch, S, M * N, out);
+        splitk_reduce<<<(M * N + 4 * SK_ENTRIES - 1) / (4 * SK_ENTRIES), 256, 0, st>>>(scratch, S, M * N, out);
         CT2_LAUNCH_CHECK();
     }
     return CT2_OK;
