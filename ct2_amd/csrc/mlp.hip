@@ -717,10 +717,14 @@ __global__ __launch_bounds__(BATCH_THREADS) void batch_kernel(const bf16_t* __re
         const uint4 v = *(const uint4*)(DX + (size_t)p * KIN + 8 * piece);
         *(uint4*)(X + (size_t)r * KIN + 8 * piece) = v;
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        // row q's 8-row chunk sits at chunk (q / 8) ^ (piece % 8) of the
+        // feature's LDS row, so that the 16 pieces a write spans fall on 8 bank
+        // groups instead of 2
+        const int col = (((q >> 3) ^ (piece & 7)) << 3) | (q & 7);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            tile[8 * piece + 2 * j][q] = (bf16_t)(w[j] & 0xffffu);
-            tile[8 * piece + 2 * j + 1][q] = (bf16_t)(w[j] >> 16);
+            tile[8 * piece + 2 * j][col] = (bf16_t)(w[j] & 0xffffu);
+            tile[8 * piece + 2 * j + 1][col] = (bf16_t)(w[j] >> 16);
         }
         if (piece == 0) *(float4*)(T + 4 * (size_t)r) = *(const float4*)(DT + 4 * (size_t)p);
     }
@@ -729,7 +733,7 @@ __global__ __launch_bounds__(BATCH_THREADS) void batch_kernel(const bf16_t* __re
 #pragma unroll
     for (int e = threadIdx.x; e < KIN * BATCH_ROWS / 8; e += BATCH_THREADS) {
         const int f = e >> 3, part = e & 7;
-        *(uint4*)(XT + (size_t)f * M + rb + 8 * part) = *(const uint4*)&tile[f][8 * part];
+        *(uint4*)(XT + (size_t)f * M + rb + 8 * part) = *(const uint4*)&tile[f][8 * (part ^ ((f >> 3) & 7))];
     }
 }
 
